@@ -1,0 +1,189 @@
+/*
+ * relearn_bh.h -- C ABI of the MI355X (gfx950) Barnes-Hut connectivity update.
+ *
+ * Drop-in boundary for RELeARN's `--algorithm barnes-hut` on one rank.  Each entry point replaces
+ * a member of the reference's Algorithm interface (source/algorithm/Algorithm.h:48-197, paths relative
+ * to the reference's relearn/ directory) as implemented by BarnesHut
+ * (source/algorithm/BarnesHutInternal/BarnesHut.{h,cpp}) on top of OctreeAlgorithm and
+ * ForwardCPUAlgorithm; the citation on each declaration names the member it stands in for.
+ *
+ * Conventions: plain C types, no exceptions.  Every function returns RL_OK (0) or a negative
+ * RL_E* status; rl_bh_last_error() returns a message for the last failure on the context.  A
+ * context owns all its device memory and its HIP stream; calls on one context must not overlap.
+ * Host arrays are caller owned and read (or written) only during the call.
+ */
+#ifndef RELEARN_BH_H
+#define RELEARN_BH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RL_OK = 0,
+    RL_EINVAL = -1,    /* bad argument: sizes, theta outside (0, 0.5], sigma <= 0, state order */
+    RL_EHIP = -2,      /* a HIP runtime call failed */
+    RL_EOOM = -3,      /* device allocation failed */
+    RL_EOUTSIDE = -4,  /* a position lies outside the simulation box */
+    RL_EDUPPOS = -5,   /* two neurons share a position, or are closer than 42 octree levels resolve */
+    RL_ESTATE = -6,    /* call out of order (e.g. search before prepare) */
+    RL_ELIMIT = -7,    /* more than 2^28 neurons or tree nodes */
+};
+
+enum {
+    RL_PREC_FP64 = 0,        /* all arithmetic in IEEE binary64, same operation order as the CPU path */
+    RL_PREC_FP32_MIXED = 1,  /* FP32 distances and Gaussian; FP64 centroids, sums and prefix sums */
+};
+
+enum {
+    RL_SIGNAL_EXCITATORY = 0,
+    RL_SIGNAL_INHIBITORY = 1,
+};
+
+typedef struct rl_bh_ctx rl_bh_ctx;
+
+typedef struct {
+    double box_min[3];   /* simulation box (bounding_box of BarnesHut's constructor) */
+    double box_max[3];
+    double theta;        /* acceptance criterion, 0 < theta <= 0.5 */
+    double sigma;        /* Gaussian kernel sigma, > 0 */
+    double mu;           /* Gaussian kernel mu */
+    uint64_t seed;       /* seed of the counter-based draw stream */
+    int32_t precision;   /* RL_PREC_* */
+    int32_t device;      /* HIP device ordinal */
+    int32_t flags;       /* reserved, 0 */
+    int32_t reserved;
+} rl_bh_config;
+
+typedef struct {
+    uint64_t searches;          /* find_target_neuron calls (vacant axons searched until the first failure) */
+    uint64_t requests;          /* searches that found a target (synapse creation requests) */
+    uint64_t created;           /* requests that were accepted by the commit */
+    uint64_t node_tests;        /* acceptance tests executed on the device */
+    uint64_t node_tests_ref;    /* acceptance tests the CPU path performs for the same searches */
+    double ms_prepare;          /* leaf fill + upward pass */
+    double ms_search;           /* target search */
+    double ms_commit;           /* conflict resolution + vacancy commit */
+} rl_bh_step_stats;
+
+/* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a
+ * GaussianDistributionKernel(mu, sigma) (source/algorithm/Kernel/Gaussian.h:47-52). */
+int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out);
+
+/* Destroys the context and frees its device memory. */
+int rl_bh_destroy(rl_bh_ctx* ctx);
+
+/* Message of the last failing call on ctx (or of rl_bh_create when ctx is NULL). */
+const char* rl_bh_last_error(const rl_bh_ctx* ctx);
+
+/* Algorithm::init(number_neurons) (Algorithm.h:111) = OctreeAlgorithm::init
+ * (source/algorithm/Internal/OctreeAlgorithm.h:58-67): builds the octree of the n neurons.
+ * xyz: n x 3 AoS positions (NeuronsExtraInfo positions); signal: n signal types of the axons
+ * (SynapticElements::get_signal_types), RL_SIGNAL_*. */
+int rl_bh_init(rl_bh_ctx* ctx, uint64_t n, const double* xyz, const uint8_t* signal);
+
+/* Algorithm::disable_neurons / NeuronsExtraInfo update flags: enabled[i] = 0 disables neuron i
+ * (its leaf gets zero vacancies and it searches no targets).  NULL enables all. */
+int rl_bh_set_enabled(rl_bh_ctx* ctx, const uint8_t* enabled);
+
+/* Algorithm::prepare_update_connectivity(signal_types, vacant_axons, vacant_excitatory_dendrites,
+ * vacant_inhibitory_dendrites) (Algorithm.h:136-140; OctreeAlgorithm::update_tree,
+ * OctreeAlgorithm.h:100-112): uploads the vacancy spans, fills the leaves and runs the upward pass. */
+int rl_bh_prepare_update_connectivity(rl_bh_ctx* ctx, const uint32_t* vacant_axons,
+                                      const uint32_t* vacant_excitatory_dendrites,
+                                      const uint32_t* vacant_inhibitory_dendrites);
+
+/* Same as rl_bh_prepare_update_connectivity on the vacancies already resident on the device
+ * (those of the last upload or commit). */
+int rl_bh_prepare_resident(rl_bh_ctx* ctx);
+
+/* BarnesHut::find_target_neurons (BarnesHut.cpp:36-94): one search per vacant axon of every enabled
+ * neuron, stopping at a neuron's first failure.  step is the plasticity-update counter that
+ * addresses the draw stream.  Returns the number of requests in *num_requests (may be NULL). */
+int rl_bh_find_target_neurons(rl_bh_ctx* ctx, uint32_t step, uint64_t* num_requests);
+
+/* Copies the requests of the last search, ordered by (source neuron, axon ordinal j), i.e. the
+ * order a single-threaded CPU run appends them (SynapseCreationRequest{target, source, signal}). */
+int rl_bh_get_requests(rl_bh_ctx* ctx, uint64_t capacity, uint32_t* target, uint32_t* source,
+                       uint8_t* signal, uint32_t* axon_ordinal);
+
+/* Per-slot diagnostics of the last search: slots are laid out by neuron id, vacant_axons[i] slots
+ * per neuron starting at the exclusive prefix sum; node_tests[slot] counts the acceptance tests the
+ * CPU path performs for that search, target[slot] the found target (0xFFFFFFFF when none). */
+int rl_bh_get_search_slots(rl_bh_ctx* ctx, uint64_t capacity, uint32_t* target, uint32_t* node_tests,
+                           uint32_t* count_per_neuron);
+
+/* ForwardConnector::process_requests + process_responses (source/algorithm/Connector.cpp:33-197):
+ * resolves conflicts in the order given by `order` (a permutation of the request indices, the
+ * shuffled order of Connector.cpp:71) or, when order is NULL, by a counter-based priority
+ * (seed, source, step, j); commits the vacancies on the device.  *created gets the number of synapses. */
+int rl_bh_commit(rl_bh_ctx* ctx, uint32_t step, const uint64_t* order, uint64_t* created);
+
+/* Accept flags of the last commit (1 = SynapseCreationResponse::Succeeded), request order. */
+int rl_bh_get_commit_result(rl_bh_ctx* ctx, uint64_t capacity, uint8_t* ok);
+
+/* Algorithm::update_connectivity(number_neurons) (Algorithm.h:149) for the forward algorithm
+ * (source/algorithm/Internal/ExchangingAlgorithm.h:58-87): prepare (resident vacancies) + search +
+ * commit with counter priorities.  stats may be NULL. */
+int rl_bh_update_connectivity(rl_bh_ctx* ctx, uint32_t step, rl_bh_step_stats* stats);
+
+/* Copies the device-resident vacancies (axon, excitatory dendrite, inhibitory dendrite). */
+int rl_bh_get_vacancies(rl_bh_ctx* ctx, uint32_t* vacant_axons, uint32_t* vacant_excitatory_dendrites,
+                        uint32_t* vacant_inhibitory_dendrites);
+
+/* Number of octree nodes and the depth (deepest level, root = 0) after rl_bh_init. */
+int rl_bh_tree_info(rl_bh_ctx* ctx, uint64_t* num_nodes, uint32_t* depth);
+
+/* Octree export in breadth-first order (level, then Morton order), the layout of the device tree.
+ * Any pointer may be NULL.  pos arrays are 3 per node; has_* = 1 when the node has a position for
+ * that dendrite type (leaves always, inner nodes when the count is non-zero). */
+int rl_bh_export_tree(rl_bh_ctx* ctx, uint8_t* level, uint8_t* nchild, uint32_t* link, uint8_t* octant,
+                      double* box_lo, double* box_hi, double* side, uint32_t* cnt_exc, uint32_t* cnt_inh,
+                      uint8_t* has_exc, uint8_t* has_inh, double* pos_exc, double* pos_inh);
+
+/* Milliseconds of the last search kernel(s), measured with HIP events on the context's stream. */
+int rl_bh_last_search_ms(rl_bh_ctx* ctx, double* ms);
+
+/* ------------------------------------------------------------------ neuron step */
+
+typedef struct {
+    double a, b, c, d;       /* Izhikevich parameters (models/izhikevich/Parameters.h:41) */
+    double v_spike;
+    double k1, k2, k3;
+    uint32_t h;              /* integration substeps (NeuronModelBase.h:241) */
+    uint32_t input_mode;     /* 0: per-step input array supplied by the caller, 1: counter-based Normal */
+    double input_mean, input_stddev;   /* background input N(mean, stddev) for input_mode 1 */
+    double tau_ca, beta_ca;  /* calcium decay and per-spike increment */
+    double eta[3];           /* minimum calcium to grow: axon, excitatory, inhibitory dendrite */
+    double nu[3];            /* growth rates */
+} rl_step_params;
+
+/* Allocates and uploads the per-neuron state of the step kernel (IzhikevichModel x/u, calcium,
+ * target calcium, grown elements and deltas; 3 element types type-major).  NULL arrays start at 0
+ * (target_ca must be given). */
+int rl_neurons_init(rl_bh_ctx* ctx, const rl_step_params* p, const double* x, const double* u,
+                    const double* ca, const double* target_ca, const double* grown,
+                    const uint32_t* connected);
+
+/* One simulation step of NeuronModel::update_electrical_activity + calcium + synaptic element
+ * deltas (Simulation.cpp:443-453): Izhikevich (IzhikevichModelCPU.cpp:75-125), calcium
+ * (CalciumCalculatorCPU.cpp:40-103), growth curve (SynapticElementsCPU.cpp:25-49).  input: n values
+ * for input_mode 0, else NULL.  step addresses the counter-based input draws. */
+int rl_neurons_step(rl_bh_ctx* ctx, uint32_t step, const double* input);
+
+/* ElementBaseBase::update (ElementBaseBase.h:420-487) for the three element types: applies the
+ * deltas to grown/vacant/connected counts at a plasticity step.  deleted (3*n, may be NULL) gets the
+ * number of bound elements each neuron must delete per type. */
+int rl_neurons_commit_elements(rl_bh_ctx* ctx, uint32_t* deleted);
+
+/* Copies the neuron state: any pointer may be NULL; delta/grown are 3*n (type-major). */
+int rl_neurons_get(rl_bh_ctx* ctx, double* x, double* u, double* ca, uint8_t* fired, double* delta,
+                   double* grown, uint32_t* connected);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RELEARN_BH_H */

@@ -1,0 +1,494 @@
+// C ABI of librelearn_bh.so: context management and the Algorithm-shaped entry points.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "rl_internal.h"
+
+namespace rl {
+
+int fail(rl_bh_ctx* c, int code, const std::string& msg) {
+    if (c != nullptr) {
+        c->st.code = code;
+        c->st.msg = msg;
+    }
+    return code;
+}
+
+int hip_fail(rl_bh_ctx* c, hipError_t e, const char* what) {
+    const int code = (e == hipErrorOutOfMemory) ? RL_EOOM : RL_EHIP;
+    return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensure_tmp(rl_bh_ctx* c, size_t bytes) {
+    if (bytes <= c->tmp_bytes && c->tmp != nullptr) {
+        return RL_OK;
+    }
+    if (c->tmp != nullptr) {
+        (void)hipFree(c->tmp);
+        c->tmp = nullptr;
+    }
+    const size_t want = std::max<size_t>(bytes, 1 << 20);
+    hipError_t e = hipMalloc(&c->tmp, want);
+    if (e != hipSuccess) {
+        c->tmp = nullptr;
+        c->tmp_bytes = 0;
+        return fail(c, RL_EOOM, "scratch allocation failed");
+    }
+    c->tmp_bytes = want;
+    return RL_OK;
+}
+
+namespace {
+
+thread_local std::string g_create_error;
+
+void free_all(rl_bh_ctx* c) {
+    free_tree(c);
+    neurons_free(c);
+    dfree(&c->pos);
+    dfree(&c->qpos);
+    dfree(&c->signal);
+    dfree(&c->enabled);
+    for (int t = 0; t < 3; t++) {
+        dfree(&c->vac[t]);
+        dfree(&c->conn[t]);
+    }
+    dfree(&c->leaf_of);
+    dfree(&c->order);
+    dfree(&c->slot_off);
+    dfree(&c->slot_target);
+    dfree(&c->slot_tests);
+    dfree(&c->found);
+    dfree(&c->searched);
+    dfree(&c->req_target);
+    dfree(&c->req_source);
+    dfree(&c->req_signal);
+    dfree(&c->req_j);
+    dfree(&c->req_ok);
+    if (c->tmp != nullptr) {
+        (void)hipFree(c->tmp);
+        c->tmp = nullptr;
+        c->tmp_bytes = 0;
+    }
+    c->slots_cap = 0;
+    c->req_cap = 0;
+}
+
+int set_device(rl_bh_ctx* c) {
+    RL_HIP(c, hipSetDevice(c->cfg.device));
+    return RL_OK;
+}
+
+}  // namespace
+}  // namespace rl
+
+using namespace rl;
+
+extern "C" {
+
+int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
+    g_create_error.clear();
+    if (out == nullptr || cfg == nullptr) {
+        g_create_error = "rl_bh_create: null argument";
+        return RL_EINVAL;
+    }
+    *out = nullptr;
+    // BarnesHut::BarnesHut (BarnesHut.h:55-60) and test_acceptance_criterion's theta range
+    // (BarnesHutBase.h:74-77); GaussianDistributionKernel (Gaussian.h:50) requires sigma > 0.
+    if (!(cfg->theta > 0.0) || !(cfg->theta <= 0.5) || !(cfg->sigma > 0.0) || !std::isfinite(cfg->mu)) {
+        g_create_error = "rl_bh_create: theta must be in (0, 0.5] and sigma > 0";
+        return RL_EINVAL;
+    }
+    if (cfg->precision != RL_PREC_FP64 && cfg->precision != RL_PREC_FP32_MIXED) {
+        g_create_error = "rl_bh_create: unknown precision";
+        return RL_EINVAL;
+    }
+    const double big = std::numeric_limits<double>::max() / 4.0;
+    for (int d = 0; d < 3; d++) {
+        if (!(cfg->box_min[d] <= cfg->box_max[d]) || !(std::fabs(cfg->box_min[d]) < big) ||
+            !(std::fabs(cfg->box_max[d]) < big)) {
+            g_create_error = "rl_bh_create: invalid simulation box";
+            return RL_EINVAL;
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_create_error = "rl_bh_create: no HIP device";
+        return RL_EHIP;
+    }
+    if (cfg->device < 0 || cfg->device >= ndev) {
+        g_create_error = "rl_bh_create: device ordinal out of range";
+        return RL_EINVAL;
+    }
+    auto* c = new rl_bh_ctx();
+    c->cfg = *cfg;
+    if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        g_create_error = "rl_bh_create: stream/event creation failed";
+        delete c;
+        return RL_EHIP;
+    }
+    for (int d = 0; d < 3; d++) {
+        const double ext = cfg->box_max[d] - cfg->box_min[d];
+        c->qscale[d] = ext > 0.0 ? 2147483647.0 / ext : 0.0;
+    }
+    *out = c;
+    return RL_OK;
+}
+
+int rl_bh_destroy(rl_bh_ctx* c) {
+    if (c == nullptr) {
+        return RL_OK;
+    }
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream != nullptr) {
+        (void)hipStreamSynchronize(c->stream);
+    }
+    free_all(c);
+    if (c->ev0 != nullptr) {
+        (void)hipEventDestroy(c->ev0);
+    }
+    if (c->ev1 != nullptr) {
+        (void)hipEventDestroy(c->ev1);
+    }
+    if (c->stream != nullptr) {
+        (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return RL_OK;
+}
+
+const char* rl_bh_last_error(const rl_bh_ctx* c) {
+    if (c == nullptr) {
+        return g_create_error.c_str();
+    }
+    return c->st.msg.c_str();
+}
+
+int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signal) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (n > 0 && (xyz == nullptr || signal == nullptr)) {
+        return fail(c, RL_EINVAL, "rl_bh_init: null positions or signal types");
+    }
+    if (n > kLinkMask) {
+        return fail(c, RL_ELIMIT, "rl_bh_init: more than 2^28 neurons");
+    }
+    RL_TRY(set_device(c));
+    free_all(c);
+    c->n = n;
+    c->prepared = false;
+    c->searched_valid = false;
+    for (uint64_t i = 0; i < n; i++) {
+        for (int d = 0; d < 3; d++) {
+            const double v = xyz[3 * i + d];
+            if (!(c->cfg.box_min[d] <= v && v <= c->cfg.box_max[d])) {
+                return fail(c, RL_EOUTSIDE, "rl_bh_init: position outside the simulation box");
+            }
+        }
+        if (signal[i] > 1) {
+            return fail(c, RL_EINVAL, "rl_bh_init: signal type must be 0 or 1");
+        }
+    }
+    RL_TRY(dalloc(c, &c->signal, n));
+    RL_TRY(dalloc(c, &c->enabled, n));
+    for (int t = 0; t < 3; t++) {
+        RL_TRY(dalloc(c, &c->vac[t], n));
+        RL_TRY(dalloc(c, &c->conn[t], n));
+    }
+    RL_TRY(dalloc(c, &c->slot_off, n + 1));
+    RL_TRY(dalloc(c, &c->found, n));
+    RL_TRY(dalloc(c, &c->searched, n));
+    hipStream_t s = c->stream;
+    if (n > 0) {
+        RL_HIP(c, hipMemcpyAsync(c->signal, signal, n, hipMemcpyHostToDevice, s));
+        RL_HIP(c, hipMemsetAsync(c->enabled, 1, n, s));
+        for (int t = 0; t < 3; t++) {
+            RL_HIP(c, hipMemsetAsync(c->vac[t], 0, n * sizeof(uint32_t), s));
+            RL_HIP(c, hipMemsetAsync(c->conn[t], 0, n * sizeof(uint32_t), s));
+        }
+    }
+    RL_HIP(c, hipStreamSynchronize(s));
+    const int st = build_tree(c, xyz);
+    if (st != RL_OK) {
+        return st;
+    }
+    return RL_OK;
+}
+
+int rl_bh_set_enabled(rl_bh_ctx* c, const uint8_t* enabled) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    if (c->n == 0) {
+        return RL_OK;
+    }
+    if (enabled == nullptr) {
+        RL_HIP(c, hipMemsetAsync(c->enabled, 1, c->n, c->stream));
+    } else {
+        RL_HIP(c, hipMemcpyAsync(c->enabled, enabled, c->n, hipMemcpyHostToDevice, c->stream));
+    }
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    c->prepared = false;
+    return RL_OK;
+}
+
+int rl_bh_prepare_update_connectivity(rl_bh_ctx* c, const uint32_t* vacant_axons, const uint32_t* vacant_exc,
+                                      const uint32_t* vacant_inh) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (c->n > 0 && (vacant_axons == nullptr || vacant_exc == nullptr || vacant_inh == nullptr)) {
+        return fail(c, RL_EINVAL, "prepare_update_connectivity: null vacancy span");
+    }
+    RL_TRY(set_device(c));
+    const uint32_t* h[3] = { vacant_axons, vacant_exc, vacant_inh };
+    for (int t = 0; t < 3 && c->n > 0; t++) {
+        RL_HIP(c, hipMemcpyAsync(c->vac[t], h[t], c->n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    }
+    return rl_bh_prepare_resident(c);
+}
+
+int rl_bh_prepare_resident(rl_bh_ctx* c) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    RL_TRY(prepare_tree(c));
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    c->prepared = true;
+    c->searched_valid = false;
+    return RL_OK;
+}
+
+int rl_bh_find_target_neurons(rl_bh_ctx* c, uint32_t step, uint64_t* num_requests) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    RL_TRY(find_targets(c, step));
+    if (num_requests != nullptr) {
+        *num_requests = c->nreq;
+    }
+    return RL_OK;
+}
+
+int rl_bh_get_requests(rl_bh_ctx* c, uint64_t capacity, uint32_t* target, uint32_t* source, uint8_t* signal,
+                       uint32_t* axon_ordinal) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (!c->searched_valid) {
+        return fail(c, RL_ESTATE, "get_requests: no search result");
+    }
+    if (capacity < c->nreq) {
+        return fail(c, RL_EINVAL, "get_requests: capacity too small");
+    }
+    RL_TRY(set_device(c));
+    const uint64_t m = c->nreq;
+    hipStream_t s = c->stream;
+    if (m > 0) {
+        if (target) RL_HIP(c, hipMemcpyAsync(target, c->req_target, m * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (source) RL_HIP(c, hipMemcpyAsync(source, c->req_source, m * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (signal) RL_HIP(c, hipMemcpyAsync(signal, c->req_signal, m, hipMemcpyDeviceToHost, s));
+        if (axon_ordinal) RL_HIP(c, hipMemcpyAsync(axon_ordinal, c->req_j, m * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    RL_HIP(c, hipStreamSynchronize(s));
+    return RL_OK;
+}
+
+int rl_bh_get_search_slots(rl_bh_ctx* c, uint64_t capacity, uint32_t* target, uint32_t* node_tests,
+                           uint32_t* count_per_neuron) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (!c->searched_valid) {
+        return fail(c, RL_ESTATE, "get_search_slots: no search result");
+    }
+    if (capacity < c->slots) {
+        return fail(c, RL_EINVAL, "get_search_slots: capacity too small");
+    }
+    RL_TRY(set_device(c));
+    hipStream_t s = c->stream;
+    if (c->slots > 0) {
+        if (target) RL_HIP(c, hipMemcpyAsync(target, c->slot_target, c->slots * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (node_tests) RL_HIP(c, hipMemcpyAsync(node_tests, c->slot_tests, c->slots * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    if (count_per_neuron && c->n > 0) {
+        RL_HIP(c, hipMemcpyAsync(count_per_neuron, c->found, c->n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    RL_HIP(c, hipStreamSynchronize(s));
+    return RL_OK;
+}
+
+int rl_bh_commit(rl_bh_ctx* c, uint32_t step, const uint64_t* order, uint64_t* created) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    if (order != nullptr && c->nreq > 0) {
+        std::vector<uint8_t> seen(c->nreq, 0);
+        for (uint64_t k = 0; k < c->nreq; k++) {
+            if (order[k] >= c->nreq || seen[order[k]] != 0) {
+                return fail(c, RL_EINVAL, "commit: order is not a permutation of the request indices");
+            }
+            seen[order[k]] = 1;
+        }
+    }
+    RL_TRY(commit(c, step, order, created));
+    c->prepared = false;  // vacancies changed: the tree must be prepared again before the next search
+    return RL_OK;
+}
+
+int rl_bh_get_commit_result(rl_bh_ctx* c, uint64_t capacity, uint8_t* ok) {
+    if (c == nullptr || (ok == nullptr && c->nreq > 0)) {
+        return RL_EINVAL;
+    }
+    if (capacity < c->nreq) {
+        return fail(c, RL_EINVAL, "get_commit_result: capacity too small");
+    }
+    RL_TRY(set_device(c));
+    if (c->nreq > 0) {
+        RL_HIP(c, hipMemcpyAsync(ok, c->req_ok, c->nreq, hipMemcpyDeviceToHost, c->stream));
+    }
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    return RL_OK;
+}
+
+int rl_bh_update_connectivity(rl_bh_ctx* c, uint32_t step, rl_bh_step_stats* stats) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    hipEvent_t e[4];
+    for (auto& ev : e) {
+        RL_HIP(c, hipEventCreate(&ev));
+    }
+    hipStream_t s = c->stream;
+    int status = RL_OK;
+    uint64_t created = 0;
+    do {
+        if (hipEventRecord(e[0], s) != hipSuccess) {
+            status = fail(c, RL_EHIP, "event record");
+            break;
+        }
+        if ((status = rl_bh_prepare_resident(c)) != RL_OK) break;
+        if (hipEventRecord(e[1], s) != hipSuccess) {
+            status = fail(c, RL_EHIP, "event record");
+            break;
+        }
+        if ((status = find_targets(c, step)) != RL_OK) break;
+        if (hipEventRecord(e[2], s) != hipSuccess) {
+            status = fail(c, RL_EHIP, "event record");
+            break;
+        }
+        if ((status = commit(c, step, nullptr, &created)) != RL_OK) break;
+        c->prepared = false;
+        if (hipEventRecord(e[3], s) != hipSuccess || hipEventSynchronize(e[3]) != hipSuccess) {
+            status = fail(c, RL_EHIP, "event record");
+            break;
+        }
+        if (stats != nullptr) {
+            float t01 = 0, t12 = 0, t23 = 0;
+            (void)hipEventElapsedTime(&t01, e[0], e[1]);
+            (void)hipEventElapsedTime(&t12, e[1], e[2]);
+            (void)hipEventElapsedTime(&t23, e[2], e[3]);
+            stats->searches = c->last_searches;
+            stats->requests = c->nreq;
+            stats->created = created;
+            stats->node_tests = c->last_tests;
+            stats->node_tests_ref = c->last_tests_ref;
+            stats->ms_prepare = t01;
+            stats->ms_search = t12;
+            stats->ms_commit = t23;
+        }
+    } while (false);
+    for (auto& ev : e) {
+        (void)hipEventDestroy(ev);
+    }
+    return status;
+}
+
+int rl_bh_get_vacancies(rl_bh_ctx* c, uint32_t* va, uint32_t* ve, uint32_t* vi) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    uint32_t* h[3] = { va, ve, vi };
+    for (int t = 0; t < 3 && c->n > 0; t++) {
+        if (h[t] != nullptr) {
+            RL_HIP(c, hipMemcpyAsync(h[t], c->vac[t], c->n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    return RL_OK;
+}
+
+int rl_bh_tree_info(rl_bh_ctx* c, uint64_t* num_nodes, uint32_t* depth) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (num_nodes) *num_nodes = c->tree.m;
+    if (depth) *depth = c->tree.depth;
+    return RL_OK;
+}
+
+int rl_bh_export_tree(rl_bh_ctx* c, uint8_t* level, uint8_t* nchild, uint32_t* link, uint8_t* octant, double* box_lo,
+                      double* box_hi, double* side, uint32_t* cnt_exc, uint32_t* cnt_inh, uint8_t* has_exc,
+                      uint8_t* has_inh, double* pos_exc, double* pos_inh) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    return export_tree(c, level, nchild, link, octant, box_lo, box_hi, side, cnt_exc, cnt_inh, has_exc, has_inh,
+                       pos_exc, pos_inh);
+}
+
+int rl_bh_last_search_ms(rl_bh_ctx* c, double* ms) {
+    if (c == nullptr || ms == nullptr) {
+        return RL_EINVAL;
+    }
+    *ms = c->last_search_ms;
+    return RL_OK;
+}
+
+int rl_neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const double* u, const double* ca,
+                    const double* target_ca, const double* grown, const uint32_t* connected) {
+    if (c == nullptr || p == nullptr || (c->n > 0 && target_ca == nullptr) || p->h == 0 || p->input_mode > 1) {
+        return c == nullptr ? RL_EINVAL : fail(c, RL_EINVAL, "rl_neurons_init: bad arguments");
+    }
+    RL_TRY(set_device(c));
+    return neurons_init(c, p, x, u, ca, target_ca, grown, connected);
+}
+
+int rl_neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    return neurons_step(c, step, input);
+}
+
+int rl_neurons_commit_elements(rl_bh_ctx* c, uint32_t* deleted) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    c->prepared = false;
+    return neurons_commit(c, deleted);
+}
+
+int rl_neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
+                   uint32_t* connected) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    return neurons_get(c, x, u, ca, fired, delta, grown, connected);
+}
+
+}  // extern "C"

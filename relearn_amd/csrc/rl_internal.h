@@ -1,0 +1,177 @@
+// Internal declarations shared by the HIP translation units of librelearn_bh.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/relearn_bh.h"
+
+namespace rl {
+
+constexpr uint32_t kMaxLevels = 42;          // octree levels below the root resolved by the 2x63-bit keys
+constexpr uint32_t kLinkBits = 28;           // node / neuron index bits in the packed link word
+constexpr uint32_t kLinkMask = (1U << kLinkBits) - 1U;
+constexpr uint32_t kNoTarget = 0xFFFFFFFFU;
+constexpr int kDrawsPerPass = 8;             // draws resolved per pick traversal
+
+struct Status {
+    int code = RL_OK;
+    std::string msg;
+};
+
+// Device octree in breadth-first order (level-major, Morton order inside a level).  Children of a
+// node are contiguous.  Per dendrite type s (0 excitatory, 1 inhibitory) the search reads
+//   rec64[s][i] = (centroid x, y, z, max edge)          FP64 path, 32 B
+//   rec32[s][i] = (centroid x, y, z as 31-bit fixed point, max edge as float bits)  FP32 path, 16 B
+//   meta[s][i]  = (vacant count, link | nchild << 28)  8 B
+// link is the first child for inner nodes and the neuron id for leaves.
+struct Tree {
+    uint64_t m = 0;
+    uint32_t depth = 0;
+    std::vector<uint64_t> level_start;  // host copy, depth + 2 entries
+    uint8_t* level = nullptr;
+    uint8_t* octant = nullptr;
+    double* box = nullptr;              // 6 per node: lo xyz, hi xyz
+    double4* rec64[2] = { nullptr, nullptr };
+    uint4* rec32[2] = { nullptr, nullptr };
+    uint2* meta[2] = { nullptr, nullptr };
+};
+
+struct Neurons {
+    double* x = nullptr;
+    double* u = nullptr;
+    double* ca = nullptr;
+    double* target_ca = nullptr;
+    double* input = nullptr;
+    uint8_t* fired = nullptr;
+    double* grown = nullptr;      // 3n
+    double* delta = nullptr;      // 3n
+    uint32_t* connected = nullptr;  // 3n
+    uint32_t* deleted = nullptr;  // 3n
+    rl_step_params p{};
+    bool ready = false;
+};
+
+}  // namespace rl
+
+struct rl_bh_ctx {
+    rl_bh_config cfg{};
+    rl::Status st;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_search_ms = 0.0;
+
+    uint64_t n = 0;
+    double* pos = nullptr;          // SoA x[n], y[n], z[n]
+    uint32_t* qpos = nullptr;       // SoA fixed-point positions (FP32 path)
+    uint8_t* signal = nullptr;
+    uint8_t* enabled = nullptr;
+    uint32_t* vac[3] = { nullptr, nullptr, nullptr };  // axon, exc dendrite, inh dendrite
+    uint32_t* conn[3] = { nullptr, nullptr, nullptr };
+    uint32_t* leaf_of = nullptr;    // neuron -> leaf node
+    uint32_t* order = nullptr;      // neurons in Morton order (search order)
+    rl::Tree tree;
+    double qscale[3] = { 0, 0, 0 }; // fixed-point units per micrometre
+    bool prepared = false;
+
+    // search results (slot layout: vacant axons per neuron at the exclusive prefix sum)
+    uint64_t slots = 0;
+    uint64_t slots_cap = 0;
+    uint64_t* slot_off = nullptr;   // n
+    uint32_t* slot_target = nullptr;
+    uint32_t* slot_tests = nullptr;
+    uint32_t* found = nullptr;      // n
+    uint32_t* searched = nullptr;   // n
+    // dense requests
+    uint64_t nreq = 0;
+    uint64_t req_cap = 0;
+    uint32_t* req_target = nullptr;
+    uint32_t* req_source = nullptr;
+    uint8_t* req_signal = nullptr;
+    uint32_t* req_j = nullptr;
+    uint8_t* req_ok = nullptr;
+    bool searched_valid = false;
+    uint64_t last_searches = 0;
+    uint64_t last_tests = 0;
+    uint64_t last_tests_ref = 0;
+
+    // scratch
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    rl::Neurons neurons;
+};
+
+namespace rl {
+
+int fail(rl_bh_ctx* c, int code, const std::string& msg);
+int hip_fail(rl_bh_ctx* c, hipError_t e, const char* what);
+int ensure_tmp(rl_bh_ctx* c, size_t bytes);
+
+template <typename T>
+int dalloc(rl_bh_ctx* c, T** p, size_t count) {
+    if (*p != nullptr) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) {
+        count = 1;
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, RL_EOOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return RL_OK;
+}
+
+template <typename T>
+void dfree(T** p) {
+    if (*p != nullptr) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+}
+
+// tree (rl_tree.hip)
+int build_tree(rl_bh_ctx* c, const double* h_xyz);
+int prepare_tree(rl_bh_ctx* c);
+void free_tree(rl_bh_ctx* c);
+int export_tree(rl_bh_ctx* c, uint8_t* level, uint8_t* nchild, uint32_t* link, uint8_t* octant, double* box_lo,
+                double* box_hi, double* side, uint32_t* cnt_exc, uint32_t* cnt_inh, uint8_t* has_exc, uint8_t* has_inh,
+                double* pos_exc, double* pos_inh);
+
+// search (rl_search.hip)
+int find_targets(rl_bh_ctx* c, uint32_t step);
+
+// commit (rl_commit.hip)
+int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* created);
+
+// neuron step (rl_step.hip)
+int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const double* u, const double* ca,
+                 const double* target_ca, const double* grown, const uint32_t* connected);
+int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input);
+int neurons_commit(rl_bh_ctx* c, uint32_t* deleted);
+int neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
+                uint32_t* connected);
+void neurons_free(rl_bh_ctx* c);
+
+}  // namespace rl
+
+#define RL_HIP(c, expr)                                   \
+    do {                                                  \
+        hipError_t rl_e_ = (expr);                        \
+        if (rl_e_ != hipSuccess) {                        \
+            return rl::hip_fail((c), rl_e_, #expr);       \
+        }                                                 \
+    } while (0)
+
+#define RL_TRY(expr)                \
+    do {                            \
+        int rl_s_ = (expr);         \
+        if (rl_s_ != RL_OK) {       \
+            return rl_s_;           \
+        }                           \
+    } while (0)

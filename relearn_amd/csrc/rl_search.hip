@@ -1,0 +1,610 @@
+// Barnes-Hut target search on the device.
+//
+// One lane per source neuron, sources in Morton order so that the lanes of a wavefront walk nearly
+// the same part of the tree.  For every source the traversal reproduces
+// BarnesHutBase::get_nodes_to_consider (source/algorithm/BarnesHutInternal/BarnesHutBase.h:129-206):
+// a LIFO stack seeded with the children of the subtree root, acceptance test
+// test_acceptance_criterion (:72-114), accepted nodes in stack order.  Children of a node are stored
+// contiguously, so a stack entry is (first child, children left) and popping the last child first
+// visits children in descending octant order exactly as the reference's pointer stack does.  The
+// stack lives in LDS, one column per lane.
+//
+// The candidate list is never stored: pass 1 sums the probabilities (Kernel::create_probability_interval,
+// source/algorithm/Kernel/Kernel.h:90-134, including the n/r fallback when the Gaussian sum
+// underflows to 0), pass 2 walks the same list again and resolves up to eight draws at once with
+// the picker's rule (ProbabilityPicker::pick_target, source/util/ProbabilityPicker.h:46-112): first
+// candidate whose running sum reaches r.  The level-1 list depends only on the source, so all vacant
+// axons of a neuron share it; every axon then descends from its picked node (find_target_neuron,
+// BarnesHutBase.h:223-255) with its own draws, and a neuron stops at its first failed search
+// (find_target_neurons, :283-289).
+//
+// Precision modes:
+//   FP64        every distance with the libstdc++ three-argument hypot, the Gaussian with FP64 exp,
+//               sums in FP64 in list order.
+//   FP32_MIXED  distances from 31-bit fixed-point centroid differences in FP32, Gaussian with FP32
+//               exp2; FP64 sums.  Acceptance decisions within a relative guard band of the threshold
+//               and exact coincidences are re-decided with the FP64 test, so the candidate lists are
+//               those of the FP64 path; a list whose FP32 sum is 0 is redone in FP64.
+#include <hipcub/hipcub.hpp>
+
+#include "rl_device.h"
+#include "rl_internal.h"
+
+namespace rl {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kNone = 0xFFFFFFFFU;
+
+inline unsigned grid_for(uint64_t n, int block = kBlock) {
+    const uint64_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+struct SearchArgs {
+    uint64_t n;
+    uint64_t m;          // tree nodes
+    const uint32_t* order;
+    const double* pos;
+    const uint32_t* qpos;
+    const uint8_t* signal;
+    const uint8_t* enabled;
+    const uint32_t* vac_axon;
+    const uint64_t* slot_off;
+    const double4* rec64[2];
+    const uint4* rec32[2];
+    const uint2* meta[2];
+    double theta;
+    double scale64;      // -1 / sigma^2
+    double mu;
+    float theta2f;       // theta^2
+    float k2f;           // -log2(e) / sigma^2
+    float sigma_inv2f;   // 1 / sigma^2
+    float muf;
+    float inv_q[3];      // micrometres per fixed-point unit
+    float guard;         // relative guard band of the FP32 acceptance test
+    uint64_t seed;
+    uint32_t step;
+    uint32_t* slot_target;
+    uint32_t* slot_tests;
+    uint32_t* found;
+    uint32_t* searched;
+    unsigned long long* counters;  // [0] executed tests, [1] reference tests, [2] searches
+};
+
+struct Lane {
+    double px, py, pz;
+    int32_t qx, qy, qz;
+    uint32_t src;
+    int sig;
+};
+
+enum : int { kDiscard = 0, kAccept = 1, kExpand = 2 };
+enum : int { kGauss = 0, kInverse = 1 };
+
+struct Eval {
+    int status;
+    bool exact;   // d64 valid (FP64 decision)
+    double d64;
+    float d2f;
+};
+
+// Exact FP64 acceptance test (BarnesHutBase.h:72-114).
+__device__ __forceinline__ Eval eval_exact(const SearchArgs& a, const Lane& L, uint32_t node, uint32_t cnt, bool leaf) {
+    Eval e;
+    e.exact = true;
+    e.d2f = 0.0f;
+    const double4 r = a.rec64[L.sig][node];
+    e.d64 = hypot3_exact(r.x - L.px, r.y - L.py, r.z - L.pz);
+    (void)cnt;
+    if (e.d64 == 0.0) {
+        e.status = kDiscard;
+    } else if (leaf) {
+        e.status = kAccept;
+    } else {
+        e.status = (r.w < (a.theta * e.d64)) ? kAccept : kExpand;
+    }
+    return e;
+}
+
+__device__ __forceinline__ Eval eval_mixed(const SearchArgs& a, const Lane& L, uint32_t node, uint32_t cnt, bool leaf) {
+    const uint4 q = a.rec32[L.sig][node];
+    const int32_t ix = static_cast<int32_t>(q.x) - L.qx;
+    const int32_t iy = static_cast<int32_t>(q.y) - L.qy;
+    const int32_t iz = static_cast<int32_t>(q.z) - L.qz;
+    if ((ix | iy | iz) == 0) {
+        return eval_exact(a, L, node, cnt, leaf);  // possible exact coincidence: decide in FP64
+    }
+    const float dx = static_cast<float>(ix) * a.inv_q[0];
+    const float dy = static_cast<float>(iy) * a.inv_q[1];
+    const float dz = static_cast<float>(iz) * a.inv_q[2];
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    Eval e;
+    e.exact = false;
+    e.d64 = 0.0;
+    e.d2f = d2;
+    if (leaf) {
+        e.status = kAccept;
+        return e;
+    }
+    const float side = __uint_as_float(q.w);
+    const float s2 = side * side;
+    const float g = __builtin_fmaf(a.theta2f, d2, -s2);
+    if (fabsf(g) <= a.guard * s2) {
+        return eval_exact(a, L, node, cnt, leaf);
+    }
+    e.status = g > 0.0f ? kAccept : kExpand;
+    return e;
+}
+
+template <bool kMixed>
+__device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e, uint32_t cnt, int kind) {
+#pragma clang fp contract(off)
+    if (kind == kInverse) {
+        return static_cast<double>(cnt) / e.d64;
+    }
+    if (!kMixed || e.exact) {
+        // GaussianDistributionKernel::get_probability (source/algorithm/Kernel/Gaussian.h:62-67) times
+        // the vacant count (Kernel.h:74)
+        const double numerator = (e.d64 - a.mu) * (e.d64 - a.mu);
+        const double exponent = numerator * a.scale64;
+        return (exp(exponent) * 1.0) * static_cast<double>(cnt);
+    }
+    float x;
+    if (a.muf == 0.0f) {
+        x = e.d2f * a.k2f;
+    } else {
+        const float d = sqrtf(e.d2f) - a.muf;
+        x = d * d * a.k2f;
+    }
+    return static_cast<double>(exp2f(x)) * static_cast<double>(cnt);
+}
+
+struct SumVisitor {
+    double sum = 0.0;
+    uint32_t count = 0;
+    __device__ __forceinline__ void operator()(uint32_t, double p) {
+        sum += p;
+        count++;
+    }
+};
+
+struct PickVisitor {
+    double sum = 0.0;
+    double r[kDrawsPerPass];
+    uint32_t pick[kDrawsPerPass];
+    int nd = 0;
+    uint32_t last_positive = kNone;
+    __device__ __forceinline__ void operator()(uint32_t node, double p) {
+        const double s = sum + p;
+        if (p > 0.0) {
+#pragma unroll
+            for (int k = 0; k < kDrawsPerPass; k++) {
+                if (k < nd && pick[k] == kNone && s >= r[k]) {
+                    pick[k] = node;
+                }
+            }
+            last_positive = node;
+        }
+        sum = s;
+    }
+};
+
+// Traversal of the subtree below `root` (get_nodes_to_consider); the visitor sees the accepted nodes
+// in reference order with their probability.  Returns the number of acceptance tests.
+template <bool kMixed, typename V>
+__device__ uint32_t traverse(const SearchArgs& a, const Lane& L, uint32_t root, int kind, uint32_t* stk,
+                             uint32_t stride, V& v) {
+    const uint2* __restrict__ meta = a.meta[L.sig];
+    const uint2 mr = meta[root];
+    if (mr.x == 0) {
+        return 0;
+    }
+    uint32_t tests = 0;
+    const uint32_t rnc = mr.y >> kLinkBits;
+    if (rnc == 0) {
+        tests++;
+        const Eval e = kMixed ? eval_mixed(a, L, root, mr.x, true) : eval_exact(a, L, root, mr.x, true);
+        if (e.status != kDiscard) {
+            const bool self = (mr.y & kLinkMask) == L.src;
+            v(root, self ? 0.0 : probability<kMixed>(a, e, mr.x, kind));
+        }
+        return tests;
+    }
+    int sp = 1;
+    stk[0] = ((mr.y & kLinkMask) << 4U) | rnc;
+    while (sp > 0) {
+        const uint32_t top = stk[(sp - 1) * stride];
+        const uint32_t left = (top & 15U) - 1U;
+        const uint32_t node = (top >> 4U) + left;
+        if (left == 0) {
+            sp--;
+        } else {
+            stk[(sp - 1) * stride] = (top & ~15U) | left;
+        }
+        tests++;
+        const uint2 m = meta[node];
+        if (m.x == 0) {
+            continue;
+        }
+        const uint32_t nc = m.y >> kLinkBits;
+        const bool leaf = nc == 0;
+        const Eval e = kMixed ? eval_mixed(a, L, node, m.x, leaf) : eval_exact(a, L, node, m.x, leaf);
+        if (e.status == kDiscard) {
+            continue;
+        }
+        if (e.status == kAccept) {
+            const bool self = leaf && (m.y & kLinkMask) == L.src;
+            v(node, self ? 0.0 : probability<kMixed>(a, e, m.x, kind));
+            continue;
+        }
+        stk[sp * stride] = ((m.y & kLinkMask) << 4U) | nc;
+        sp++;
+    }
+    return tests;
+}
+
+struct ListSum {
+    double sum;
+    uint32_t count;
+    uint32_t tests_ref;    // tests of one reference traversal
+    uint32_t tests_exec;   // tests executed here
+    bool exact;            // list probabilities from the FP64 formulas
+    int kind;
+};
+
+// Pass 1 on a subtree: probability total with the reference's fallbacks.
+template <bool kMixed>
+__device__ ListSum list_sum(const SearchArgs& a, const Lane& L, uint32_t root, uint32_t* stk, uint32_t stride) {
+    ListSum ls;
+    SumVisitor v;
+    ls.tests_ref = traverse<kMixed>(a, L, root, kGauss, stk, stride, v);
+    ls.tests_exec = ls.tests_ref;
+    ls.exact = !kMixed;
+    ls.kind = kGauss;
+    ls.sum = v.sum;
+    ls.count = v.count;
+    if (v.count > 0 && v.sum == 0.0) {
+        if (kMixed) {
+            SumVisitor w;
+            ls.tests_exec += traverse<false>(a, L, root, kGauss, stk, stride, w);
+            ls.exact = true;
+            ls.sum = w.sum;
+        }
+        if (ls.sum == 0.0) {
+            SumVisitor w;
+            ls.tests_exec += traverse<false>(a, L, root, kInverse, stk, stride, w);
+            ls.exact = true;
+            ls.kind = kInverse;
+            ls.sum = w.sum;
+        }
+    }
+    return ls;
+}
+
+template <bool kMixed>
+__device__ uint32_t list_pick(const SearchArgs& a, const Lane& L, uint32_t root, const ListSum& ls, PickVisitor& pv,
+                              uint32_t* stk, uint32_t stride) {
+    uint32_t t;
+    if (ls.exact) {
+        t = traverse<false>(a, L, root, ls.kind, stk, stride, pv);
+    } else {
+        t = traverse<kMixed>(a, L, root, ls.kind, stk, stride, pv);
+    }
+#pragma unroll
+    for (int k = 0; k < kDrawsPerPass; k++) {
+        if (k < pv.nd && pv.pick[k] == kNone) {
+            pv.pick[k] = pv.last_positive;  // r above the recomputed sum: last positive candidate
+        }
+    }
+    return t;
+}
+
+template <bool kMixed>
+__global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= a.n) {
+        return;
+    }
+    uint32_t* stk = lds_stack + threadIdx.x;
+    const uint32_t stride = blockDim.x;
+    const uint32_t src = a.order[t];
+    const uint32_t na = a.vac_axon[src];
+    const uint64_t slot0 = a.slot_off[src];
+    if (a.enabled[src] == 0 || na == 0) {
+        a.found[src] = 0;
+        a.searched[src] = 0;
+        return;
+    }
+    Lane L;
+    L.src = src;
+    L.sig = a.signal[src] == 0 ? 0 : 1;
+    L.px = a.pos[src];
+    L.py = a.pos[a.n + src];
+    L.pz = a.pos[2 * a.n + src];
+    if (kMixed) {
+        L.qx = static_cast<int32_t>(a.qpos[src]);
+        L.qy = static_cast<int32_t>(a.qpos[a.n + src]);
+        L.qz = static_cast<int32_t>(a.qpos[2 * a.n + src]);
+    } else {
+        L.qx = L.qy = L.qz = 0;
+    }
+    uint64_t exec = 0;
+    uint64_t ref = 0;
+    const uint2 mroot = a.meta[L.sig][0];
+    if ((mroot.y >> kLinkBits) == 0) {
+        // the root is a leaf (one neuron): it holds the source, so no target (BarnesHutBase.h:231-237)
+        const uint32_t nid = mroot.y & kLinkMask;
+        uint32_t f = 0;
+        if (nid != src) {
+            for (uint32_t j = 0; j < na; j++) {
+                a.slot_target[slot0 + j] = nid;
+                a.slot_tests[slot0 + j] = 0;
+            }
+            f = na;
+        } else {
+            a.slot_tests[slot0] = 0;
+        }
+        a.found[src] = f;
+        a.searched[src] = f == na ? na : 1;
+        atomicAdd(&a.counters[2], static_cast<unsigned long long>(f == na ? na : 1));
+        return;
+    }
+    // level 1: one list for all axons of the neuron
+    const ListSum l1 = list_sum<kMixed>(a, L, 0, stk, stride);
+    exec += l1.tests_exec;
+    if (l1.count == 0 || l1.sum == 0.0) {
+        a.slot_tests[slot0] = l1.tests_ref;
+        a.found[src] = 0;
+        a.searched[src] = 1;
+        atomicAdd(&a.counters[0], static_cast<unsigned long long>(exec));
+        atomicAdd(&a.counters[1], static_cast<unsigned long long>(l1.tests_ref));
+        atomicAdd(&a.counters[2], 1ULL);
+        return;
+    }
+    const double next1 = nextafter(l1.sum, l1.sum + 0.00001);
+    for (uint32_t j0 = 0; j0 < na; j0 += kDrawsPerPass) {
+        PickVisitor pv;
+        pv.nd = static_cast<int>(min(na - j0, static_cast<uint32_t>(kDrawsPerPass)));
+#pragma unroll
+        for (int k = 0; k < kDrawsPerPass; k++) {
+            pv.pick[k] = kNone;
+            pv.r[k] = k < pv.nd ? draw_to_uniform(pick_draw(a.seed, src, a.step, j0 + k, 1U), next1) : 0.0;
+        }
+        exec += list_pick<kMixed>(a, L, 0, l1, pv, stk, stride);
+#pragma unroll
+        for (int k = 0; k < kDrawsPerPass; k++) {
+            if (k < pv.nd) {
+                a.slot_target[slot0 + j0 + k] = pv.pick[k];
+            }
+        }
+    }
+    // descents, axon by axon, stopping at the first failure
+    uint32_t found = 0;
+    uint32_t searched = 0;
+    for (uint32_t j = 0; j < na; j++) {
+        uint32_t node = a.slot_target[slot0 + j];
+        uint32_t tests_j = l1.tests_ref;
+        bool ok = true;
+        searched++;
+        for (uint32_t level = 2;; level++) {
+            if (node >= a.m || level > kMaxLevels + 2) {
+                ok = false;  // unreachable for a consistent tree; keeps every access in bounds
+                break;
+            }
+            const uint2 m = a.meta[L.sig][node];
+            if ((m.y >> kLinkBits) == 0) {
+                node = m.y & kLinkMask;  // leaf: the target neuron
+                break;
+            }
+            const ListSum ls = list_sum<kMixed>(a, L, node, stk, stride);
+            exec += ls.tests_exec;
+            tests_j += ls.tests_ref;
+            if (ls.count == 0 || ls.sum == 0.0) {
+                ok = false;
+                break;
+            }
+            PickVisitor pv;
+            pv.nd = 1;
+#pragma unroll
+            for (int k = 0; k < kDrawsPerPass; k++) {
+                pv.pick[k] = kNone;
+                pv.r[k] = 0.0;
+            }
+            pv.r[0] = draw_to_uniform(pick_draw(a.seed, src, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
+            exec += list_pick<kMixed>(a, L, node, ls, pv, stk, stride);
+            node = pv.pick[0];
+        }
+        a.slot_tests[slot0 + j] = tests_j;
+        ref += tests_j;
+        if (!ok) {
+            a.slot_target[slot0 + j] = kNone;
+            break;
+        }
+        a.slot_target[slot0 + j] = node;
+        found++;
+    }
+    a.found[src] = found;
+    a.searched[src] = searched;
+    atomicAdd(&a.counters[0], static_cast<unsigned long long>(exec));
+    atomicAdd(&a.counters[1], static_cast<unsigned long long>(ref));
+    atomicAdd(&a.counters[2], static_cast<unsigned long long>(searched));
+}
+
+__global__ void k_slot_offsets_input(uint64_t n, const uint32_t* __restrict__ vac, uint64_t* __restrict__ tmp) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        tmp[i] = vac[i];
+    } else if (i == n) {
+        tmp[i] = 0;
+    }
+}
+
+__global__ void k_widen(uint64_t n, const uint32_t* __restrict__ v, uint64_t* __restrict__ out) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        out[i] = v[i];
+    } else if (i == n) {
+        out[i] = 0;
+    }
+}
+
+__global__ void k_emit_requests(uint64_t n, const uint32_t* __restrict__ found, const uint64_t* __restrict__ req_off,
+                                const uint64_t* __restrict__ slot_off, const uint32_t* __restrict__ slot_target,
+                                const uint8_t* __restrict__ signal, uint32_t* __restrict__ rt, uint32_t* __restrict__ rs,
+                                uint8_t* __restrict__ rg, uint32_t* __restrict__ rj) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const uint32_t f = found[i];
+    const uint64_t o = req_off[i];
+    const uint64_t s = slot_off[i];
+    for (uint32_t j = 0; j < f; j++) {
+        rt[o + j] = slot_target[s + j];
+        rs[o + j] = static_cast<uint32_t>(i);
+        rg[o + j] = signal[i];
+        rj[o + j] = j;
+    }
+}
+
+}  // namespace
+
+int find_targets(rl_bh_ctx* c, uint32_t step) {
+    if (!c->prepared) {
+        return fail(c, RL_ESTATE, "find_target_neurons called before prepare_update_connectivity");
+    }
+    const uint64_t n = c->n;
+    hipStream_t s = c->stream;
+    c->nreq = 0;
+    c->searched_valid = false;
+    if (n == 0) {
+        c->searched_valid = true;
+        return RL_OK;
+    }
+    // slot offsets: exclusive prefix sum of the vacant axons
+    uint64_t* wide = nullptr;
+    RL_TRY(dalloc(c, &wide, n + 1));
+    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->vac[0], wide);
+    size_t tb = 0;
+    int status = RL_OK;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, wide, c->slot_off, static_cast<int>(n) + 1, s);
+    if (e == hipSuccess) {
+        status = ensure_tmp(c, tb);
+        if (status == RL_OK) {
+            e = hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, wide, c->slot_off, static_cast<int>(n) + 1, s);
+        }
+    }
+    dfree(&wide);
+    if (status != RL_OK) {
+        return status;
+    }
+    if (e != hipSuccess) {
+        return hip_fail(c, e, "slot offset scan");
+    }
+    uint64_t slots = 0;
+    RL_HIP(c, hipMemcpyAsync(&slots, c->slot_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipStreamSynchronize(s));
+    if (slots > c->slots_cap) {
+        RL_TRY(dalloc(c, &c->slot_target, slots));
+        RL_TRY(dalloc(c, &c->slot_tests, slots));
+        c->slots_cap = slots;
+    }
+    c->slots = slots;
+    unsigned long long* counters = nullptr;
+    RL_TRY(dalloc(c, &counters, 4));
+    RL_HIP(c, hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), s));
+    if (slots > 0) {
+        RL_HIP(c, hipMemsetAsync(c->slot_target, 0xFF, slots * sizeof(uint32_t), s));
+        RL_HIP(c, hipMemsetAsync(c->slot_tests, 0, slots * sizeof(uint32_t), s));
+    }
+
+    SearchArgs a{};
+    a.n = n;
+    a.m = c->tree.m;
+    a.order = c->order;
+    a.pos = c->pos;
+    a.qpos = c->qpos;
+    a.signal = c->signal;
+    a.enabled = c->enabled;
+    a.vac_axon = c->vac[0];
+    a.slot_off = c->slot_off;
+    for (int k = 0; k < 2; k++) {
+        a.rec64[k] = c->tree.rec64[k];
+        a.rec32[k] = c->tree.rec32[k];
+        a.meta[k] = c->tree.meta[k];
+    }
+    a.theta = c->cfg.theta;
+    a.scale64 = -1.0 / (c->cfg.sigma * c->cfg.sigma);
+    a.mu = c->cfg.mu;
+    a.theta2f = static_cast<float>(c->cfg.theta * c->cfg.theta);
+    a.k2f = static_cast<float>(-1.4426950408889634 / (c->cfg.sigma * c->cfg.sigma));
+    a.sigma_inv2f = static_cast<float>(1.0 / (c->cfg.sigma * c->cfg.sigma));
+    a.muf = static_cast<float>(c->cfg.mu);
+    for (int d = 0; d < 3; d++) {
+        a.inv_q[d] = c->qscale[d] > 0.0 ? static_cast<float>(1.0 / c->qscale[d]) : 0.0f;
+    }
+    a.guard = 2.0e-5f;
+    a.seed = c->cfg.seed;
+    a.step = step;
+    a.slot_target = c->slot_target;
+    a.slot_tests = c->slot_tests;
+    a.found = c->found;
+    a.searched = c->searched;
+    a.counters = counters;
+    const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
+    RL_HIP(c, hipEventRecord(c->ev0, s));
+    if (c->cfg.precision == RL_PREC_FP32_MIXED) {
+        k_search<true><<<grid_for(n), kBlock, lds, s>>>(a);
+    } else {
+        k_search<false><<<grid_for(n), kBlock, lds, s>>>(a);
+    }
+    RL_HIP(c, hipGetLastError());
+    RL_HIP(c, hipEventRecord(c->ev1, s));
+    // dense requests in (source, j) order
+    uint64_t* found_wide = nullptr;
+    uint64_t* req_off = nullptr;
+    RL_TRY(dalloc(c, &found_wide, n + 1));
+    RL_TRY(dalloc(c, &req_off, n + 1));
+    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->found, found_wide);
+    e = hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, found_wide, req_off, static_cast<int>(n) + 1, s);
+    if (e != hipSuccess) {
+        dfree(&found_wide);
+        dfree(&req_off);
+        return hip_fail(c, e, "request offset scan");
+    }
+    uint64_t nreq = 0;
+    unsigned long long hc[4] = { 0, 0, 0, 0 };
+    RL_HIP(c, hipMemcpyAsync(&nreq, req_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipStreamSynchronize(s));
+    float ms = 0.0f;
+    RL_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_search_ms = ms;
+    if (nreq > c->req_cap) {
+        RL_TRY(dalloc(c, &c->req_target, nreq));
+        RL_TRY(dalloc(c, &c->req_source, nreq));
+        RL_TRY(dalloc(c, &c->req_signal, nreq));
+        RL_TRY(dalloc(c, &c->req_j, nreq));
+        RL_TRY(dalloc(c, &c->req_ok, nreq));
+        c->req_cap = nreq;
+    }
+    c->nreq = nreq;
+    if (nreq > 0) {
+        k_emit_requests<<<grid_for(n), kBlock, 0, s>>>(n, c->found, req_off, c->slot_off, c->slot_target, c->signal,
+                                                        c->req_target, c->req_source, c->req_signal, c->req_j);
+        RL_HIP(c, hipGetLastError());
+    }
+    RL_HIP(c, hipStreamSynchronize(s));
+    dfree(&found_wide);
+    dfree(&req_off);
+    dfree(&counters);
+    c->last_tests = hc[0];
+    c->last_tests_ref = hc[1];
+    c->last_searches = hc[2];
+    c->searched_valid = true;
+    return RL_OK;
+}
+
+}  // namespace rl

@@ -1,0 +1,286 @@
+// Fused per-step neuron update: background input, Izhikevich membrane update, calcium integrator and
+// the growth-curve deltas of the three synaptic element types, one lane per neuron, FP64 throughout
+// and in the CPU path's operation order (files compiled with -ffp-contract=off):
+//   IzhikevichModelCPU::update_activity   (source/neurons/models/izhikevich/IzhikevichModelCPU.cpp:75-125)
+//   CalciumCalculatorCPU::update_current_calcium (source/neurons/calcium/CalciumCalculatorCPU.cpp:40-103)
+//   update_number_elements_openmp + gaussian_growth_curve
+//                                       (source/neurons/synaptic_elements/SynapticElementsCPU.cpp:25-49,
+//                                        GrowthCurves.h:30-54)
+// and, at plasticity steps, ElementBaseBase::update (ElementBaseBase.h:420-487) per element type.
+#include "rl_device.h"
+#include "rl_internal.h"
+
+namespace rl {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(uint64_t n, int block = kBlock) {
+    const uint64_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+__device__ __forceinline__ double growth_curve(double current, double eta, double epsilon) {
+#pragma clang fp contract(off)
+    if (eta == epsilon) {
+        return current == eta ? 0.0 : -1.0;
+    }
+    constexpr double factor = 1.6651092223153955127063292897904020952611777045288814583336582344;
+    constexpr double factor_inv = 1.0 / factor;
+    const double xi = (eta + epsilon) * 0.5;
+    const double zeta = (eta - epsilon) * factor_inv;
+    const double difference = current - xi;
+    const double quotient = difference / zeta;
+    const double product = quotient * quotient;
+    return (2.0 * exp(-product)) - 1.0;
+}
+
+// Normal(mean, stddev) from two 32-bit Philox words by the Box-Muller transform (tag 3).
+__device__ __forceinline__ double normal_input(uint64_t seed, uint32_t neuron, uint32_t step, double mean, double sd) {
+    const uint4 o = philox4x32_10(make_uint4(neuron, step, 0U, 3U),
+                                  make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32U)));
+    const double u1 = (static_cast<double>(o.x) + 1.0) * 2.3283064365386963e-10;  // (0, 1]
+    const double u2 = static_cast<double>(o.y) * 2.3283064365386963e-10;          // [0, 1)
+    const double z = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+    return mean + sd * z;
+}
+
+struct StepArgs {
+    uint64_t n;
+    rl_step_params p;
+    uint64_t seed;
+    uint32_t step;
+    double* x;
+    double* u;
+    double* ca;
+    const double* input;
+    uint8_t* fired;
+    const double* target_ca;
+    double* delta;
+    const uint8_t* enabled;
+};
+
+__global__ void k_step(StepArgs a) {
+#pragma clang fp contract(off)
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= a.n || a.enabled[i] == 0) {
+        return;
+    }
+    const rl_step_params& p = a.p;
+    const double input = p.input_mode == 0 ? a.input[i]
+                                           : normal_input(a.seed, static_cast<uint32_t>(i), a.step, p.input_mean,
+                                                          p.input_stddev);
+    const double scale = 1.0 / static_cast<double>(p.h);
+    const double ab = p.a * p.b;
+    double xv = a.x[i];
+    double uv = a.u[i];
+    uint8_t spiked = 0;
+    for (uint32_t s = 0; s < p.h; s++) {
+        const double x_increase = (p.k1 * xv * xv) + (p.k2 * xv) + p.k3 - uv + input;
+        const double u_increase = (ab * xv) - (p.a * uv);
+        xv += x_increase * scale;
+        uv += u_increase * scale;
+        if (xv >= p.v_spike) {
+            xv = p.c;
+            uv += p.d;
+            spiked = 1;
+            break;
+        }
+    }
+    a.fired[i] = spiked;
+    a.x[i] = xv;
+    a.u[i] = uv;
+    const double tau_inv = -1.0 / p.tau_ca;
+    const double local_beta = p.beta_ca * static_cast<double>(spiked == 1);
+    double c = a.ca[i];
+    for (uint32_t s = 0; s < p.h; s++) {
+        c += scale * (c * tau_inv + local_beta);
+    }
+    a.ca[i] = c;
+    const double tgt = a.target_ca[i];
+    for (int t = 0; t < 3; t++) {
+        const double clamped_target = tgt < p.eta[t] ? p.eta[t] : tgt;  // std::max(target, eta)
+        const double dz = growth_curve(c, p.eta[t], clamped_target);
+        a.delta[t * a.n + i] += dz * p.nu[t];
+    }
+}
+
+__global__ void k_elements_update(uint64_t n, double* __restrict__ grown, double* __restrict__ delta,
+                                  uint32_t* __restrict__ c0, uint32_t* __restrict__ c1, uint32_t* __restrict__ c2,
+                                  uint32_t* __restrict__ v0, uint32_t* __restrict__ v1, uint32_t* __restrict__ v2,
+                                  uint32_t* __restrict__ deleted) {
+#pragma clang fp contract(off)
+    const uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (k >= 3 * n) {
+        return;
+    }
+    const int t = static_cast<int>(k / n);
+    const uint64_t i = k - static_cast<uint64_t>(t) * n;
+    uint32_t* connected = t == 0 ? c0 : (t == 1 ? c1 : c2);
+    uint32_t* vacant = t == 0 ? v0 : (t == 1 ? v1 : v2);
+    const double current_count = grown[k];
+    const double current_delta = delta[k];
+    const uint32_t connected_integral = connected[i];
+    const double retract = 0.0;
+    const double current_connected = static_cast<double>(connected_integral);
+    const double current_vacant = current_count - current_connected;
+    const double new_vacant = current_vacant + current_delta;
+    if (new_vacant >= 0.0) {
+        const double new_vacant_after_retract = (1 - retract) * new_vacant;
+        grown[k] = new_vacant_after_retract + current_connected;
+        delta[k] = 0.0;
+        vacant[i] = static_cast<uint32_t>(new_vacant_after_retract);
+        deleted[k] = 0;
+        return;
+    }
+    if (current_count + current_delta <= 0.0) {
+        grown[k] = 0.0;
+        delta[k] = 0.0;
+        vacant[i] = 0;
+        connected[i] = 0;
+        deleted[k] = connected_integral;
+        return;
+    }
+    const double new_count = current_count + current_delta;
+    const double new_connected_count = floor(new_count);
+    const double num_vacant = new_count - new_connected_count;
+    grown[k] = ((1 - retract) * num_vacant) + new_connected_count;
+    delta[k] = 0.0;
+    vacant[i] = 0;
+    connected[i] = static_cast<uint32_t>(new_connected_count);
+    deleted[k] = static_cast<uint32_t>(current_connected - new_connected_count);
+}
+
+}  // namespace
+
+void neurons_free(rl_bh_ctx* c) {
+    Neurons& s = c->neurons;
+    dfree(&s.x);
+    dfree(&s.u);
+    dfree(&s.ca);
+    dfree(&s.target_ca);
+    dfree(&s.input);
+    dfree(&s.fired);
+    dfree(&s.grown);
+    dfree(&s.delta);
+    dfree(&s.deleted);
+    s.ready = false;
+}
+
+int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const double* u, const double* ca,
+                 const double* target_ca, const double* grown, const uint32_t* connected) {
+    const uint64_t n = c->n;
+    Neurons& s = c->neurons;
+    hipStream_t st = c->stream;
+    neurons_free(c);
+    s.p = *p;
+    RL_TRY(dalloc(c, &s.x, n));
+    RL_TRY(dalloc(c, &s.u, n));
+    RL_TRY(dalloc(c, &s.ca, n));
+    RL_TRY(dalloc(c, &s.target_ca, n));
+    RL_TRY(dalloc(c, &s.input, n));
+    RL_TRY(dalloc(c, &s.fired, n));
+    RL_TRY(dalloc(c, &s.grown, 3 * n));
+    RL_TRY(dalloc(c, &s.delta, 3 * n));
+    RL_TRY(dalloc(c, &s.deleted, 3 * n));
+    auto up = [&](double* d, const double* h, uint64_t cnt) -> int {
+        if (h != nullptr) {
+            RL_HIP(c, hipMemcpyAsync(d, h, cnt * sizeof(double), hipMemcpyHostToDevice, st));
+        } else {
+            RL_HIP(c, hipMemsetAsync(d, 0, cnt * sizeof(double), st));
+        }
+        return RL_OK;
+    };
+    RL_TRY(up(s.x, x, n));
+    RL_TRY(up(s.u, u, n));
+    RL_TRY(up(s.ca, ca, n));
+    RL_TRY(up(s.target_ca, target_ca, n));
+    RL_TRY(up(s.grown, grown, 3 * n));
+    RL_HIP(c, hipMemsetAsync(s.delta, 0, 3 * n * sizeof(double), st));
+    RL_HIP(c, hipMemsetAsync(s.fired, 0, n, st));
+    for (int t = 0; t < 3; t++) {
+        if (connected != nullptr) {
+            RL_HIP(c, hipMemcpyAsync(c->conn[t], connected + t * n, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        }
+    }
+    RL_HIP(c, hipStreamSynchronize(st));
+    s.ready = true;
+    return RL_OK;
+}
+
+int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
+    Neurons& s = c->neurons;
+    if (!s.ready) {
+        return fail(c, RL_ESTATE, "rl_neurons_step called before rl_neurons_init");
+    }
+    if (s.p.input_mode == 0 && input == nullptr) {
+        return fail(c, RL_EINVAL, "input_mode 0 needs an input array");
+    }
+    const uint64_t n = c->n;
+    if (n == 0) {
+        return RL_OK;
+    }
+    if (s.p.input_mode == 0) {
+        RL_HIP(c, hipMemcpyAsync(s.input, input, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    StepArgs a{};
+    a.n = n;
+    a.p = s.p;
+    a.seed = c->cfg.seed;
+    a.step = step;
+    a.x = s.x;
+    a.u = s.u;
+    a.ca = s.ca;
+    a.input = s.input;
+    a.fired = s.fired;
+    a.target_ca = s.target_ca;
+    a.delta = s.delta;
+    a.enabled = c->enabled;
+    k_step<<<grid_for(n), kBlock, 0, c->stream>>>(a);
+    RL_HIP(c, hipGetLastError());
+    return RL_OK;
+}
+
+int neurons_commit(rl_bh_ctx* c, uint32_t* deleted) {
+    Neurons& s = c->neurons;
+    if (!s.ready) {
+        return fail(c, RL_ESTATE, "rl_neurons_commit_elements called before rl_neurons_init");
+    }
+    const uint64_t n = c->n;
+    if (n == 0) {
+        return RL_OK;
+    }
+    k_elements_update<<<grid_for(3 * n), kBlock, 0, c->stream>>>(n, s.grown, s.delta, c->conn[0], c->conn[1], c->conn[2],
+                                                                  c->vac[0], c->vac[1], c->vac[2], s.deleted);
+    RL_HIP(c, hipGetLastError());
+    if (deleted != nullptr) {
+        RL_HIP(c, hipMemcpyAsync(deleted, s.deleted, 3 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    }
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    return RL_OK;
+}
+
+int neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
+                uint32_t* connected) {
+    Neurons& s = c->neurons;
+    if (!s.ready) {
+        return fail(c, RL_ESTATE, "rl_neurons_get called before rl_neurons_init");
+    }
+    const uint64_t n = c->n;
+    hipStream_t st = c->stream;
+    if (x) RL_HIP(c, hipMemcpyAsync(x, s.x, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (u) RL_HIP(c, hipMemcpyAsync(u, s.u, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (ca) RL_HIP(c, hipMemcpyAsync(ca, s.ca, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (fired) RL_HIP(c, hipMemcpyAsync(fired, s.fired, n, hipMemcpyDeviceToHost, st));
+    if (delta) RL_HIP(c, hipMemcpyAsync(delta, s.delta, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (grown) RL_HIP(c, hipMemcpyAsync(grown, s.grown, 3 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (connected) {
+        for (int t = 0; t < 3; t++) {
+            RL_HIP(c, hipMemcpyAsync(connected + t * n, c->conn[t], n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        }
+    }
+    RL_HIP(c, hipStreamSynchronize(st));
+    return RL_OK;
+}
+
+}  // namespace rl

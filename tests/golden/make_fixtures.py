@@ -1,0 +1,45 @@
+"""Writes the committed fixtures of tests/golden/ (run in the build container, where the reference is mounted).
+
+positions_51842.npz: the 51,842 real neuron positions of the reference's input/positions.txt (a data file
+of the reference's own test inputs), parsed from its text columns <id> <x> <y> <z> <area> <type> into
+float64 positions, a 0/1 signal type (ex -> 0, in -> 1) and the subdomain box stated in its header.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = "/root/reference/relearn/input/positions.txt"
+
+
+def positions_fixture(path=SRC):
+    xyz, sig = [], []
+    box_max = None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#"):
+                if "Local subdomain 0 boundaries" in line:
+                    tail = line.split("-")[-1].strip().strip("()")
+                    box_max = [float(v) for v in tail.split(",")]
+                continue
+            parts = line.split()
+            if len(parts) < 6:
+                continue
+            xyz.append([float(parts[1]), float(parts[2]), float(parts[3])])
+            sig.append(0 if parts[-1] == "ex" else 1)
+    xyz = np.asarray(xyz, dtype=np.float64)
+    return xyz, np.asarray(sig, dtype=np.uint8), np.asarray(box_max, dtype=np.float64)
+
+
+def main():
+    xyz, sig, box_max = positions_fixture()
+    out = os.path.join(HERE, "positions_51842.npz")
+    np.savez_compressed(out, xyz=xyz, signal=sig, box_min=np.zeros(3), box_max=box_max)
+    print(out, xyz.shape, box_max, os.path.getsize(out))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
