@@ -1,0 +1,231 @@
+"""Benchmark of the Barnes-Hut connectivity update on one MI355X.
+
+Workload (default): 16M neurons uniform in a 25,000 um cube, 80/20 E/I, vacant axons ~ Poisson(2),
+vacant excitatory/inhibitory dendrites ~ Poisson(1.5), theta 0.3, sigma 750 um, leaf capacity 1.
+A step is one plasticity update on the device-resident state: vacancies restored to the workload's
+values (device-to-device copy), leaf fill + upward pass, target search, conflict resolution and
+commit.  The FP32-mixed and the FP64 paths run the same seeded input; `value` is the FP32-mixed
+whole-step throughput in target searches per second.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "target searches/s @16M neurons, FP32-mixed vs FP64; ms per plasticity step"
+HBM_PEAK_GBS = 8000.0
+B_NODE = {"fp64": 40, "fp32_mixed": 24}   # bytes read per acceptance test (record + count/link)
+B_IO = 32                                  # source position + output ids per search
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def run_precision(net, precision, steps, warmup, seed, d_vac):
+    from relearn_amd import BarnesHutHIP
+
+    g = BarnesHutHIP(net.box_min, net.box_max, theta=0.3, sigma=750.0, seed=seed, precision=precision)
+    t0 = time.perf_counter()
+    g.init(net.xyz, net.signal)
+    build_s = time.perf_counter() - t0
+    m, depth = g.tree_info()
+    log(f"precision {precision}: tree {m} nodes, depth {depth}, build {build_s:.3f} s")
+
+    def one_step(step):
+        t_a = time.perf_counter()
+        g.prepare_update_connectivity_device(*d_vac)
+        t_b = time.perf_counter()
+        g.find_target_neurons(step)
+        t_c = time.perf_counter()
+        st = g.last_search_stats()
+        created = g.commit(step)
+        t_d = time.perf_counter()
+        return st, created, (t_b - t_a, t_c - t_b, t_d - t_c)
+
+    for w in range(warmup):
+        one_step(1000 + w)
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = []
+    for k in range(steps):
+        recs.append(one_step(k))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    searches = sum(r[0]["searches"] for r in recs)
+    tests = sum(r[0]["node_tests"] for r in recs)
+    tests_ref = sum(r[0]["node_tests_ref"] for r in recs)
+    kernel_ms = [r[0]["ms_search"] for r in recs]
+    # pick results of step 0 for the precision comparison
+    g.prepare_update_connectivity_device(*d_vac)
+    g.find_target_neurons(0)
+    slots = int(net.vac_axon.astype(np.uint64).sum())
+    targets, _, counts = g.get_search_slots(slots)
+    created0 = g.commit(0)
+    g.close()
+    return {
+        "build_s": build_s,
+        "nodes": m,
+        "depth": depth,
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "searches_per_step": searches / steps,
+        "searches_per_s": searches / wall,
+        "search_kernel_ms": float(np.mean(kernel_ms)),
+        "search_kernel_searches_per_s": searches / (1e-3 * sum(kernel_ms)),
+        "node_tests_per_step": tests / steps,
+        "node_tests_ref_per_step": tests_ref / steps,
+        "phase_s": [float(np.mean([r[2][i] for r in recs])) for i in range(3)],
+        "requests_per_step": float(np.mean([r[0]["requests"] for r in recs])),
+        "created_per_step": float(np.mean([r[1] for r in recs])),
+        "targets": targets,
+        "counts": counts,
+        "created0": created0,
+    }
+
+
+def cpu_baseline(net, sample, threads, seed):
+    from oracle import pyoracle
+
+    t0 = time.perf_counter()
+    t = pyoracle.Tree(net.xyz, net.box_min, net.box_max)
+    t.update(net.vac_exc, net.vac_inh, threads=threads)
+    prep = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    cand = np.nonzero(net.vac_axon)[0]
+    src = np.sort(rng.choice(cand, min(sample, len(cand)), replace=False))
+    r = t.find_targets_faithful(net.signal, net.vac_axon, src, theta=0.3, sigma=750.0, seed=seed, threads=threads)
+    t.close()
+    return r, prep, len(src)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--neurons", type=int, default=16_000_000)
+    ap.add_argument("--side", type=float, default=25_000.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=20_000, help="source neurons searched by the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32_mixed"], default=None)
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the sharded design is documented, not built)", file=sys.stderr)
+        return 2
+
+    from relearn_amd import RL_PREC_FP32_MIXED, RL_PREC_FP64
+    from relearn_amd.workloads import uniform_cube
+    import torch
+
+    t0 = time.perf_counter()
+    net = uniform_cube(args.neurons, args.side, seed=args.seed)
+    log(f"workload {args.neurons} neurons in {args.side} um cube generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    d_arrays = [torch.from_numpy(a.view(np.int32)).to(dev) for a in (net.vac_axon, net.vac_exc, net.vac_inh)]
+    torch.cuda.synchronize()
+    d_vac = [int(a.data_ptr()) for a in d_arrays]
+
+    res = {}
+    precs = [("fp32_mixed", RL_PREC_FP32_MIXED), ("fp64", RL_PREC_FP64)]
+    if args.only:
+        precs = [p for p in precs if p[0] == args.only]
+    for name, prec in precs:
+        res[name] = run_precision(net, prec, args.steps, args.warmup, args.seed + 1, d_vac)
+        r = res[name]
+        log(f"{name}: {r['searches_per_s']:.4g} searches/s whole step, {r['ms_per_step']:.1f} ms/step, "
+            f"search kernel {r['search_kernel_ms']:.1f} ms, phases {['%.3f' % x for x in r['phase_s']]} s")
+
+    main_name = "fp32_mixed" if "fp32_mixed" in res else "fp64"
+    main_r = res[main_name]
+    b_node = B_NODE[main_name]
+    bytes_per_launch = main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO
+    achieved = bytes_per_launch / (1e-3 * main_r["search_kernel_ms"]) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": main_r["searches_per_s"],
+        "unit": "searches/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed" if main_name == "fp32_mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.neurons} neurons uniform in a {args.side:g} um cube, 80/20 E/I, vacant axons "
+                        "Poisson(2), vacant dendrites Poisson(1.5) per type, theta 0.3, sigma 750 um, one "
+                        "plasticity update per step",
+            "neurons": args.neurons,
+            "side_um": args.side,
+            "seed": args.seed,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_search",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_model": f"executed acceptance tests x {b_node} B + searches x {B_IO} B",
+        },
+    }
+    for name, r in res.items():
+        out[name] = {k: v for k, v in r.items() if k not in ("targets", "counts")}
+    if "fp64" in res and "fp32_mixed" in res:
+        a, b = res["fp64"], res["fp32_mixed"]
+        valid = a["targets"] != 0xFFFFFFFF
+        div = float(np.mean(a["targets"][valid] != b["targets"][valid]))
+        out["fp32_vs_fp64"] = {
+            "speedup_whole_step": a["ms_per_step"] / b["ms_per_step"],
+            "speedup_search_kernel": a["search_kernel_ms"] / b["search_kernel_ms"],
+            "pick_divergence": div,
+            "requests_fp64": int(a["counts"].sum()),
+            "requests_fp32": int(b["counts"].sum()),
+            "mean_synapses_per_neuron_fp64": a["created0"] / args.neurons,
+            "mean_synapses_per_neuron_fp32": b["created0"] / args.neurons,
+            "synapses_rel_diff": abs(b["created0"] - a["created0"]) / max(1, a["created0"]),
+        }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        r, prep, ns = cpu_baseline(net, args.cpu_sample, threads, args.seed + 1)
+        sps = r["searches"] / r["seconds"]
+        out["cpu_baseline"] = {
+            "value": sps,
+            "unit": "searches/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{r['searches']} searches of {ns} random source neurons on the full "
+                      f"{args.neurons}-neuron pointer octree (mt19937 stream, omp parallel for + critical "
+                      f"appends); tree build + upward pass {prep:.1f} s not in the rate",
+            "ms_per_step_extrapolated": 1e3 * main_r["searches_per_step"] / sps,
+            "gpu_speedup": out["value"] / sps,
+        }
+        log(f"cpu baseline: {sps:.4g} searches/s on {threads} threads ({r['searches']} searches)")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

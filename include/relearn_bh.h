@@ -99,6 +99,13 @@ int rl_bh_prepare_update_connectivity(rl_bh_ctx* ctx, const uint32_t* vacant_axo
  * (those of the last upload or commit). */
 int rl_bh_prepare_resident(rl_bh_ctx* ctx);
 
+/* rl_bh_prepare_update_connectivity for callers whose vacancy spans live in device memory of the
+ * context's GPU (e.g. synaptic elements kept on the device): device-to-device copies, then leaf fill
+ * and upward pass.  The pointers are read during the call only. */
+int rl_bh_prepare_update_connectivity_device(rl_bh_ctx* ctx, const uint32_t* d_vacant_axons,
+                                             const uint32_t* d_vacant_excitatory_dendrites,
+                                             const uint32_t* d_vacant_inhibitory_dendrites);
+
 /* BarnesHut::find_target_neurons (BarnesHut.cpp:36-94): one search per vacant axon of every enabled
  * neuron, stopping at a neuron's first failure.  step is the plasticity-update counter that
  * addresses the draw stream.  Returns the number of requests in *num_requests (may be NULL). */
@@ -145,6 +152,10 @@ int rl_bh_export_tree(rl_bh_ctx* ctx, uint8_t* level, uint8_t* nchild, uint32_t*
 
 /* Milliseconds of the last search kernel(s), measured with HIP events on the context's stream. */
 int rl_bh_last_search_ms(rl_bh_ctx* ctx, double* ms);
+
+/* Counters of the last search: searches, requests, node_tests, node_tests_ref and ms_search are
+ * filled (created and the other times are 0). */
+int rl_bh_last_search_stats(rl_bh_ctx* ctx, rl_bh_step_stats* stats);
 
 /* ------------------------------------------------------------------ neuron step */
 

@@ -24,10 +24,11 @@ _f64p = C.POINTER(C.c_double)
 
 EXPORTED = [
     "rl_bh_create", "rl_bh_destroy", "rl_bh_last_error", "rl_bh_init", "rl_bh_set_enabled",
-    "rl_bh_prepare_update_connectivity", "rl_bh_prepare_resident", "rl_bh_find_target_neurons",
+    "rl_bh_prepare_update_connectivity", "rl_bh_prepare_resident", "rl_bh_prepare_update_connectivity_device",
+    "rl_bh_find_target_neurons",
     "rl_bh_get_requests", "rl_bh_get_search_slots", "rl_bh_commit", "rl_bh_get_commit_result",
     "rl_bh_update_connectivity", "rl_bh_get_vacancies", "rl_bh_tree_info", "rl_bh_export_tree",
-    "rl_bh_last_search_ms", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
+    "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
     "rl_neurons_get",
 ]
 
@@ -110,6 +111,7 @@ def load():
     L.rl_bh_set_enabled.argtypes = [vp, _u8p]
     L.rl_bh_prepare_update_connectivity.argtypes = [vp, _u32p, _u32p, _u32p]
     L.rl_bh_prepare_resident.argtypes = [vp]
+    L.rl_bh_prepare_update_connectivity_device.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
     L.rl_bh_find_target_neurons.argtypes = [vp, C.c_uint32, _u64p]
     L.rl_bh_get_requests.argtypes = [vp, C.c_uint64, _u32p, _u32p, _u8p, _u32p]
     L.rl_bh_get_search_slots.argtypes = [vp, C.c_uint64, _u32p, _u32p, _u32p]
@@ -121,6 +123,7 @@ def load():
     L.rl_bh_export_tree.argtypes = [vp, _u8p, _u8p, _u32p, _u8p, _f64p, _f64p, _f64p, _u32p, _u32p, _u8p, _u8p,
                                     _f64p, _f64p]
     L.rl_bh_last_search_ms.argtypes = [vp, _f64p]
+    L.rl_bh_last_search_stats.argtypes = [vp, C.POINTER(StepStats)]
     L.rl_neurons_init.argtypes = [vp, C.POINTER(StepParams), _f64p, _f64p, _f64p, _f64p, _f64p, _u32p]
     L.rl_neurons_step.argtypes = [vp, C.c_uint32, _f64p]
     L.rl_neurons_commit_elements.argtypes = [vp, _u32p]
@@ -207,6 +210,11 @@ class BarnesHutHIP:
         self._check(self.L.rl_bh_prepare_update_connectivity(self.h, _ptr(va, _u32p), _ptr(ve, _u32p),
                                                              _ptr(vi, _u32p)))
 
+    def prepare_update_connectivity_device(self, d_va, d_ve, d_vi):
+        """Vacancy spans given as device addresses (ints) on the context's GPU."""
+        self._check(self.L.rl_bh_prepare_update_connectivity_device(self.h, C.c_void_p(d_va), C.c_void_p(d_ve),
+                                                                    C.c_void_p(d_vi)))
+
     def prepare_resident(self):
         self._check(self.L.rl_bh_prepare_resident(self.h))
 
@@ -284,6 +292,11 @@ class BarnesHutHIP:
         ms = C.c_double()
         self._check(self.L.rl_bh_last_search_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def last_search_stats(self):
+        st = StepStats()
+        self._check(self.L.rl_bh_last_search_stats(self.h, C.byref(st)))
+        return st.as_dict()
 
     # neuron step
     def neurons_init(self, params: StepParams, target_ca, x=None, u=None, ca=None, grown=None, connected=None):

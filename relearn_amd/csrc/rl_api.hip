@@ -67,6 +67,18 @@ void free_all(rl_bh_ctx* c) {
     dfree(&c->req_signal);
     dfree(&c->req_j);
     dfree(&c->req_ok);
+    dfree(&c->wide);
+    dfree(&c->req_off);
+    dfree(&c->counters);
+    dfree(&c->flags);
+    dfree(&c->c_prio);
+    dfree(&c->c_prio2);
+    dfree(&c->c_order);
+    dfree(&c->c_idx);
+    dfree(&c->c_idx2);
+    dfree(&c->c_g);
+    dfree(&c->c_g2);
+    c->commit_cap = 0;
     if (c->tmp != nullptr) {
         (void)hipFree(c->tmp);
         c->tmp = nullptr;
@@ -200,6 +212,10 @@ int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signa
         RL_TRY(dalloc(c, &c->conn[t], n));
     }
     RL_TRY(dalloc(c, &c->slot_off, n + 1));
+    RL_TRY(dalloc(c, &c->wide, n + 1));
+    RL_TRY(dalloc(c, &c->req_off, n + 1));
+    RL_TRY(dalloc(c, &c->counters, 8));
+    RL_TRY(dalloc(c, &c->flags, 4));
     RL_TRY(dalloc(c, &c->found, n));
     RL_TRY(dalloc(c, &c->searched, n));
     hipStream_t s = c->stream;
@@ -249,6 +265,22 @@ int rl_bh_prepare_update_connectivity(rl_bh_ctx* c, const uint32_t* vacant_axons
     const uint32_t* h[3] = { vacant_axons, vacant_exc, vacant_inh };
     for (int t = 0; t < 3 && c->n > 0; t++) {
         RL_HIP(c, hipMemcpyAsync(c->vac[t], h[t], c->n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+    }
+    return rl_bh_prepare_resident(c);
+}
+
+int rl_bh_prepare_update_connectivity_device(rl_bh_ctx* c, const uint32_t* d_va, const uint32_t* d_ve,
+                                             const uint32_t* d_vi) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (c->n > 0 && (d_va == nullptr || d_ve == nullptr || d_vi == nullptr)) {
+        return fail(c, RL_EINVAL, "prepare_update_connectivity_device: null vacancy span");
+    }
+    RL_TRY(set_device(c));
+    const uint32_t* d[3] = { d_va, d_ve, d_vi };
+    for (int t = 0; t < 3 && c->n > 0; t++) {
+        RL_HIP(c, hipMemcpyAsync(c->vac[t], d[t], c->n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
     }
     return rl_bh_prepare_resident(c);
 }
@@ -453,6 +485,22 @@ int rl_bh_last_search_ms(rl_bh_ctx* c, double* ms) {
         return RL_EINVAL;
     }
     *ms = c->last_search_ms;
+    return RL_OK;
+}
+
+int rl_bh_last_search_stats(rl_bh_ctx* c, rl_bh_step_stats* st) {
+    if (c == nullptr || st == nullptr) {
+        return RL_EINVAL;
+    }
+    if (!c->searched_valid) {
+        return fail(c, RL_ESTATE, "last_search_stats: no search result");
+    }
+    *st = rl_bh_step_stats{};
+    st->searches = c->last_searches;
+    st->requests = c->nreq;
+    st->node_tests = c->last_tests;
+    st->node_tests_ref = c->last_tests_ref;
+    st->ms_search = c->last_search_ms;
     return RL_OK;
 }
 

@@ -113,17 +113,27 @@ int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* creat
     if (m == 0) {
         return RL_OK;
     }
-    uint64_t *prio = nullptr, *prio2 = nullptr, *order = nullptr;
-    uint32_t *idx = nullptr, *idx2 = nullptr, *g = nullptr, *g2 = nullptr;
-    int* err = nullptr;
-    unsigned long long* cnt = nullptr;
+    if (m > c->commit_cap) {
+        RL_TRY(dalloc(c, &c->c_prio, m));
+        RL_TRY(dalloc(c, &c->c_prio2, m));
+        RL_TRY(dalloc(c, &c->c_order, m));
+        RL_TRY(dalloc(c, &c->c_idx, m));
+        RL_TRY(dalloc(c, &c->c_idx2, m));
+        RL_TRY(dalloc(c, &c->c_g, m));
+        RL_TRY(dalloc(c, &c->c_g2, m));
+        c->commit_cap = m;
+    }
+    uint64_t* prio = c->c_prio;
+    uint64_t* prio2 = c->c_prio2;
+    uint64_t* order = c->c_order;
+    uint32_t* idx = c->c_idx;
+    uint32_t* idx2 = c->c_idx2;
+    uint32_t* g = c->c_g;
+    uint32_t* g2 = c->c_g2;
+    int* err = c->flags;
+    unsigned long long* cnt = c->counters + 4;
     int status = RL_OK;
     do {
-        if ((status = dalloc(c, &prio, m)) || (status = dalloc(c, &prio2, m)) || (status = dalloc(c, &idx, m)) ||
-            (status = dalloc(c, &idx2, m)) || (status = dalloc(c, &g, m)) || (status = dalloc(c, &g2, m)) ||
-            (status = dalloc(c, &err, 1)) || (status = dalloc(c, &cnt, 1))) {
-            break;
-        }
 #define RL_STEP(expr)                            \
     {                                            \
         hipError_t e_ = (expr);                  \
@@ -135,9 +145,6 @@ int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* creat
         RL_STEP(hipMemsetAsync(err, 0, sizeof(int), s));
         RL_STEP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), s));
         if (h_order != nullptr) {
-            if ((status = dalloc(c, &order, m)) != RL_OK) {
-                break;
-            }
             RL_STEP(hipMemcpyAsync(order, h_order, m * sizeof(uint64_t), hipMemcpyHostToDevice, s));
             RL_STEP(hipMemsetAsync(prio, 0xFF, m * sizeof(uint64_t), s));
             k_rank_from_order<<<grid_for(m), kBlock, 0, s>>>(m, order, prio, idx, err);
@@ -174,15 +181,6 @@ int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* creat
         }
 #undef RL_STEP
     } while (false);
-    dfree(&prio);
-    dfree(&prio2);
-    dfree(&order);
-    dfree(&idx);
-    dfree(&idx2);
-    dfree(&g);
-    dfree(&g2);
-    dfree(&err);
-    dfree(&cnt);
     return status;
 }
 

@@ -98,7 +98,19 @@ struct rl_bh_ctx {
     uint64_t last_tests = 0;
     uint64_t last_tests_ref = 0;
 
-    // scratch
+    // persistent scratch (grown on demand, never per call)
+    uint64_t* wide = nullptr;        // n + 1
+    uint64_t* req_off = nullptr;     // n + 1
+    unsigned long long* counters = nullptr;  // 8
+    int* flags = nullptr;            // 4
+    uint64_t commit_cap = 0;
+    uint64_t* c_prio = nullptr;
+    uint64_t* c_prio2 = nullptr;
+    uint64_t* c_order = nullptr;
+    uint32_t* c_idx = nullptr;
+    uint32_t* c_idx2 = nullptr;
+    uint32_t* c_g = nullptr;
+    uint32_t* c_g2 = nullptr;
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
     rl::Neurons neurons;

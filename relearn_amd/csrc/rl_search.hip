@@ -432,15 +432,6 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
     atomicAdd(&a.counters[2], static_cast<unsigned long long>(searched));
 }
 
-__global__ void k_slot_offsets_input(uint64_t n, const uint32_t* __restrict__ vac, uint64_t* __restrict__ tmp) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i < n) {
-        tmp[i] = vac[i];
-    } else if (i == n) {
-        tmp[i] = 0;
-    }
-}
-
 __global__ void k_widen(uint64_t n, const uint32_t* __restrict__ v, uint64_t* __restrict__ out) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i < n) {
@@ -484,25 +475,12 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
         return RL_OK;
     }
     // slot offsets: exclusive prefix sum of the vacant axons
-    uint64_t* wide = nullptr;
-    RL_TRY(dalloc(c, &wide, n + 1));
-    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->vac[0], wide);
+    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->vac[0], c->wide);
+    RL_HIP(c, hipGetLastError());
     size_t tb = 0;
-    int status = RL_OK;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, wide, c->slot_off, static_cast<int>(n) + 1, s);
-    if (e == hipSuccess) {
-        status = ensure_tmp(c, tb);
-        if (status == RL_OK) {
-            e = hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, wide, c->slot_off, static_cast<int>(n) + 1, s);
-        }
-    }
-    dfree(&wide);
-    if (status != RL_OK) {
-        return status;
-    }
-    if (e != hipSuccess) {
-        return hip_fail(c, e, "slot offset scan");
-    }
+    RL_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, c->wide, c->slot_off, static_cast<int>(n) + 1, s));
+    RL_TRY(ensure_tmp(c, tb));
+    RL_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, c->wide, c->slot_off, static_cast<int>(n) + 1, s));
     uint64_t slots = 0;
     RL_HIP(c, hipMemcpyAsync(&slots, c->slot_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     RL_HIP(c, hipStreamSynchronize(s));
@@ -512,8 +490,7 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
         c->slots_cap = slots;
     }
     c->slots = slots;
-    unsigned long long* counters = nullptr;
-    RL_TRY(dalloc(c, &counters, 4));
+    unsigned long long* counters = c->counters;
     RL_HIP(c, hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), s));
     if (slots > 0) {
         RL_HIP(c, hipMemsetAsync(c->slot_target, 0xFF, slots * sizeof(uint32_t), s));
@@ -563,20 +540,12 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     RL_HIP(c, hipGetLastError());
     RL_HIP(c, hipEventRecord(c->ev1, s));
     // dense requests in (source, j) order
-    uint64_t* found_wide = nullptr;
-    uint64_t* req_off = nullptr;
-    RL_TRY(dalloc(c, &found_wide, n + 1));
-    RL_TRY(dalloc(c, &req_off, n + 1));
-    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->found, found_wide);
-    e = hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, found_wide, req_off, static_cast<int>(n) + 1, s);
-    if (e != hipSuccess) {
-        dfree(&found_wide);
-        dfree(&req_off);
-        return hip_fail(c, e, "request offset scan");
-    }
+    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->found, c->wide);
+    RL_HIP(c, hipGetLastError());
+    RL_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, c->wide, c->req_off, static_cast<int>(n) + 1, s));
     uint64_t nreq = 0;
     unsigned long long hc[4] = { 0, 0, 0, 0 };
-    RL_HIP(c, hipMemcpyAsync(&nreq, req_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipMemcpyAsync(&nreq, c->req_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     RL_HIP(c, hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
     RL_HIP(c, hipStreamSynchronize(s));
     float ms = 0.0f;
@@ -592,14 +561,11 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     }
     c->nreq = nreq;
     if (nreq > 0) {
-        k_emit_requests<<<grid_for(n), kBlock, 0, s>>>(n, c->found, req_off, c->slot_off, c->slot_target, c->signal,
+        k_emit_requests<<<grid_for(n), kBlock, 0, s>>>(n, c->found, c->req_off, c->slot_off, c->slot_target, c->signal,
                                                         c->req_target, c->req_source, c->req_signal, c->req_j);
         RL_HIP(c, hipGetLastError());
     }
     RL_HIP(c, hipStreamSynchronize(s));
-    dfree(&found_wide);
-    dfree(&req_off);
-    dfree(&counters);
     c->last_tests = hc[0];
     c->last_tests_ref = hc[1];
     c->last_searches = hc[2];
