@@ -33,10 +33,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def run_precision(net, precision, steps, warmup, seed, d_vac):
+def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     from relearn_amd import BarnesHutHIP
 
-    g = BarnesHutHIP(net.box_min, net.box_max, theta=0.3, sigma=750.0, seed=seed, precision=precision)
+    g = BarnesHutHIP(net.box_min, net.box_max, theta=0.3, sigma=750.0, seed=seed, precision=precision, flags=flags)
     t0 = time.perf_counter()
     g.init(net.xyz, net.signal)
     build_s = time.perf_counter() - t0
@@ -124,6 +124,8 @@ def main(argv=None):
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32_mixed"], default=None)
+    ap.add_argument("--kernel", choices=["coop", "lane"], default="coop",
+                    help="search kernel: wavefront-cooperative (default) or one lane per source in list order")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -131,7 +133,7 @@ def main(argv=None):
               "(the sharded design is documented, not built)", file=sys.stderr)
         return 2
 
-    from relearn_amd import RL_PREC_FP32_MIXED, RL_PREC_FP64
+    from relearn_amd import RL_FLAG_EXACT_ORDER, RL_PREC_FP32_MIXED, RL_PREC_FP64
     from relearn_amd.workloads import uniform_cube
     import torch
 
@@ -148,7 +150,8 @@ def main(argv=None):
     if args.only:
         precs = [p for p in precs if p[0] == args.only]
     for name, prec in precs:
-        res[name] = run_precision(net, prec, args.steps, args.warmup, args.seed + 1, d_vac)
+        res[name] = run_precision(net, prec, args.steps, args.warmup, args.seed + 1, d_vac,
+                                  flags=RL_FLAG_EXACT_ORDER if args.kernel == "lane" else 0)
         r = res[name]
         log(f"{name}: {r['searches_per_s']:.4g} searches/s whole step, {r['ms_per_step']:.1f} ms/step, "
             f"search kernel {r['search_kernel_ms']:.1f} ms, phases {['%.3f' % x for x in r['phase_s']]} s")

@@ -38,6 +38,10 @@ enum {
 };
 
 enum {
+    RL_FLAG_EXACT_ORDER = 1,  /* per-source search with sums in the CPU path's list order (slower) */
+};
+
+enum {
     RL_SIGNAL_EXCITATORY = 0,
     RL_SIGNAL_INHIBITORY = 1,
 };
@@ -53,7 +57,7 @@ typedef struct {
     uint64_t seed;       /* seed of the counter-based draw stream */
     int32_t precision;   /* RL_PREC_* */
     int32_t device;      /* HIP device ordinal */
-    int32_t flags;       /* reserved, 0 */
+    int32_t flags;       /* RL_FLAG_* bits */
     int32_t reserved;
 } rl_bh_config;
 

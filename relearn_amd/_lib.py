@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "librelearn_bh.so")
 RL_OK = 0
 RL_PREC_FP64 = 0
 RL_PREC_FP32_MIXED = 1
+RL_FLAG_EXACT_ORDER = 1
 
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)
@@ -150,7 +151,7 @@ class BarnesHutHIP:
     """
 
     def __init__(self, box_min, box_max, theta=0.3, sigma=750.0, mu=0.0, seed=0,
-                 precision=RL_PREC_FP64, device=0):
+                 precision=RL_PREC_FP64, device=0, flags=0):
         L = load()
         cfg = Config()
         for d in range(3):
@@ -162,6 +163,7 @@ class BarnesHutHIP:
         cfg.seed = seed
         cfg.precision = precision
         cfg.device = device
+        cfg.flags = flags
         h = C.c_void_p()
         st = L.rl_bh_create(C.byref(cfg), C.byref(h))
         if st != 0:

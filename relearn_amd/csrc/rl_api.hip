@@ -78,6 +78,18 @@ void free_all(rl_bh_ctx* c) {
     dfree(&c->c_idx2);
     dfree(&c->c_g);
     dfree(&c->c_g2);
+    dfree(&c->coop_flag);
+    dfree(&c->coop_work);
+    dfree(&c->coop_nwork);
+    dfree(&c->coop_cursor);
+    dfree(&c->coop_overflow);
+    dfree(&c->redo);
+    dfree(&c->coop_node);
+    dfree(&c->coop_parent);
+    dfree(&c->coop_fexp);
+    dfree(&c->coop_mass);
+    c->coop_cap = 0;
+    c->coop_blocks = 0;
     c->commit_cap = 0;
     if (c->tmp != nullptr) {
         (void)hipFree(c->tmp);
@@ -218,6 +230,23 @@ int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signa
     RL_TRY(dalloc(c, &c->flags, 4));
     RL_TRY(dalloc(c, &c->found, n));
     RL_TRY(dalloc(c, &c->searched, n));
+    RL_TRY(dalloc(c, &c->coop_flag, n));
+    RL_TRY(dalloc(c, &c->coop_work, n));
+    RL_TRY(dalloc(c, &c->coop_nwork, 1));
+    RL_TRY(dalloc(c, &c->coop_cursor, 10));
+    RL_TRY(dalloc(c, &c->coop_overflow, n));
+    RL_TRY(dalloc(c, &c->redo, n));
+    {
+        int cus = 0;
+        RL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
+        c->coop_blocks = static_cast<uint32_t>(std::max(cus, 1) * 16);
+        c->coop_cap = 2048;
+        const size_t slots = static_cast<size_t>(c->coop_blocks) * c->coop_cap;
+        RL_TRY(dalloc(c, &c->coop_node, slots));
+        RL_TRY(dalloc(c, &c->coop_parent, slots));
+        RL_TRY(dalloc(c, &c->coop_fexp, slots));
+        RL_TRY(dalloc(c, &c->coop_mass, slots));
+    }
     hipStream_t s = c->stream;
     if (n > 0) {
         RL_HIP(c, hipMemcpyAsync(c->signal, signal, n, hipMemcpyHostToDevice, s));

@@ -97,6 +97,7 @@ struct rl_bh_ctx {
     uint64_t last_searches = 0;
     uint64_t last_tests = 0;
     uint64_t last_tests_ref = 0;
+    uint64_t last_lane_sources = 0;  // sources handled by the per-lane kernel in the last search
 
     // persistent scratch (grown on demand, never per call)
     uint64_t* wide = nullptr;        // n + 1
@@ -111,6 +112,19 @@ struct rl_bh_ctx {
     uint32_t* c_idx2 = nullptr;
     uint32_t* c_g = nullptr;
     uint32_t* c_g2 = nullptr;
+    // cooperative search
+    uint32_t* coop_flag = nullptr;   // n
+    uint32_t* coop_work = nullptr;   // n
+    int* coop_nwork = nullptr;
+    unsigned long long* coop_cursor = nullptr;  // 10
+    uint32_t* coop_overflow = nullptr;  // n
+    uint32_t* redo = nullptr;           // n
+    uint32_t* coop_node = nullptr;   // blocks * cap
+    uint32_t* coop_parent = nullptr;
+    uint32_t* coop_fexp = nullptr;
+    double* coop_mass = nullptr;
+    uint32_t coop_cap = 0;
+    uint32_t coop_blocks = 0;
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
     rl::Neurons neurons;
@@ -155,8 +169,11 @@ int export_tree(rl_bh_ctx* c, uint8_t* level, uint8_t* nchild, uint32_t* link, u
                 double* box_hi, double* side, uint32_t* cnt_exc, uint32_t* cnt_inh, uint8_t* has_exc, uint8_t* has_inh,
                 double* pos_exc, double* pos_inh);
 
-// search (rl_search.hip)
+// search (rl_search.hip, rl_search_coop.hip)
 int find_targets(rl_bh_ctx* c, uint32_t step);
+struct SearchArgs;
+int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow);
+int compact_active(rl_bh_ctx* c, uint64_t* nwork);
 
 // commit (rl_commit.hip)
 int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* created);

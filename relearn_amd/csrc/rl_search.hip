@@ -27,137 +27,16 @@
 //               those of the FP64 path; a list whose FP32 sum is 0 is redone in FP64.
 #include <hipcub/hipcub.hpp>
 
-#include "rl_device.h"
-#include "rl_internal.h"
+#include "rl_search_eval.h"
 
 namespace rl {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kNone = 0xFFFFFFFFU;
 
 inline unsigned grid_for(uint64_t n, int block = kBlock) {
     const uint64_t g = (n + block - 1) / block;
     return static_cast<unsigned>(g == 0 ? 1 : g);
-}
-
-struct SearchArgs {
-    uint64_t n;
-    uint64_t m;          // tree nodes
-    const uint32_t* order;
-    const double* pos;
-    const uint32_t* qpos;
-    const uint8_t* signal;
-    const uint8_t* enabled;
-    const uint32_t* vac_axon;
-    const uint64_t* slot_off;
-    const double4* rec64[2];
-    const uint4* rec32[2];
-    const uint2* meta[2];
-    double theta;
-    double scale64;      // -1 / sigma^2
-    double mu;
-    float theta2f;       // theta^2
-    float k2f;           // -log2(e) / sigma^2
-    float sigma_inv2f;   // 1 / sigma^2
-    float muf;
-    float inv_q[3];      // micrometres per fixed-point unit
-    float guard;         // relative guard band of the FP32 acceptance test
-    uint64_t seed;
-    uint32_t step;
-    uint32_t* slot_target;
-    uint32_t* slot_tests;
-    uint32_t* found;
-    uint32_t* searched;
-    unsigned long long* counters;  // [0] executed tests, [1] reference tests, [2] searches
-};
-
-struct Lane {
-    double px, py, pz;
-    int32_t qx, qy, qz;
-    uint32_t src;
-    int sig;
-};
-
-enum : int { kDiscard = 0, kAccept = 1, kExpand = 2 };
-enum : int { kGauss = 0, kInverse = 1 };
-
-struct Eval {
-    int status;
-    bool exact;   // d64 valid (FP64 decision)
-    double d64;
-    float d2f;
-};
-
-// Exact FP64 acceptance test (BarnesHutBase.h:72-114).
-__device__ __forceinline__ Eval eval_exact(const SearchArgs& a, const Lane& L, uint32_t node, uint32_t cnt, bool leaf) {
-    Eval e;
-    e.exact = true;
-    e.d2f = 0.0f;
-    const double4 r = a.rec64[L.sig][node];
-    e.d64 = hypot3_exact(r.x - L.px, r.y - L.py, r.z - L.pz);
-    (void)cnt;
-    if (e.d64 == 0.0) {
-        e.status = kDiscard;
-    } else if (leaf) {
-        e.status = kAccept;
-    } else {
-        e.status = (r.w < (a.theta * e.d64)) ? kAccept : kExpand;
-    }
-    return e;
-}
-
-__device__ __forceinline__ Eval eval_mixed(const SearchArgs& a, const Lane& L, uint32_t node, uint32_t cnt, bool leaf) {
-    const uint4 q = a.rec32[L.sig][node];
-    const int32_t ix = static_cast<int32_t>(q.x) - L.qx;
-    const int32_t iy = static_cast<int32_t>(q.y) - L.qy;
-    const int32_t iz = static_cast<int32_t>(q.z) - L.qz;
-    if ((ix | iy | iz) == 0) {
-        return eval_exact(a, L, node, cnt, leaf);  // possible exact coincidence: decide in FP64
-    }
-    const float dx = static_cast<float>(ix) * a.inv_q[0];
-    const float dy = static_cast<float>(iy) * a.inv_q[1];
-    const float dz = static_cast<float>(iz) * a.inv_q[2];
-    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-    Eval e;
-    e.exact = false;
-    e.d64 = 0.0;
-    e.d2f = d2;
-    if (leaf) {
-        e.status = kAccept;
-        return e;
-    }
-    const float side = __uint_as_float(q.w);
-    const float s2 = side * side;
-    const float g = __builtin_fmaf(a.theta2f, d2, -s2);
-    if (fabsf(g) <= a.guard * s2) {
-        return eval_exact(a, L, node, cnt, leaf);
-    }
-    e.status = g > 0.0f ? kAccept : kExpand;
-    return e;
-}
-
-template <bool kMixed>
-__device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e, uint32_t cnt, int kind) {
-#pragma clang fp contract(off)
-    if (kind == kInverse) {
-        return static_cast<double>(cnt) / e.d64;
-    }
-    if (!kMixed || e.exact) {
-        // GaussianDistributionKernel::get_probability (source/algorithm/Kernel/Gaussian.h:62-67) times
-        // the vacant count (Kernel.h:74)
-        const double numerator = (e.d64 - a.mu) * (e.d64 - a.mu);
-        const double exponent = numerator * a.scale64;
-        return (exp(exponent) * 1.0) * static_cast<double>(cnt);
-    }
-    float x;
-    if (a.muf == 0.0f) {
-        x = e.d2f * a.k2f;
-    } else {
-        const float d = sqrtf(e.d2f) - a.muf;
-        x = d * d * a.k2f;
-    }
-    return static_cast<double>(exp2f(x)) * static_cast<double>(cnt);
 }
 
 struct SumVisitor {
@@ -195,7 +74,7 @@ struct PickVisitor {
 template <bool kMixed, typename V>
 __device__ uint32_t traverse(const SearchArgs& a, const Lane& L, uint32_t root, int kind, uint32_t* stk,
                              uint32_t stride, V& v) {
-    const uint2* __restrict__ meta = a.meta[L.sig];
+    const uint2* __restrict__ meta = L.meta;
     const uint2 mr = meta[root];
     if (mr.x == 0) {
         return 0;
@@ -211,25 +90,39 @@ __device__ uint32_t traverse(const SearchArgs& a, const Lane& L, uint32_t root, 
         }
         return tests;
     }
-    int sp = 1;
-    stk[0] = ((mr.y & kLinkMask) << 4U) | rnc;
-    while (sp > 0) {
-        const uint32_t top = stk[(sp - 1) * stride];
+    // stack of (first child << 4 | children left); the top entry lives in a register and only the
+    // entries below it are in LDS, so a pop touches LDS only when the top range is used up
+    uint32_t top = ((mr.y & kLinkMask) << 4U) | rnc;
+    int sp = 0;
+    while (true) {
+        if (top == 0) {
+            if (sp == 0) {
+                break;
+            }
+            sp--;
+            top = stk[sp * stride];
+        }
         const uint32_t left = (top & 15U) - 1U;
         const uint32_t node = (top >> 4U) + left;
-        if (left == 0) {
-            sp--;
-        } else {
-            stk[(sp - 1) * stride] = (top & ~15U) | left;
-        }
+        top = left == 0 ? 0U : ((top & ~15U) | left);
         tests++;
+        // issue the count/link and the position loads together
         const uint2 m = meta[node];
+        double4 r64 = make_double4(0.0, 0.0, 0.0, 0.0);
+        uint4 r32 = make_uint4(0U, 0U, 0U, 0U);
+        if (kMixed) {
+            r32 = L.rec32[node];
+            __asm__ volatile("" ::"v"(r32.x));  // keep the load ahead of the count test
+        } else {
+            r64 = L.rec64[node];
+            __asm__ volatile("" ::"v"(r64.x));
+        }
         if (m.x == 0) {
             continue;
         }
         const uint32_t nc = m.y >> kLinkBits;
         const bool leaf = nc == 0;
-        const Eval e = kMixed ? eval_mixed(a, L, node, m.x, leaf) : eval_exact(a, L, node, m.x, leaf);
+        const Eval e = kMixed ? eval_mixed_rec(a, L, r32, node, leaf) : eval_exact_rec(a, L, r64, leaf);
         if (e.status == kDiscard) {
             continue;
         }
@@ -238,8 +131,11 @@ __device__ uint32_t traverse(const SearchArgs& a, const Lane& L, uint32_t root, 
             v(node, self ? 0.0 : probability<kMixed>(a, e, m.x, kind));
             continue;
         }
-        stk[sp * stride] = ((m.y & kLinkMask) << 4U) | nc;
-        sp++;
+        if (top != 0) {
+            stk[sp * stride] = top;
+            sp++;
+        }
+        top = ((m.y & kLinkMask) << 4U) | nc;
     }
     return tests;
 }
@@ -250,21 +146,29 @@ struct ListSum {
     uint32_t tests_ref;    // tests of one reference traversal
     uint32_t tests_exec;   // tests executed here
     bool exact;            // list probabilities from the FP64 formulas
+    bool redo;             // the Gaussian sum underflowed: the source goes to the fallback launch
     int kind;
 };
 
-// Pass 1 on a subtree: probability total with the reference's fallbacks.
-template <bool kMixed>
+// Pass 1 on a subtree: probability total.  The reference's fallbacks (FP64 re-evaluation of an
+// FP32 sum of 0, then n/r when the Gaussian sum is 0) are compiled only into the fallback
+// instantiation, which keeps the register footprint of the main launch small.
+template <bool kMixed, bool kFallback>
 __device__ ListSum list_sum(const SearchArgs& a, const Lane& L, uint32_t root, uint32_t* stk, uint32_t stride) {
     ListSum ls;
     SumVisitor v;
     ls.tests_ref = traverse<kMixed>(a, L, root, kGauss, stk, stride, v);
     ls.tests_exec = ls.tests_ref;
     ls.exact = !kMixed;
+    ls.redo = false;
     ls.kind = kGauss;
     ls.sum = v.sum;
     ls.count = v.count;
     if (v.count > 0 && v.sum == 0.0) {
+        if (!kFallback) {
+            ls.redo = true;
+            return ls;
+        }
         if (kMixed) {
             SumVisitor w;
             ls.tests_exec += traverse<false>(a, L, root, kGauss, stk, stride, w);
@@ -282,11 +186,11 @@ __device__ ListSum list_sum(const SearchArgs& a, const Lane& L, uint32_t root, u
     return ls;
 }
 
-template <bool kMixed>
+template <bool kMixed, bool kFallback>
 __device__ uint32_t list_pick(const SearchArgs& a, const Lane& L, uint32_t root, const ListSum& ls, PickVisitor& pv,
                               uint32_t* stk, uint32_t stride) {
     uint32_t t;
-    if (ls.exact) {
+    if (kFallback && ls.exact) {
         t = traverse<false>(a, L, root, ls.kind, stk, stride, pv);
     } else {
         t = traverse<kMixed>(a, L, root, ls.kind, stk, stride, pv);
@@ -300,11 +204,11 @@ __device__ uint32_t list_pick(const SearchArgs& a, const Lane& L, uint32_t root,
     return t;
 }
 
-template <bool kMixed>
+template <bool kMixed, bool kFallback>
 __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
     extern __shared__ uint32_t lds_stack[];
     const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (t >= a.n) {
+    if (t >= a.nwork) {
         return;
     }
     uint32_t* stk = lds_stack + threadIdx.x;
@@ -320,6 +224,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
     Lane L;
     L.src = src;
     L.sig = a.signal[src] == 0 ? 0 : 1;
+    lane_bind(a, L);
     L.px = a.pos[src];
     L.py = a.pos[a.n + src];
     L.pz = a.pos[2 * a.n + src];
@@ -332,7 +237,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
     }
     uint64_t exec = 0;
     uint64_t ref = 0;
-    const uint2 mroot = a.meta[L.sig][0];
+    const uint2 mroot = L.meta[0];
     if ((mroot.y >> kLinkBits) == 0) {
         // the root is a leaf (one neuron): it holds the source, so no target (BarnesHutBase.h:231-237)
         const uint32_t nid = mroot.y & kLinkMask;
@@ -352,7 +257,11 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
         return;
     }
     // level 1: one list for all axons of the neuron
-    const ListSum l1 = list_sum<kMixed>(a, L, 0, stk, stride);
+    const ListSum l1 = list_sum<kMixed, kFallback>(a, L, 0, stk, stride);
+    if (l1.redo) {
+        a.redo[atomicAdd(a.n_redo, 1U)] = src;
+        return;
+    }
     exec += l1.tests_exec;
     if (l1.count == 0 || l1.sum == 0.0) {
         a.slot_tests[slot0] = l1.tests_ref;
@@ -372,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
             pv.pick[k] = kNone;
             pv.r[k] = k < pv.nd ? draw_to_uniform(pick_draw(a.seed, src, a.step, j0 + k, 1U), next1) : 0.0;
         }
-        exec += list_pick<kMixed>(a, L, 0, l1, pv, stk, stride);
+        exec += list_pick<kMixed, kFallback>(a, L, 0, l1, pv, stk, stride);
 #pragma unroll
         for (int k = 0; k < kDrawsPerPass; k++) {
             if (k < pv.nd) {
@@ -393,12 +302,16 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
                 ok = false;  // unreachable for a consistent tree; keeps every access in bounds
                 break;
             }
-            const uint2 m = a.meta[L.sig][node];
+            const uint2 m = L.meta[node];
             if ((m.y >> kLinkBits) == 0) {
                 node = m.y & kLinkMask;  // leaf: the target neuron
                 break;
             }
-            const ListSum ls = list_sum<kMixed>(a, L, node, stk, stride);
+            const ListSum ls = list_sum<kMixed, kFallback>(a, L, node, stk, stride);
+            if (ls.redo) {
+                a.redo[atomicAdd(a.n_redo, 1U)] = src;
+                return;
+            }
             exec += ls.tests_exec;
             tests_j += ls.tests_ref;
             if (ls.count == 0 || ls.sum == 0.0) {
@@ -413,7 +326,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
                 pv.r[k] = 0.0;
             }
             pv.r[0] = draw_to_uniform(pick_draw(a.seed, src, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
-            exec += list_pick<kMixed>(a, L, node, ls, pv, stk, stride);
+            exec += list_pick<kMixed, kFallback>(a, L, node, ls, pv, stk, stride);
             node = pv.pick[0];
         }
         a.slot_tests[slot0 + j] = tests_j;
@@ -460,7 +373,35 @@ __global__ void k_emit_requests(uint64_t n, const uint32_t* __restrict__ found, 
     }
 }
 
+__global__ void k_active_flags(uint64_t n, const uint32_t* __restrict__ order, const uint8_t* __restrict__ enabled,
+                               const uint32_t* __restrict__ vac, uint32_t* __restrict__ flag) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t < n) {
+        const uint32_t src = order[t];
+        flag[t] = (enabled[src] != 0 && vac[src] != 0) ? 1U : 0U;
+    }
+}
+
 }  // namespace
+
+// Active sources (enabled, vacant axons > 0) in Morton order into c->coop_work.
+int compact_active(rl_bh_ctx* c, uint64_t* nwork) {
+    const uint64_t n = c->n;
+    hipStream_t s = c->stream;
+    k_active_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->coop_flag);
+    RL_HIP(c, hipGetLastError());
+    size_t tb = 0;
+    RL_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, tb, c->order, c->coop_flag, c->coop_work, c->coop_nwork,
+                                            static_cast<int>(n), s));
+    RL_TRY(ensure_tmp(c, tb));
+    RL_HIP(c, hipcub::DeviceSelect::Flagged(c->tmp, tb, c->order, c->coop_flag, c->coop_work, c->coop_nwork,
+                                            static_cast<int>(n), s));
+    int nw = 0;
+    RL_HIP(c, hipMemcpyAsync(&nw, c->coop_nwork, sizeof(int), hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipStreamSynchronize(s));
+    *nwork = static_cast<uint64_t>(nw);
+    return RL_OK;
+}
 
 int find_targets(rl_bh_ctx* c, uint32_t step) {
     if (!c->prepared) {
@@ -531,13 +472,47 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     a.searched = c->searched;
     a.counters = counters;
     const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
+    RL_HIP(c, hipMemsetAsync(c->found, 0, n * sizeof(uint32_t), s));
+    RL_HIP(c, hipMemsetAsync(c->searched, 0, n * sizeof(uint32_t), s));
     RL_HIP(c, hipEventRecord(c->ev0, s));
-    if (c->cfg.precision == RL_PREC_FP32_MIXED) {
-        k_search<true><<<grid_for(n), kBlock, lds, s>>>(a);
+    const bool exact = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0;
+    const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
+    uint64_t nlane = 0;
+    a.redo = c->redo;
+    a.n_redo = reinterpret_cast<unsigned int*>(c->coop_cursor + 9);
+    RL_HIP(c, hipMemsetAsync(c->coop_cursor, 0, 10 * sizeof(unsigned long long), s));
+    if (exact) {
+        // per-lane kernel over the active sources in Morton order, then the fallback launch
+        RL_TRY(compact_active(c, &nlane));
+        a.order = c->coop_work;
+        a.nwork = nlane;
+        if (nlane > 0) {
+            if (mixed) {
+                k_search<true, false><<<grid_for(nlane), kBlock, lds, s>>>(a);
+            } else {
+                k_search<false, false><<<grid_for(nlane), kBlock, lds, s>>>(a);
+            }
+            RL_HIP(c, hipGetLastError());
+        }
+        unsigned int nredo = 0;
+        RL_HIP(c, hipMemcpyAsync(&nredo, a.n_redo, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipStreamSynchronize(s));
+        nlane = nredo;
+        a.order = c->redo;
     } else {
-        k_search<false><<<grid_for(n), kBlock, lds, s>>>(a);
+        RL_TRY(launch_search_coop(c, a, &nlane));
+        a.order = c->coop_overflow;
     }
-    RL_HIP(c, hipGetLastError());
+    a.nwork = nlane;
+    if (nlane > 0) {
+        if (mixed) {
+            k_search<true, true><<<grid_for(nlane), kBlock, lds, s>>>(a);
+        } else {
+            k_search<false, true><<<grid_for(nlane), kBlock, lds, s>>>(a);
+        }
+        RL_HIP(c, hipGetLastError());
+    }
+    c->last_lane_sources = nlane;
     RL_HIP(c, hipEventRecord(c->ev1, s));
     // dense requests in (source, j) order
     k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->found, c->wide);

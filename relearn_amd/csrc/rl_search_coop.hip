@@ -1,0 +1,650 @@
+// Wavefront-cooperative Barnes-Hut target search.
+//
+// One wavefront handles one source neuron at a time.  The lanes form 8 groups of 8; in the
+// traversal a group takes one expanded node and each of its lanes tests one child, so a wavefront
+// tests the children of 8 nodes per step.  The traversal is breadth-first over exactly the nodes
+// that BarnesHutBase::get_nodes_to_consider (source/algorithm/BarnesHutInternal/BarnesHutBase.h:
+// 129-206) tests -- the children of every expanded node, each with test_acceptance_criterion
+// (:72-114) -- so the acceptance-test count equals the CPU path's.
+//
+// Expanded nodes get a slot in a per-wave table: node, first expanded-child slot and count, and
+// the node's mass = total probability of the accepted candidates below it.  A group sums the
+// probabilities of its accepted children with a DPP butterfly; after the traversal the masses of
+// expanded children are pulled into their parents, deepest level first.  The mass of a node is the
+// span its subtree occupies in the reference's candidate list (descending Morton order = children
+// visited in descending octant order), so a draw r is resolved top-down
+// (ProbabilityPicker::pick_target, source/util/ProbabilityPicker.h:46-112): at each expanded node
+// the children are scanned in descending octant order with the running sum of their masses and the
+// first child whose running sum reaches r is entered, or returned when it is an accepted candidate.
+// Up to 8 draws (one per group) are resolved at once.  The descents below a picked node
+// (BarnesHutBase::find_target_neuron, :223-255) usually accept all children of the node; a group
+// then does the whole level -- test, sum, draw, pick -- on its own, in parallel for all axons of the
+// neuron.  Levels that need expansion (rare) run through the general traversal afterwards.
+//
+// Sums are formed in a fixed order, so results are deterministic and independent of launch
+// geometry; they differ from the CPU's left-to-right sums only by rounding, so only draws within
+// ~1e-15 of a cumulative boundary can resolve differently.
+//
+// Work distribution: persistent waves; active sources in Morton order are split into one
+// contiguous segment per XCD so that an XCD's L2 caches one region of the tree; waves take sources
+// from their XCD's segment, then from the others.  A source whose slot table would overflow is handed
+// to the per-lane kernel.
+#include <hipcub/hipcub.hpp>
+
+#include "rl_search_eval.h"
+
+namespace rl {
+namespace {
+
+constexpr int kWave = 64;
+constexpr uint32_t kLevelCap = 1024;   // expanded nodes per level held in LDS
+constexpr int kXcds = 8;
+constexpr uint32_t kSlotNone = 0x0FFFFFFFU;
+
+struct CoopArgs {
+    SearchArgs s;
+    const uint32_t* work;
+    uint64_t seg_begin[kXcds];
+    uint64_t seg_end[kXcds];
+    unsigned long long* cursor;   // kXcds counters
+    uint32_t* t_node;             // per wave: cap slots
+    uint32_t* t_fexp;             // first expanded child slot | expanded-child count << 28
+    double* t_mass;
+    uint32_t cap;
+    uint32_t* overflow;
+    unsigned int* n_overflow;
+};
+
+struct CoopShared {
+    uint32_t lvl[2][kLevelCap];        // packed child range (meta.y) of the slots of a level
+    uint32_t lvl_start[kMaxLevels + 4];
+};
+
+__device__ __forceinline__ uint32_t lane() {
+    return __lane_id();
+}
+
+__device__ __forceinline__ uint32_t rank_below(uint64_t mask, uint32_t l) {
+    return static_cast<uint32_t>(__popcll(mask & ((1ULL << l) - 1ULL)));
+}
+
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 8 lanes of a group, valid in the group's first lane: ((p0+p1)+(p2+p3))+((p4+p5)+(p6+p7)).
+__device__ __forceinline__ double group_sum8(double v) {
+    v = v + dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = v + dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp_f64<0x104>(v);  // row_shl:4
+    return v;
+}
+
+// Running sum in descending child order within a group: lane j gets sum of lanes j..7 of its group.
+__device__ __forceinline__ double group_suffix8(double v) {
+    const uint32_t j = lane() & 7U;
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+        const double t = __shfl_down(v, d, kWave);
+        if (j + d < 8) {
+            v += t;
+        }
+    }
+    return v;
+}
+
+enum : int { kModeMixed = 0, kModeExact = 1 };
+
+template <bool kMixed>
+__device__ __forceinline__ int test_node(const SearchArgs& a, const Lane& L, uint32_t node, int mode, int kind,
+                                         double* p_out, uint32_t* meta_y) {
+    const uint2 m = L.meta[node];
+    const bool use_mixed = kMixed && mode == kModeMixed;
+    double4 r64 = make_double4(0.0, 0.0, 0.0, 0.0);
+    uint4 r32 = make_uint4(0U, 0U, 0U, 0U);
+    if (use_mixed) {
+        r32 = L.rec32[node];
+        __asm__ volatile("" ::"v"(r32.x));
+    } else {
+        r64 = L.rec64[node];
+        __asm__ volatile("" ::"v"(r64.x));
+    }
+    *p_out = 0.0;
+    *meta_y = m.y;
+    if (m.x == 0) {
+        return kDiscard;
+    }
+    const bool leaf = (m.y >> kLinkBits) == 0;
+    Eval e;
+    if (use_mixed) {
+        e = eval_mixed_rec(a, L, r32, node, leaf);
+    } else {
+        e = eval_exact_rec(a, L, r64, leaf);
+    }
+    if (e.status == kAccept) {
+        const bool self = leaf && (m.y & kLinkMask) == L.src;
+        if (!self) {
+            *p_out = use_mixed ? probability<true>(a, e, m.x, kind) : probability<false>(a, e, m.x, kind);
+        }
+    }
+    return e.status;
+}
+
+struct Bfs {
+    double total;
+    uint32_t count;
+    uint32_t tests;
+    bool overflow;
+};
+
+// Breadth-first traversal below `root` (inner, with vacant dendrites) filling the slot table at `base`.
+template <bool kMixed>
+__device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root, uint32_t root_meta_y, int mode, int kind,
+                   CoopShared& sh, uint64_t base) {
+    const SearchArgs& s = a.s;
+    const uint32_t ln = lane();
+    const uint32_t q = ln >> 3U, j = ln & 7U;
+    Bfs r{ 0.0, 0, 0, false };
+    if (ln == 0) {
+        a.t_node[base] = root;
+        sh.lvl[0][0] = root_meta_y;
+        sh.lvl_start[0] = 0;
+    }
+    __syncthreads();
+    uint32_t lb = 0, le = 1, nslots = 1, lev = 0, cur = 0;
+    while (lb < le) {
+        const uint32_t nb = nslots;
+        for (uint32_t kb = lb; kb < le; kb += 8) {
+            const uint32_t k = kb + q;
+            const bool gact = k < le;
+            const uint32_t packed = gact ? sh.lvl[cur][k - lb] : 0U;
+            const uint32_t fc = packed & kLinkMask;
+            const uint32_t nc = packed >> kLinkBits;
+            const bool act = gact && j < nc;
+            const uint32_t child = fc + j;
+            int st = kDiscard;
+            double p = 0.0;
+            uint32_t my = 0;
+            if (act) {
+                st = test_node<kMixed>(s, L, child, mode, kind, &p, &my);
+            }
+            r.tests += static_cast<uint32_t>(__popcll(__ballot(act)));
+            r.count += static_cast<uint32_t>(__popcll(__ballot(act && st == kAccept)));
+            const double gsum = group_sum8(p);
+            const uint64_t em = __ballot(act && st == kExpand);
+            const uint32_t ne = static_cast<uint32_t>(__popcll(em));
+            if (nslots + ne > a.cap || nslots + ne - nb > kLevelCap) {
+                r.overflow = true;
+                return r;
+            }
+            if (act && st == kExpand) {
+                const uint32_t slot = nslots + rank_below(em, ln);
+                a.t_node[base + slot] = child;
+                sh.lvl[cur ^ 1U][slot - nb] = my;
+            }
+            if (gact && j == 0) {
+                const uint32_t gem = static_cast<uint32_t>((em >> (8U * q)) & 0xFFULL);
+                const uint32_t first = nslots + rank_below(em, 8U * q);
+                a.t_mass[base + k] = gsum;
+                a.t_fexp[base + k] = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
+            }
+            nslots += ne;
+        }
+        __syncthreads();
+        lb = nb;
+        le = nslots;
+        cur ^= 1U;
+        lev++;
+        if (lev + 1 >= kMaxLevels + 4) {
+            r.overflow = true;
+            return r;
+        }
+        if (ln == 0) {
+            sh.lvl_start[lev] = lb;
+        }
+    }
+    __syncthreads();
+    // pull the masses of expanded children into their parents, deepest parents first
+    for (int t = static_cast<int>(lev) - 2; t >= 0; t--) {
+        const uint32_t sb = sh.lvl_start[t], se = sh.lvl_start[t + 1];
+        for (uint32_t i0 = sb; i0 < se; i0 += kWave) {
+            const uint32_t sl = i0 + ln;
+            if (sl < se) {
+                const uint32_t fx = a.t_fexp[base + sl];
+                const uint32_t ne = fx >> 28U;
+                if (ne != 0) {
+                    const uint32_t f = fx & kSlotNone;
+                    double sum = 0.0;
+                    for (uint32_t c = 0; c < ne; c++) {
+                        sum += a.t_mass[base + f + c];
+                    }
+                    a.t_mass[base + sl] += sum;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    r.total = a.t_mass[base];
+    return r;
+}
+
+// Resolves the draws of the active groups top-down through the slot table (one draw per group).
+// Returns the picked node in every lane of the group, kNone when the list had no positive mass.
+template <bool kMixed>
+__device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool gact, int mode, int kind,
+                             uint64_t base, uint32_t* tests) {
+    const SearchArgs& s = a.s;
+    const uint32_t ln = lane();
+    const uint32_t q = ln >> 3U, j = ln & 7U;
+    const uint32_t g0 = 8U * q;
+    uint32_t slot = 0;
+    bool done = !gact;
+    uint32_t res = kNone;
+    for (uint32_t guard = 0; guard < kMaxLevels + 2; guard++) {
+        if (__ballot(!done) == 0) {
+            break;
+        }
+        uint32_t fx = 0, fc = 0, nc = 0;
+        if (!done) {
+            const uint32_t x = a.t_node[base + slot];
+            fx = a.t_fexp[base + slot];
+            const uint32_t my = L.meta[x].y;
+            fc = my & kLinkMask;
+            nc = my >> kLinkBits;
+        }
+        const bool act = !done && j < nc;
+        int st = kDiscard;
+        double p = 0.0;
+        uint32_t my = 0;
+        if (act) {
+            st = test_node<kMixed>(s, L, fc + j, mode, kind, &p, &my);
+        }
+        *tests += static_cast<uint32_t>(__popcll(__ballot(act)));
+        const uint64_t em = __ballot(act && st == kExpand);
+        const uint32_t gem = static_cast<uint32_t>((em >> g0) & 0xFFULL);
+        double mass = 0.0;
+        if (st == kAccept) {
+            mass = p;
+        } else if (st == kExpand) {
+            const uint32_t f = fx & kSlotNone;
+            const uint32_t rk = static_cast<uint32_t>(__popc(gem & ((1U << j) - 1U)));
+            if (f != kSlotNone && f + rk < a.cap) {
+                mass = a.t_mass[base + f + rk];
+            }
+        }
+        const double cum = group_suffix8(mass);
+        const double cum_next_raw = __shfl_down(cum, 1, kWave);
+        const double cum_next = (j + 1 < 8) ? cum_next_raw : 0.0;
+        const uint32_t gcm = static_cast<uint32_t>((__ballot(act && mass > 0.0 && cum >= rr) >> g0) & 0xFFULL);
+        const uint32_t gpm = static_cast<uint32_t>((__ballot(act && mass > 0.0) >> g0) & 0xFFULL);
+        int jstar = -1;
+        if (gcm != 0) {
+            jstar = 31 - __clz(static_cast<int>(gcm));
+        } else if (gpm != 0) {
+            jstar = __ffs(static_cast<int>(gpm)) - 1;
+        }
+        const uint32_t src_lane = g0 + static_cast<uint32_t>(jstar < 0 ? 0 : jstar);
+        const int stj = __shfl(st, static_cast<int>(src_lane), kWave);
+        const double before = __shfl(cum_next, static_cast<int>(src_lane), kWave);
+        if (!done) {
+            if (jstar < 0) {
+                done = true;
+                res = kNone;
+            } else if (stj == kAccept) {
+                done = true;
+                res = fc + static_cast<uint32_t>(jstar);
+            } else {
+                const uint32_t f = fx & kSlotNone;
+                rr = rr - before;
+                slot = f + static_cast<uint32_t>(__popc(gem & ((1U << jstar) - 1U)));
+                if (f == kSlotNone || slot >= a.cap) {
+                    done = true;
+                    res = kNone;
+                }
+            }
+        }
+    }
+    return res;
+}
+
+struct ListResult {
+    double total;
+    uint32_t count;
+    uint32_t tests_ref;
+    uint32_t tests_exec;
+    int mode;
+    int kind;
+    bool overflow;
+};
+
+// Traversal of the subtree of `root` with the reference's fallbacks (FP64 re-evaluation of an FP32
+// sum of 0, then n/r when the Gaussian sum is 0: Kernel::create_probability_interval, Kernel.h:109-126).
+template <bool kMixed>
+__device__ ListResult list_of(const CoopArgs& a, const Lane& L, uint32_t root, CoopShared& sh, uint64_t base) {
+    ListResult lr{ 0.0, 0, 0, 0, kMixed ? kModeMixed : kModeExact, kGauss, false };
+    const uint2 mr = L.meta[root];
+    if (mr.x == 0) {
+        return lr;
+    }
+    Bfs b = bfs<kMixed>(a, L, root, mr.y, lr.mode, kGauss, sh, base);
+    lr.tests_ref = b.tests;
+    lr.tests_exec = b.tests;
+    if (b.overflow) {
+        lr.overflow = true;
+        return lr;
+    }
+    lr.total = b.total;
+    lr.count = b.count;
+    if (b.count > 0 && b.total == 0.0) {
+        if (kMixed) {
+            b = bfs<kMixed>(a, L, root, mr.y, kModeExact, kGauss, sh, base);
+            lr.tests_exec += b.tests;
+            lr.mode = kModeExact;
+            lr.total = b.total;
+            if (b.overflow) {
+                lr.overflow = true;
+                return lr;
+            }
+        }
+        if (lr.total == 0.0) {
+            b = bfs<kMixed>(a, L, root, mr.y, kModeExact, kInverse, sh, base);
+            lr.tests_exec += b.tests;
+            lr.mode = kModeExact;
+            lr.kind = kInverse;
+            lr.total = b.total;
+            if (b.overflow) {
+                lr.overflow = true;
+                return lr;
+            }
+        }
+    }
+    return lr;
+}
+
+// Per-draw descent state, identical in the 8 lanes of the draw's group.
+enum : int { kDrawActive = 0, kDrawDone = 1, kDrawFailed = 2, kDrawSlow = 3, kDrawIdle = 4 };
+
+template <bool kMixed>
+__global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
+    __shared__ CoopShared sh;
+    const SearchArgs& s = a.s;
+    const uint32_t ln = lane();
+    const uint32_t q = ln >> 3U, j = ln & 7U;
+    const uint32_t g0 = 8U * q;
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * a.cap;
+    int seg = static_cast<int>(blockIdx.x % kXcds);
+    int tried = 0;
+    unsigned long long acc_exec = 0, acc_ref = 0, acc_searches = 0;
+    while (true) {
+        uint64_t w = 0;
+        if (ln == 0) {
+            w = a.seg_begin[seg] + atomicAdd(&a.cursor[seg], 1ULL);
+        }
+        w = __shfl(w, 0, kWave);
+        if (w >= a.seg_end[seg]) {
+            if (++tried >= kXcds) {
+                break;
+            }
+            seg = (seg + 1) % kXcds;
+            continue;
+        }
+        const uint32_t src = __builtin_amdgcn_readfirstlane(a.work[w]);
+        const uint32_t na = __builtin_amdgcn_readfirstlane(s.vac_axon[src]);
+        const uint64_t slot0 = s.slot_off[src];
+        Lane L;
+        L.src = src;
+        L.sig = s.signal[src] == 0 ? 0 : 1;
+        lane_bind(s, L);
+        L.px = s.pos[src];
+        L.py = s.pos[s.n + src];
+        L.pz = s.pos[2 * s.n + src];
+        if (kMixed) {
+            L.qx = static_cast<int32_t>(s.qpos[src]);
+            L.qy = static_cast<int32_t>(s.qpos[s.n + src]);
+            L.qz = static_cast<int32_t>(s.qpos[2 * s.n + src]);
+        } else {
+            L.qx = L.qy = L.qz = 0;
+        }
+        uint64_t exec = 0, ref = 0;
+        bool overflow = false;
+        uint32_t found = 0, searched = 0;
+        const ListResult l1 = list_of<kMixed>(a, L, 0, sh, base);
+        exec += l1.tests_exec;
+        if (l1.overflow) {
+            overflow = true;
+        } else if (l1.count == 0 || l1.total == 0.0) {
+            if (ln == 0) {
+                s.slot_tests[slot0] = l1.tests_ref;
+            }
+            ref += l1.tests_ref;
+            searched = 1;
+        } else {
+            const double next1 = nextafter(l1.total, l1.total + 0.00001);
+            // level 1: all draws against the one slot table, 8 at a time
+            uint32_t nav_tests = 0;
+            for (uint32_t j0 = 0; j0 < na; j0 += 8) {
+                const bool gact = j0 + q < na;
+                const double r = gact ? draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + q, 1U), next1) : 0.0;
+                const uint32_t x = navigate<kMixed>(a, L, r, gact, l1.mode, l1.kind, base, &nav_tests);
+                if (gact && j == 0) {
+                    s.slot_target[slot0 + j0 + q] = x;
+                }
+            }
+            exec += nav_tests;
+            __syncthreads();
+            // descents, 8 axons at a time; draw j0+q is handled by group q
+            bool stop = false;
+            for (uint32_t j0 = 0; j0 < na && !stop && !overflow; j0 += 8) {
+                const uint32_t jd = j0 + q;
+                const bool gact = jd < na;
+                uint32_t node = gact ? s.slot_target[slot0 + jd] : kNone;
+                int state = gact ? kDrawActive : kDrawIdle;
+                uint32_t level = 2;
+                uint32_t tests_d = l1.tests_ref;
+                if (gact && node == kNone) {
+                    state = kDrawFailed;
+                }
+                // fast levels: the group tests the children of its node; no expansion expected
+                for (uint32_t guard = 0; guard < kMaxLevels + 2; guard++) {
+                    uint32_t my = 0;
+                    if (state == kDrawActive) {
+                        if (node >= s.m) {
+                            state = kDrawFailed;
+                        } else {
+                            const uint2 mn = L.meta[node];
+                            my = mn.y;
+                            if ((my >> kLinkBits) == 0) {
+                                node = my & kLinkMask;  // leaf: the target neuron
+                                state = kDrawDone;
+                            } else if (mn.x == 0) {
+                                state = kDrawFailed;  // no vacant dendrites: empty list
+                            }
+                        }
+                    }
+                    if (__ballot(state == kDrawActive) == 0) {
+                        break;
+                    }
+                    const bool act_g = state == kDrawActive;
+                    const uint32_t fc = my & kLinkMask;
+                    const uint32_t nc = act_g ? (my >> kLinkBits) : 0U;
+                    const bool act = act_g && j < nc;
+                    int st = kDiscard;
+                    double p = 0.0;
+                    uint32_t cy = 0;
+                    if (act) {
+                        st = test_node<kMixed>(s, L, fc + j, kMixed ? kModeMixed : kModeExact, kGauss, &p, &cy);
+                    }
+                    exec += static_cast<uint32_t>(__popcll(__ballot(act)));
+                    const uint32_t gexp = static_cast<uint32_t>((__ballot(act && st == kExpand) >> g0) & 0xFFULL);
+                    const uint32_t gacc = static_cast<uint32_t>((__ballot(act && st == kAccept) >> g0) & 0xFFULL);
+                    const double gsum = __shfl(group_sum8(p), static_cast<int>(g0), kWave);
+                    if (act_g) {
+                        if (gexp != 0 || (gacc != 0 && gsum == 0.0)) {
+                            state = kDrawSlow;  // needs the general traversal (expansion or fallback)
+                        } else {
+                            tests_d += nc;
+                            if (gacc == 0) {
+                                state = kDrawFailed;
+                            }
+                        }
+                    }
+                    const bool pick = state == kDrawActive;
+                    const double r = pick ? draw_to_uniform(pick_draw(s.seed, src, s.step, jd, level),
+                                                            nextafter(gsum, gsum + 0.00001))
+                                          : 0.0;
+                    const double pa = (act && st == kAccept) ? p : 0.0;
+                    const double cum = group_suffix8(pa);
+                    const uint32_t gcm = static_cast<uint32_t>((__ballot(pa > 0.0 && cum >= r) >> g0) & 0xFFULL);
+                    const uint32_t gpm = static_cast<uint32_t>((__ballot(pa > 0.0) >> g0) & 0xFFULL);
+                    if (pick) {
+                        int jstar = -1;
+                        if (gcm != 0) {
+                            jstar = 31 - __clz(static_cast<int>(gcm));
+                        } else if (gpm != 0) {
+                            jstar = __ffs(static_cast<int>(gpm)) - 1;
+                        }
+                        if (jstar < 0) {
+                            state = kDrawFailed;
+                        } else {
+                            node = fc + static_cast<uint32_t>(jstar);
+                            level++;
+                        }
+                    }
+                }
+                // general traversal for the draws that need it, in axon order
+                for (uint32_t g = 0; g < 8 && !overflow; g++) {
+                    const int sg = __shfl(state, static_cast<int>(8U * g), kWave);
+                    if (sg != kDrawSlow) {
+                        continue;
+                    }
+                    uint32_t nd = __shfl(node, static_cast<int>(8U * g), kWave);
+                    uint32_t lv = __shfl(level, static_cast<int>(8U * g), kWave);
+                    uint32_t td = __shfl(tests_d, static_cast<int>(8U * g), kWave);
+                    int res = kDrawSlow;
+                    for (uint32_t guard = 0; guard < kMaxLevels + 2 && res == kDrawSlow; guard++) {
+                        if (nd >= s.m) {
+                            res = kDrawFailed;
+                            break;
+                        }
+                        const uint2 m = L.meta[nd];
+                        if ((m.y >> kLinkBits) == 0) {
+                            nd = m.y & kLinkMask;
+                            res = kDrawDone;
+                            break;
+                        }
+                        const ListResult ls = list_of<kMixed>(a, L, nd, sh, base);
+                        exec += ls.tests_exec;
+                        td += ls.tests_ref;
+                        if (ls.overflow) {
+                            overflow = true;
+                            break;
+                        }
+                        if (ls.count == 0 || ls.total == 0.0) {
+                            res = kDrawFailed;
+                            break;
+                        }
+                        const double r = draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + g, lv),
+                                                         nextafter(ls.total, ls.total + 0.00001));
+                        uint32_t nt = 0;
+                        nd = navigate<kMixed>(a, L, r, q == 0, ls.mode, ls.kind, base, &nt);
+                        nd = __shfl(nd, 0, kWave);
+                        exec += nt;
+                        lv++;
+                        if (nd == kNone) {
+                            res = kDrawFailed;
+                        }
+                    }
+                    if (q == g) {
+                        node = nd;
+                        state = res;
+                        tests_d = td;
+                    }
+                }
+                if (overflow) {
+                    break;
+                }
+                // results in axon order, stopping at the first failure (find_target_neurons, :283-289)
+                for (uint32_t g = 0; g < 8 && j0 + g < na; g++) {
+                    const int sg = __shfl(state, static_cast<int>(8U * g), kWave);
+                    const uint32_t ng = __shfl(node, static_cast<int>(8U * g), kWave);
+                    const uint32_t tg = __shfl(tests_d, static_cast<int>(8U * g), kWave);
+                    searched++;
+                    ref += tg;
+                    if (ln == 0) {
+                        s.slot_tests[slot0 + j0 + g] = tg;
+                        s.slot_target[slot0 + j0 + g] = sg == kDrawDone ? ng : kNone;
+                    }
+                    if (sg != kDrawDone) {
+                        stop = true;
+                        break;
+                    }
+                    found++;
+                }
+            }
+        }
+        if (overflow) {
+            if (ln == 0) {
+                const unsigned int k = atomicAdd(a.n_overflow, 1U);
+                a.overflow[k] = src;
+            }
+        } else {
+            if (ln == 0) {
+                s.found[src] = found;
+                s.searched[src] = searched;
+            }
+            acc_exec += exec;
+            acc_ref += ref;
+            acc_searches += searched;
+        }
+        __syncthreads();
+    }
+    if (ln == 0) {
+        atomicAdd(&s.counters[0], acc_exec);
+        atomicAdd(&s.counters[1], acc_ref);
+        atomicAdd(&s.counters[2], acc_searches);
+    }
+}
+
+}  // namespace
+
+// Launches the cooperative search over the active sources; sources that overflow the slot table
+// are returned in c->coop_overflow (count in *n_overflow) for the per-lane kernel.
+int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow) {
+    hipStream_t s = c->stream;
+    uint64_t nw = 0;
+    RL_TRY(compact_active(c, &nw));
+    const int nwork = static_cast<int>(nw);
+    RL_HIP(c, hipMemsetAsync(c->coop_cursor, 0, 9 * sizeof(unsigned long long), s));
+    CoopArgs a{};
+    a.s = sa;
+    a.work = c->coop_work;
+    for (int x = 0; x < kXcds; x++) {
+        a.seg_begin[x] = static_cast<uint64_t>(nwork) * x / kXcds;
+        a.seg_end[x] = static_cast<uint64_t>(nwork) * (x + 1) / kXcds;
+    }
+    a.cursor = c->coop_cursor;
+    a.t_node = c->coop_node;
+    a.t_fexp = c->coop_fexp;
+    a.t_mass = c->coop_mass;
+    a.cap = c->coop_cap;
+    a.overflow = c->coop_overflow;
+    a.n_overflow = reinterpret_cast<unsigned int*>(c->coop_cursor + kXcds);
+    if (nwork > 0) {
+        if (c->cfg.precision == RL_PREC_FP32_MIXED) {
+            k_search_coop<true><<<c->coop_blocks, kWave, 0, s>>>(a);
+        } else {
+            k_search_coop<false><<<c->coop_blocks, kWave, 0, s>>>(a);
+        }
+        RL_HIP(c, hipGetLastError());
+    }
+    unsigned int nov = 0;
+    RL_HIP(c, hipMemcpyAsync(&nov, a.n_overflow, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipStreamSynchronize(s));
+    *n_overflow = nov;
+    return RL_OK;
+}
+
+}  // namespace rl

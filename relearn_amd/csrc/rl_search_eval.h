@@ -1,0 +1,153 @@
+// Acceptance test and probability of one octree node for one source, shared by the search kernels.
+#pragma once
+
+#include "rl_device.h"
+#include "rl_internal.h"
+
+namespace rl {
+
+constexpr uint32_t kNone = 0xFFFFFFFFU;
+
+struct SearchArgs {
+    uint64_t n;
+    uint64_t nwork;      // sources in `order` processed by the per-lane kernel
+    uint64_t m;          // tree nodes
+    const uint32_t* order;
+    const double* pos;
+    const uint32_t* qpos;
+    const uint8_t* signal;
+    const uint8_t* enabled;
+    const uint32_t* vac_axon;
+    const uint64_t* slot_off;
+    const double4* rec64[2];
+    const uint4* rec32[2];
+    const uint2* meta[2];
+    double theta;
+    double scale64;      // -1 / sigma^2
+    double mu;
+    float theta2f;       // theta^2
+    float k2f;           // -log2(e) / sigma^2
+    float sigma_inv2f;   // 1 / sigma^2
+    float muf;
+    float inv_q[3];      // micrometres per fixed-point unit
+    float guard;         // relative guard band of the FP32 acceptance test
+    uint64_t seed;
+    uint32_t step;
+    uint32_t* slot_target;
+    uint32_t* slot_tests;
+    uint32_t* found;
+    uint32_t* searched;
+    unsigned long long* counters;  // [0] executed tests, [1] reference tests, [2] searches
+    uint32_t* redo;                // sources whose Gaussian sum underflowed (fallback launch)
+    unsigned int* n_redo;
+};
+
+// Per-source state; the record pointers are resolved once for the source's signal type so that the
+// traversal loops index plain pointers instead of reloading them from the argument block.
+struct Lane {
+    double px, py, pz;
+    int32_t qx, qy, qz;
+    uint32_t src;
+    int sig;
+    const uint2* meta;
+    const double4* rec64;
+    const uint4* rec32;
+};
+
+__device__ __forceinline__ void lane_bind(const SearchArgs& a, Lane& L) {
+    L.meta = L.sig ? a.meta[1] : a.meta[0];
+    L.rec64 = L.sig ? a.rec64[1] : a.rec64[0];
+    L.rec32 = L.sig ? a.rec32[1] : a.rec32[0];
+}
+
+enum : int { kDiscard = 0, kAccept = 1, kExpand = 2 };
+enum : int { kGauss = 0, kInverse = 1 };
+
+struct Eval {
+    int status;
+    bool exact;   // d64 valid (FP64 decision)
+    double d64;
+    float d2f;
+};
+
+// Exact FP64 acceptance test (BarnesHutBase.h:72-114) on a loaded record (centroid, max edge).
+__device__ __forceinline__ Eval eval_exact_rec(const SearchArgs& a, const Lane& L, double4 r, bool leaf) {
+    Eval e;
+    e.exact = true;
+    e.d2f = 0.0f;
+    e.d64 = hypot3_exact(r.x - L.px, r.y - L.py, r.z - L.pz);
+    if (e.d64 == 0.0) {
+        e.status = kDiscard;
+    } else if (leaf) {
+        e.status = kAccept;
+    } else {
+        e.status = (r.w < (a.theta * e.d64)) ? kAccept : kExpand;
+    }
+    return e;
+}
+
+__device__ __forceinline__ Eval eval_exact(const SearchArgs& a, const Lane& L, uint32_t node, uint32_t cnt, bool leaf) {
+    (void)cnt;
+    return eval_exact_rec(a, L, L.rec64[node], leaf);
+}
+
+// FP32 test on the fixed-point record; decisions inside the guard band (and exact coincidences)
+// are taken by the FP64 test.
+__device__ __forceinline__ Eval eval_mixed_rec(const SearchArgs& a, const Lane& L, uint4 q, uint32_t node, bool leaf) {
+    const int32_t ix = static_cast<int32_t>(q.x) - L.qx;
+    const int32_t iy = static_cast<int32_t>(q.y) - L.qy;
+    const int32_t iz = static_cast<int32_t>(q.z) - L.qz;
+    if ((ix | iy | iz) == 0) {
+        return eval_exact(a, L, node, 0, leaf);
+    }
+    const float dx = static_cast<float>(ix) * a.inv_q[0];
+    const float dy = static_cast<float>(iy) * a.inv_q[1];
+    const float dz = static_cast<float>(iz) * a.inv_q[2];
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    Eval e;
+    e.exact = false;
+    e.d64 = 0.0;
+    e.d2f = d2;
+    if (leaf) {
+        e.status = kAccept;
+        return e;
+    }
+    const float side = __uint_as_float(q.w);
+    const float s2 = side * side;
+    const float g = __builtin_fmaf(a.theta2f, d2, -s2);
+    if (fabsf(g) <= a.guard * s2) {
+        return eval_exact(a, L, node, 0, leaf);
+    }
+    e.status = g > 0.0f ? kAccept : kExpand;
+    return e;
+}
+
+__device__ __forceinline__ Eval eval_mixed(const SearchArgs& a, const Lane& L, uint32_t node, uint32_t cnt, bool leaf) {
+    (void)cnt;
+    return eval_mixed_rec(a, L, L.rec32[node], node, leaf);
+}
+
+template <bool kMixed>
+__device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e, uint32_t cnt, int kind) {
+#pragma clang fp contract(off)
+    if (kind == kInverse) {
+        return static_cast<double>(cnt) / e.d64;
+    }
+    if (!kMixed || e.exact) {
+        // GaussianDistributionKernel::get_probability (source/algorithm/Kernel/Gaussian.h:62-67) times
+        // the vacant count (Kernel.h:74)
+        const double numerator = (e.d64 - a.mu) * (e.d64 - a.mu);
+        const double exponent = numerator * a.scale64;
+        return (exp(exponent) * 1.0) * static_cast<double>(cnt);
+    }
+    float x;
+    if (a.muf == 0.0f) {
+        x = e.d2f * a.k2f;
+    } else {
+        const float d = sqrtf(e.d2f) - a.muf;
+        x = d * d * a.k2f;
+    }
+    return static_cast<double>(exp2f(x)) * static_cast<double>(cnt);
+}
+
+}  // namespace rl

@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from relearn_amd import RL_PREC_FP32_MIXED, RL_PREC_FP64, BarnesHutHIP
+from relearn_amd import RL_FLAG_EXACT_ORDER, RL_PREC_FP32_MIXED, RL_PREC_FP64, BarnesHutHIP
 from relearn_amd.workloads import uniform_cube
 
 pytestmark = pytest.mark.gpu
@@ -84,12 +84,12 @@ def test_tree_rejects_bad_inputs():
         BarnesHutHIP((0, 0, 0), (10, 10, 10), theta=0.6)
 
 
-def _search_pair(oracle, net, seed, step=0, theta=0.3, sigma=750.0, enabled=None, precision=RL_PREC_FP64):
+def _search_pair(oracle, net, seed, step=0, theta=0.3, sigma=750.0, enabled=None, precision=RL_PREC_FP64, flags=0):
     t = oracle.Tree(net.xyz, net.box_min, net.box_max)
     t.update(net.vac_exc, net.vac_inh, enabled)
     ref = t.find_targets(net.signal, net.vac_axon, enabled, theta=theta, sigma=sigma, seed=seed, step=step,
                          threads=0)
-    g = BarnesHutHIP(net.box_min, net.box_max, theta=theta, sigma=sigma, seed=seed, precision=precision)
+    g = BarnesHutHIP(net.box_min, net.box_max, theta=theta, sigma=sigma, seed=seed, precision=precision, flags=flags)
     g.init(net.xyz, net.signal)
     if enabled is not None:
         g.set_enabled(enabled)
@@ -127,11 +127,12 @@ def _check_search(ref, tgt, tests, cnt, net):
     return bad, int(near.sum())
 
 
+@pytest.mark.parametrize("flags", [0, RL_FLAG_EXACT_ORDER])
 @pytest.mark.parametrize("n,side,seed", [(1000, 1000.0, 0), (30_000, 10_000.0 * (0.03 ** (1 / 3)), 5)])
-def test_search_parity_fp64(oracle, n, side, seed):
-    _say("search fp64", n, side, seed)
+def test_search_parity_fp64(oracle, n, side, seed, flags):
+    _say("search fp64", n, side, seed, "flags", flags)
     net = uniform_cube(n, side, seed)
-    ref, g, nreq, tgt, tests, cnt = _search_pair(oracle, net, seed=seed + 11, step=3)
+    ref, g, nreq, tgt, tests, cnt = _search_pair(oracle, net, seed=seed + 11, step=3, flags=flags)
     bad, near = _check_search(ref, tgt, tests, cnt, net)
     _say("near-boundary searches", near, "mismatches", len(bad), bad[:5])
     assert not bad
@@ -139,20 +140,22 @@ def test_search_parity_fp64(oracle, n, side, seed):
     g.close()
 
 
-def test_search_parity_fp64_disabled_and_sparse(oracle):
-    _say("search fp64 disabled + sparse vacancies")
+@pytest.mark.parametrize("flags", [0, RL_FLAG_EXACT_ORDER])
+def test_search_parity_fp64_disabled_and_sparse(oracle, flags):
+    _say("search fp64 disabled + sparse vacancies", flags)
     net = uniform_cube(4000, 1500.0, 9)
     rng = np.random.default_rng(9)
     en = (rng.random(4000) > 0.2).astype(np.uint8)
     net.vac_exc[rng.random(4000) < 0.7] = 0
-    ref, g, nreq, tgt, tests, cnt = _search_pair(oracle, net, seed=21, step=1, enabled=en)
+    ref, g, nreq, tgt, tests, cnt = _search_pair(oracle, net, seed=21, step=1, enabled=en, flags=flags)
     bad, near = _check_search(ref, tgt, tests, cnt, net)
     assert not bad
     g.close()
 
 
-def test_search_parity_real_positions(oracle):
-    _say("search fp64 real positions")
+@pytest.mark.parametrize("flags", [0, RL_FLAG_EXACT_ORDER])
+def test_search_parity_real_positions(oracle, flags):
+    _say("search fp64 real positions", flags)
     d = np.load(os.path.join(GOLDEN, "positions_51842.npz"))
     n = len(d["xyz"])
     rng = np.random.default_rng(6)
@@ -161,19 +164,20 @@ def test_search_parity_real_positions(oracle):
     sig = (rng.random(n) < 0.2).astype(np.uint8)
     net = Network(d["xyz"], sig, rng.poisson(2, n).astype(np.uint32), rng.poisson(1.5, n).astype(np.uint32),
                   rng.poisson(1.5, n).astype(np.uint32), tuple(d["box_min"]), tuple(d["box_max"]))
-    ref, g, nreq, tgt, tests, cnt = _search_pair(oracle, net, seed=7, step=0, sigma=20.0)
+    ref, g, nreq, tgt, tests, cnt = _search_pair(oracle, net, seed=7, step=0, sigma=20.0, flags=flags)
     bad, near = _check_search(ref, tgt, tests, cnt, net)
     _say("near-boundary searches", near, "mismatches", len(bad))
     assert not bad
     g.close()
 
 
-def test_search_fp32_mixed_divergence(oracle):
-    _say("search fp32 mixed vs fp64")
+@pytest.mark.parametrize("flags", [0, RL_FLAG_EXACT_ORDER])
+def test_search_fp32_mixed_divergence(oracle, flags):
+    _say("search fp32 mixed vs fp64", flags)
     net = uniform_cube(50_000, 10_000.0 * (0.05 ** (1 / 3)), 12)
     outs = {}
     for prec in (RL_PREC_FP64, RL_PREC_FP32_MIXED):
-        g = BarnesHutHIP(net.box_min, net.box_max, seed=5, precision=prec)
+        g = BarnesHutHIP(net.box_min, net.box_max, seed=5, precision=prec, flags=flags)
         g.init(net.xyz, net.signal)
         g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
         g.find_target_neurons(2)
@@ -278,11 +282,12 @@ def test_neuron_step_parity(oracle):
     g.close()
 
 
-def test_full_plasticity_step_invariants():
-    _say("full plasticity step invariants 200k")
+@pytest.mark.parametrize("flags", [0, RL_FLAG_EXACT_ORDER])
+def test_full_plasticity_step_invariants(flags):
+    _say("full plasticity step invariants 200k", flags)
     net = uniform_cube(200_000, 10_000.0 * (0.2 ** (1 / 3)), 17)
     for prec in (RL_PREC_FP64, RL_PREC_FP32_MIXED):
-        g = BarnesHutHIP(net.box_min, net.box_max, seed=1, precision=prec)
+        g = BarnesHutHIP(net.box_min, net.box_max, seed=1, precision=prec, flags=flags)
         g.init(net.xyz, net.signal)
         g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
         st = g.update_connectivity(1)
@@ -293,3 +298,22 @@ def test_full_plasticity_step_invariants():
         assert np.all(va <= net.vac_axon) and np.all(ve <= net.vac_exc) and np.all(vi <= net.vac_inh)
         assert st["requests"] <= st["searches"] <= int(net.vac_axon.sum())
         g.close()
+
+
+def test_coop_matches_exact_order_kernel():
+    _say("cooperative vs exact-order kernel 100k")
+    net = uniform_cube(100_000, 10_000.0 * (0.1 ** (1 / 3)), 18)
+    outs = {}
+    for flags in (0, RL_FLAG_EXACT_ORDER):
+        g = BarnesHutHIP(net.box_min, net.box_max, seed=9, flags=flags)
+        g.init(net.xyz, net.signal)
+        g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+        g.find_target_neurons(5)
+        outs[flags] = g.get_search_slots(int(net.vac_axon.sum())) + (g.last_search_stats(),)
+        g.close()
+    a, b = outs[0], outs[RL_FLAG_EXACT_ORDER]
+    assert np.array_equal(a[1], b[1])          # reference test counts per search
+    div = np.mean(a[0] != b[0])
+    _say("cooperative vs exact-order pick difference", div)
+    assert div < 1e-4
+    assert a[3]["node_tests_ref"] == b[3]["node_tests_ref"]
