@@ -47,7 +47,7 @@ struct CoopArgs {
     uint64_t seg_begin[kXcds];
     uint64_t seg_end[kXcds];
     unsigned long long* cursor;   // kXcds counters
-    uint32_t* t_node;             // per wave: cap slots
+    uint32_t* t_my;               // per wave: cap slots, packed child range of the slot's node
     uint32_t* t_fexp;             // first expanded child slot | expanded-child count << 28
     double* t_mass;
     uint32_t cap;
@@ -56,7 +56,8 @@ struct CoopArgs {
 };
 
 struct CoopShared {
-    uint32_t lvl[2][kLevelCap];        // packed child range (meta.y) of the slots of a level
+    uint32_t lvl[2][kLevelCap + 1];    // packed child range (meta.y) of the slots of a level; the
+                                       // last entry takes the writes of lanes with nothing to write
     uint32_t lvl_start[kMaxLevels + 4];
 };
 
@@ -66,6 +67,12 @@ __device__ __forceinline__ uint32_t lane() {
 
 __device__ __forceinline__ uint32_t rank_below(uint64_t mask, uint32_t l) {
     return static_cast<uint32_t>(__popcll(mask & ((1ULL << l) - 1ULL)));
+}
+
+__device__ __forceinline__ double uniform_f64(double v) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
 }
 
 template <int kCtrl>
@@ -96,38 +103,46 @@ __device__ __forceinline__ double group_suffix8(double v) {
     return v;
 }
 
-enum : int { kModeMixed = 0, kModeExact = 1 };
+struct NodeData {
+    uint2 m;
+    uint4 r32;
+    double4 r64;
+};
 
+// Loads the count/link word and the position record of `node` (both issued together).  The loads
+// are issued in every lane (node 0 stands in for inactive lanes) so that no branch hides them from
+// the compiler's memory-counter bookkeeping: a load in a skippable branch makes the next wait drain
+// every outstanding load, prefetches included.
 template <bool kMixed>
-__device__ __forceinline__ int test_node(const SearchArgs& a, const Lane& L, uint32_t node, int mode, int kind,
-                                         double* p_out, uint32_t* meta_y) {
-    const uint2 m = L.meta[node];
-    const bool use_mixed = kMixed && mode == kModeMixed;
-    double4 r64 = make_double4(0.0, 0.0, 0.0, 0.0);
-    uint4 r32 = make_uint4(0U, 0U, 0U, 0U);
-    if (use_mixed) {
-        r32 = L.rec32[node];
-        __asm__ volatile("" ::"v"(r32.x));
+__device__ __forceinline__ NodeData load_node(const Lane& L, uint32_t node, bool active) {
+    const uint32_t x = active ? node : 0U;
+    NodeData d;
+    d.m = L.meta[x];
+    if (kMixed) {
+        d.r32 = L.rec32[x];
+        d.r64 = make_double4(0.0, 0.0, 0.0, 0.0);
     } else {
-        r64 = L.rec64[node];
-        __asm__ volatile("" ::"v"(r64.x));
+        d.r32 = make_uint4(0U, 0U, 0U, 0U);
+        d.r64 = L.rec64[x];
     }
+    return d;
+}
+
+// Acceptance test and probability of a loaded node (test_acceptance_criterion, BarnesHutBase.h:72-114;
+// Kernel::calculate_attractiveness_to_connect, Kernel.h:61-75).
+template <bool kMixed>
+__device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, const NodeData& d, uint32_t node,
+                                         double* p_out) {
     *p_out = 0.0;
-    *meta_y = m.y;
-    if (m.x == 0) {
+    if (d.m.x == 0) {
         return kDiscard;
     }
-    const bool leaf = (m.y >> kLinkBits) == 0;
-    Eval e;
-    if (use_mixed) {
-        e = eval_mixed_rec(a, L, r32, node, leaf);
-    } else {
-        e = eval_exact_rec(a, L, r64, leaf);
-    }
+    const bool leaf = (d.m.y >> kLinkBits) == 0;
+    const Eval e = kMixed ? eval_mixed_rec(a, L, d.r32, node, leaf) : eval_exact_rec(a, L, d.r64, leaf);
     if (e.status == kAccept) {
-        const bool self = leaf && (m.y & kLinkMask) == L.src;
+        const bool self = leaf && (d.m.y & kLinkMask) == L.src;
         if (!self) {
-            *p_out = use_mixed ? probability<true>(a, e, m.x, kind) : probability<false>(a, e, m.x, kind);
+            *p_out = probability<kMixed>(a, e, d.m.x, kGauss);
         }
     }
     return e.status;
@@ -141,15 +156,17 @@ struct Bfs {
 };
 
 // Breadth-first traversal below `root` (inner, with vacant dendrites) filling the slot table at `base`.
+// The loads of the next step's 64 children are issued before the current step is evaluated.
 template <bool kMixed>
-__device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root, uint32_t root_meta_y, int mode, int kind,
-                   CoopShared& sh, uint64_t base) {
+__device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopShared& sh, uint64_t base) {
     const SearchArgs& s = a.s;
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
     Bfs r{ 0.0, 0, 0, false };
+    // the last slot of the table takes the stores of lanes with nothing to store (unconditional stores)
+    const uint64_t dump = base + a.cap - 1;
     if (ln == 0) {
-        a.t_node[base] = root;
+        a.t_my[base] = root_meta_y;
         sh.lvl[0][0] = root_meta_y;
         sh.lvl_start[0] = 0;
     }
@@ -157,41 +174,56 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root, uint32_t roo
     uint32_t lb = 0, le = 1, nslots = 1, lev = 0, cur = 0;
     while (lb < le) {
         const uint32_t nb = nslots;
+        uint32_t k = lb + q;
+        bool gact = k < le;
+        uint32_t packed = sh.lvl[cur][gact ? k - lb : kLevelCap];
+        uint32_t child = (packed & kLinkMask) + j;
+        bool act = gact && j < (packed >> kLinkBits);
+        NodeData nd = load_node<kMixed>(L, child, act);
+        // Wait for the first step's data here, so that the loop header carries no pending load and
+        // the stores of the previous step are never waited for.
+        if (kMixed) {
+            __asm__ volatile("" ::"v"(nd.m.x), "v"(nd.m.y), "v"(nd.r32.x), "v"(nd.r32.y), "v"(nd.r32.z), "v"(nd.r32.w));
+        } else {
+            __asm__ volatile("" ::"v"(nd.m.x), "v"(nd.m.y), "v"(nd.r64.x), "v"(nd.r64.y), "v"(nd.r64.z), "v"(nd.r64.w));
+        }
         for (uint32_t kb = lb; kb < le; kb += 8) {
-            const uint32_t k = kb + q;
-            const bool gact = k < le;
-            const uint32_t packed = gact ? sh.lvl[cur][k - lb] : 0U;
-            const uint32_t fc = packed & kLinkMask;
-            const uint32_t nc = packed >> kLinkBits;
-            const bool act = gact && j < nc;
-            const uint32_t child = fc + j;
+            // issue the next step's loads before evaluating this step
+            const uint32_t k2 = kb + 8 + q;
+            const bool gact2 = k2 < le;
+            const uint32_t packed2 = sh.lvl[cur][gact2 ? k2 - lb : kLevelCap];
+            const uint32_t child2 = (packed2 & kLinkMask) + j;
+            const bool act2 = gact2 && j < (packed2 >> kLinkBits);
+            const NodeData nd2 = load_node<kMixed>(L, child2, act2);
             int st = kDiscard;
             double p = 0.0;
-            uint32_t my = 0;
             if (act) {
-                st = test_node<kMixed>(s, L, child, mode, kind, &p, &my);
+                st = eval_node<kMixed>(s, L, nd, child, &p);
             }
             r.tests += static_cast<uint32_t>(__popcll(__ballot(act)));
             r.count += static_cast<uint32_t>(__popcll(__ballot(act && st == kAccept)));
             const double gsum = group_sum8(p);
-            const uint64_t em = __ballot(act && st == kExpand);
+            const bool ex = act && st == kExpand;
+            const uint64_t em = __ballot(ex);
             const uint32_t ne = static_cast<uint32_t>(__popcll(em));
-            if (nslots + ne > a.cap || nslots + ne - nb > kLevelCap) {
+            if (nslots + ne > a.cap - 1 || nslots + ne - nb > kLevelCap) {
                 r.overflow = true;
                 return r;
             }
-            if (act && st == kExpand) {
-                const uint32_t slot = nslots + rank_below(em, ln);
-                a.t_node[base + slot] = child;
-                sh.lvl[cur ^ 1U][slot - nb] = my;
-            }
-            if (gact && j == 0) {
-                const uint32_t gem = static_cast<uint32_t>((em >> (8U * q)) & 0xFFULL);
-                const uint32_t first = nslots + rank_below(em, 8U * q);
-                a.t_mass[base + k] = gsum;
-                a.t_fexp[base + k] = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
-            }
+            const uint32_t slot = nslots + rank_below(em, ln);
+            a.t_my[ex ? base + slot : dump] = nd.m.y;
+            sh.lvl[cur ^ 1U][ex ? slot - nb : kLevelCap] = nd.m.y;
+            const bool gw = gact && j == 0;
+            const uint32_t gem = static_cast<uint32_t>((em >> (8U * q)) & 0xFFULL);
+            const uint32_t first = nslots + rank_below(em, 8U * q);
+            a.t_mass[gw ? base + k : dump] = gsum;
+            a.t_fexp[gw ? base + k : dump] = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
             nslots += ne;
+            k = k2;
+            gact = gact2;
+            child = child2;
+            act = act2;
+            nd = nd2;
         }
         __syncthreads();
         lb = nb;
@@ -217,9 +249,17 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root, uint32_t roo
                 const uint32_t ne = fx >> 28U;
                 if (ne != 0) {
                     const uint32_t f = fx & kSlotNone;
+                    double m8[8];
+#pragma unroll
+                    for (uint32_t c = 0; c < 8; c++) {
+                        m8[c] = c < ne ? a.t_mass[base + f + c] : 0.0;
+                    }
                     double sum = 0.0;
-                    for (uint32_t c = 0; c < ne; c++) {
-                        sum += a.t_mass[base + f + c];
+#pragma unroll
+                    for (uint32_t c = 0; c < 8; c++) {
+                        if (c < ne) {
+                            sum += m8[c];
+                        }
                     }
                     a.t_mass[base + sl] += sum;
                 }
@@ -234,8 +274,7 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root, uint32_t roo
 // Resolves the draws of the active groups top-down through the slot table (one draw per group).
 // Returns the picked node in every lane of the group, kNone when the list had no positive mass.
 template <bool kMixed>
-__device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool gact, int mode, int kind,
-                             uint64_t base, uint32_t* tests) {
+__device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool gact, uint64_t base, uint32_t* tests) {
     const SearchArgs& s = a.s;
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
@@ -247,33 +286,38 @@ __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool g
         if (__ballot(!done) == 0) {
             break;
         }
-        uint32_t fx = 0, fc = 0, nc = 0;
-        if (!done) {
-            const uint32_t x = a.t_node[base + slot];
-            fx = a.t_fexp[base + slot];
-            const uint32_t my = L.meta[x].y;
-            fc = my & kLinkMask;
-            nc = my >> kLinkBits;
+        // slot < cap always holds: it only advances to a checked child slot
+        uint32_t fx = a.t_fexp[base + slot];
+        uint32_t my = a.t_my[base + slot];
+        if (done) {
+            fx = 0;
+            my = 0;
         }
+        const uint32_t fc = my & kLinkMask;
+        const uint32_t nc = my >> kLinkBits;
+        const uint32_t f = fx & kSlotNone;
+        const uint32_t nexp = fx >> 28U;
         const bool act = !done && j < nc;
+        // the children and the masses of the expanded ones, loaded together
+        const NodeData nd = load_node<kMixed>(L, fc + j, act);
+        const bool mact = !done && j < nexp && f != kSlotNone && f + j < a.cap;
+        const double mj_raw = a.t_mass[mact ? base + f + j : base];
+        const double mj = mact ? mj_raw : 0.0;
         int st = kDiscard;
         double p = 0.0;
-        uint32_t my = 0;
         if (act) {
-            st = test_node<kMixed>(s, L, fc + j, mode, kind, &p, &my);
+            st = eval_node<kMixed>(s, L, nd, fc + j, &p);
         }
         *tests += static_cast<uint32_t>(__popcll(__ballot(act)));
         const uint64_t em = __ballot(act && st == kExpand);
         const uint32_t gem = static_cast<uint32_t>((em >> g0) & 0xFFULL);
+        const uint32_t rk = static_cast<uint32_t>(__popc(gem & ((1U << j) - 1U)));
+        const double mexp = __shfl(mj, static_cast<int>(g0 + (rk & 7U)), kWave);
         double mass = 0.0;
         if (st == kAccept) {
             mass = p;
         } else if (st == kExpand) {
-            const uint32_t f = fx & kSlotNone;
-            const uint32_t rk = static_cast<uint32_t>(__popc(gem & ((1U << j) - 1U)));
-            if (f != kSlotNone && f + rk < a.cap) {
-                mass = a.t_mass[base + f + rk];
-            }
+            mass = mexp;
         }
         const double cum = group_suffix8(mass);
         const double cum_next_raw = __shfl_down(cum, 1, kWave);
@@ -297,12 +341,13 @@ __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool g
                 done = true;
                 res = fc + static_cast<uint32_t>(jstar);
             } else {
-                const uint32_t f = fx & kSlotNone;
                 rr = rr - before;
-                slot = f + static_cast<uint32_t>(__popc(gem & ((1U << jstar) - 1U)));
-                if (f == kSlotNone || slot >= a.cap) {
+                const uint32_t next = f + static_cast<uint32_t>(__popc(gem & ((1U << jstar) - 1U)));
+                if (f == kSlotNone || next >= a.cap - 1) {
                     done = true;
                     res = kNone;
+                } else {
+                    slot = next;
                 }
             }
         }
@@ -313,54 +358,25 @@ __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool g
 struct ListResult {
     double total;
     uint32_t count;
-    uint32_t tests_ref;
-    uint32_t tests_exec;
-    int mode;
-    int kind;
-    bool overflow;
+    uint32_t tests;
+    bool overflow;   // slot table overflow or a Gaussian sum of 0: the per-lane kernel redoes the source
 };
 
-// Traversal of the subtree of `root` with the reference's fallbacks (FP64 re-evaluation of an FP32
-// sum of 0, then n/r when the Gaussian sum is 0: Kernel::create_probability_interval, Kernel.h:109-126).
+// Candidate list of the subtree of `root`; a list whose Gaussian sum underflows to 0 needs the
+// reference's fallbacks (Kernel::create_probability_interval, Kernel.h:109-126), which the per-lane
+// kernel implements, so the source is handed over.
 template <bool kMixed>
 __device__ ListResult list_of(const CoopArgs& a, const Lane& L, uint32_t root, CoopShared& sh, uint64_t base) {
-    ListResult lr{ 0.0, 0, 0, 0, kMixed ? kModeMixed : kModeExact, kGauss, false };
+    ListResult lr{ 0.0, 0, 0, false };
     const uint2 mr = L.meta[root];
     if (mr.x == 0) {
         return lr;
     }
-    Bfs b = bfs<kMixed>(a, L, root, mr.y, lr.mode, kGauss, sh, base);
-    lr.tests_ref = b.tests;
-    lr.tests_exec = b.tests;
-    if (b.overflow) {
-        lr.overflow = true;
-        return lr;
-    }
+    const Bfs b = bfs<kMixed>(a, L, mr.y, sh, base);
+    lr.tests = b.tests;
     lr.total = b.total;
     lr.count = b.count;
-    if (b.count > 0 && b.total == 0.0) {
-        if (kMixed) {
-            b = bfs<kMixed>(a, L, root, mr.y, kModeExact, kGauss, sh, base);
-            lr.tests_exec += b.tests;
-            lr.mode = kModeExact;
-            lr.total = b.total;
-            if (b.overflow) {
-                lr.overflow = true;
-                return lr;
-            }
-        }
-        if (lr.total == 0.0) {
-            b = bfs<kMixed>(a, L, root, mr.y, kModeExact, kInverse, sh, base);
-            lr.tests_exec += b.tests;
-            lr.mode = kModeExact;
-            lr.kind = kInverse;
-            lr.total = b.total;
-            if (b.overflow) {
-                lr.overflow = true;
-                return lr;
-            }
-        }
-    }
+    lr.overflow = b.overflow || (b.count > 0 && b.total == 0.0);
     return lr;
 }
 
@@ -394,17 +410,18 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
         const uint32_t src = __builtin_amdgcn_readfirstlane(a.work[w]);
         const uint32_t na = __builtin_amdgcn_readfirstlane(s.vac_axon[src]);
         const uint64_t slot0 = s.slot_off[src];
+        // source data is wave-uniform: keep it in scalar registers
         Lane L;
         L.src = src;
-        L.sig = s.signal[src] == 0 ? 0 : 1;
+        L.sig = __builtin_amdgcn_readfirstlane(s.signal[src] == 0 ? 0 : 1);
         lane_bind(s, L);
-        L.px = s.pos[src];
-        L.py = s.pos[s.n + src];
-        L.pz = s.pos[2 * s.n + src];
+        L.px = uniform_f64(s.pos[src]);
+        L.py = uniform_f64(s.pos[s.n + src]);
+        L.pz = uniform_f64(s.pos[2 * s.n + src]);
         if (kMixed) {
-            L.qx = static_cast<int32_t>(s.qpos[src]);
-            L.qy = static_cast<int32_t>(s.qpos[s.n + src]);
-            L.qz = static_cast<int32_t>(s.qpos[2 * s.n + src]);
+            L.qx = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(s.qpos[src]));
+            L.qy = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(s.qpos[s.n + src]));
+            L.qz = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(s.qpos[2 * s.n + src]));
         } else {
             L.qx = L.qy = L.qz = 0;
         }
@@ -412,14 +429,14 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
         bool overflow = false;
         uint32_t found = 0, searched = 0;
         const ListResult l1 = list_of<kMixed>(a, L, 0, sh, base);
-        exec += l1.tests_exec;
+        exec += l1.tests;
         if (l1.overflow) {
             overflow = true;
-        } else if (l1.count == 0 || l1.total == 0.0) {
+        } else if (l1.count == 0) {
             if (ln == 0) {
-                s.slot_tests[slot0] = l1.tests_ref;
+                s.slot_tests[slot0] = l1.tests;
             }
-            ref += l1.tests_ref;
+            ref += l1.tests;
             searched = 1;
         } else {
             const double next1 = nextafter(l1.total, l1.total + 0.00001);
@@ -428,7 +445,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
             for (uint32_t j0 = 0; j0 < na; j0 += 8) {
                 const bool gact = j0 + q < na;
                 const double r = gact ? draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + q, 1U), next1) : 0.0;
-                const uint32_t x = navigate<kMixed>(a, L, r, gact, l1.mode, l1.kind, base, &nav_tests);
+                const uint32_t x = navigate<kMixed>(a, L, r, gact, base, &nav_tests);
                 if (gact && j == 0) {
                     s.slot_target[slot0 + j0 + q] = x;
                 }
@@ -443,7 +460,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                 uint32_t node = gact ? s.slot_target[slot0 + jd] : kNone;
                 int state = gact ? kDrawActive : kDrawIdle;
                 uint32_t level = 2;
-                uint32_t tests_d = l1.tests_ref;
+                uint32_t tests_d = l1.tests;
                 if (gact && node == kNone) {
                     state = kDrawFailed;
                 }
@@ -473,17 +490,20 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                     const bool act = act_g && j < nc;
                     int st = kDiscard;
                     double p = 0.0;
-                    uint32_t cy = 0;
+                    const NodeData cd = load_node<kMixed>(L, fc + j, act);
                     if (act) {
-                        st = test_node<kMixed>(s, L, fc + j, kMixed ? kModeMixed : kModeExact, kGauss, &p, &cy);
+                        st = eval_node<kMixed>(s, L, cd, fc + j, &p);
                     }
                     exec += static_cast<uint32_t>(__popcll(__ballot(act)));
                     const uint32_t gexp = static_cast<uint32_t>((__ballot(act && st == kExpand) >> g0) & 0xFFULL);
                     const uint32_t gacc = static_cast<uint32_t>((__ballot(act && st == kAccept) >> g0) & 0xFFULL);
                     const double gsum = __shfl(group_sum8(p), static_cast<int>(g0), kWave);
                     if (act_g) {
-                        if (gexp != 0 || (gacc != 0 && gsum == 0.0)) {
-                            state = kDrawSlow;  // needs the general traversal (expansion or fallback)
+                        if (gacc != 0 && gexp == 0 && gsum == 0.0) {
+                            overflow = true;  // Gaussian sum of 0: the per-lane kernel applies the fallback
+                            state = kDrawIdle;
+                        } else if (gexp != 0) {
+                            state = kDrawSlow;  // needs the general traversal
                         } else {
                             tests_d += nc;
                             if (gacc == 0) {
@@ -514,6 +534,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                         }
                     }
                 }
+                overflow = __ballot(overflow) != 0;
                 // general traversal for the draws that need it, in axon order
                 for (uint32_t g = 0; g < 8 && !overflow; g++) {
                     const int sg = __shfl(state, static_cast<int>(8U * g), kWave);
@@ -536,20 +557,20 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                             break;
                         }
                         const ListResult ls = list_of<kMixed>(a, L, nd, sh, base);
-                        exec += ls.tests_exec;
-                        td += ls.tests_ref;
+                        exec += ls.tests;
+                        td += ls.tests;
                         if (ls.overflow) {
                             overflow = true;
                             break;
                         }
-                        if (ls.count == 0 || ls.total == 0.0) {
+                        if (ls.count == 0) {
                             res = kDrawFailed;
                             break;
                         }
                         const double r = draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + g, lv),
                                                          nextafter(ls.total, ls.total + 0.00001));
                         uint32_t nt = 0;
-                        nd = navigate<kMixed>(a, L, r, q == 0, ls.mode, ls.kind, base, &nt);
+                        nd = navigate<kMixed>(a, L, r, q == 0, base, &nt);
                         nd = __shfl(nd, 0, kWave);
                         exec += nt;
                         lv++;
@@ -626,7 +647,7 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow)
         a.seg_end[x] = static_cast<uint64_t>(nwork) * (x + 1) / kXcds;
     }
     a.cursor = c->coop_cursor;
-    a.t_node = c->coop_node;
+    a.t_my = c->coop_node;
     a.t_fexp = c->coop_fexp;
     a.t_mass = c->coop_mass;
     a.cap = c->coop_cap;
