@@ -75,10 +75,11 @@ __device__ __forceinline__ double uniform_f64(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// Lanes whose DPP source lies outside their row read 0 (bound_ctrl).
 template <int kCtrl>
 __device__ __forceinline__ double dpp_f64(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 
@@ -113,9 +114,10 @@ struct NodeData {
 // are issued in every lane (node 0 stands in for inactive lanes) so that no branch hides them from
 // the compiler's memory-counter bookkeeping: a load in a skippable branch makes the next wait drain
 // every outstanding load, prefetches included.
-template <bool kMixed>
+template <bool kMixed, bool kMu0>
 __device__ __forceinline__ NodeData load_node(const Lane& L, uint32_t node, bool active) {
     const uint32_t x = active ? node : 0U;
+    __builtin_assume(x < (1U << kLinkBits));   // node ids fit the link field: 32-bit byte offsets
     NodeData d;
     d.m = L.meta[x];
     if (kMixed) {
@@ -130,7 +132,7 @@ __device__ __forceinline__ NodeData load_node(const Lane& L, uint32_t node, bool
 
 // Acceptance test and probability of a loaded node (test_acceptance_criterion, BarnesHutBase.h:72-114;
 // Kernel::calculate_attractiveness_to_connect, Kernel.h:61-75).
-template <bool kMixed>
+template <bool kMixed, bool kMu0>
 __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, const NodeData& d, uint32_t node,
                                          double* p_out) {
     *p_out = 0.0;
@@ -142,7 +144,7 @@ __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, con
     if (e.status == kAccept) {
         const bool self = leaf && (d.m.y & kLinkMask) == L.src;
         if (!self) {
-            *p_out = probability<kMixed>(a, e, d.m.x, kGauss);
+            *p_out = probability<kMixed, kMu0>(a, e, d.m.x, kGauss);
         }
     }
     return e.status;
@@ -157,16 +159,19 @@ struct Bfs {
 
 // Breadth-first traversal below `root` (inner, with vacant dendrites) filling the slot table at `base`.
 // The loads of the next step's 64 children are issued before the current step is evaluated.
-template <bool kMixed>
+template <bool kMixed, bool kMu0>
 __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopShared& sh, uint64_t base) {
     const SearchArgs& s = a.s;
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
     Bfs r{ 0.0, 0, 0, false };
     // the last slot of the table takes the stores of lanes with nothing to store (unconditional stores)
-    const uint64_t dump = base + a.cap - 1;
+    const uint32_t dump = a.cap - 1;
+    uint32_t* const tmy = a.t_my + base;
+    uint32_t* const tfx = a.t_fexp + base;
+    double* const tms = a.t_mass + base;
     if (ln == 0) {
-        a.t_my[base] = root_meta_y;
+        tmy[0] = root_meta_y;
         sh.lvl[0][0] = root_meta_y;
         sh.lvl_start[0] = 0;
     }
@@ -179,7 +184,7 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
         uint32_t packed = sh.lvl[cur][gact ? k - lb : kLevelCap];
         uint32_t child = (packed & kLinkMask) + j;
         bool act = gact && j < (packed >> kLinkBits);
-        NodeData nd = load_node<kMixed>(L, child, act);
+        NodeData nd = load_node<kMixed, kMu0>(L, child, act);
         // Wait for the first step's data here, so that the loop header carries no pending load and
         // the stores of the previous step are never waited for.
         if (kMixed) {
@@ -194,11 +199,11 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
             const uint32_t packed2 = sh.lvl[cur][gact2 ? k2 - lb : kLevelCap];
             const uint32_t child2 = (packed2 & kLinkMask) + j;
             const bool act2 = gact2 && j < (packed2 >> kLinkBits);
-            const NodeData nd2 = load_node<kMixed>(L, child2, act2);
+            const NodeData nd2 = load_node<kMixed, kMu0>(L, child2, act2);
             int st = kDiscard;
             double p = 0.0;
             if (act) {
-                st = eval_node<kMixed>(s, L, nd, child, &p);
+                st = eval_node<kMixed, kMu0>(s, L, nd, child, &p);
             }
             r.tests += static_cast<uint32_t>(__popcll(__ballot(act)));
             r.count += static_cast<uint32_t>(__popcll(__ballot(act && st == kAccept)));
@@ -211,13 +216,13 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
                 return r;
             }
             const uint32_t slot = nslots + rank_below(em, ln);
-            a.t_my[ex ? base + slot : dump] = nd.m.y;
+            tmy[ex ? slot : dump] = nd.m.y;
             sh.lvl[cur ^ 1U][ex ? slot - nb : kLevelCap] = nd.m.y;
             const bool gw = gact && j == 0;
             const uint32_t gem = static_cast<uint32_t>((em >> (8U * q)) & 0xFFULL);
             const uint32_t first = nslots + rank_below(em, 8U * q);
-            a.t_mass[gw ? base + k : dump] = gsum;
-            a.t_fexp[gw ? base + k : dump] = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
+            tms[gw ? k : dump] = gsum;
+            tfx[gw ? k : dump] = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
             nslots += ne;
             k = k2;
             gact = gact2;
@@ -245,14 +250,14 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
         for (uint32_t i0 = sb; i0 < se; i0 += kWave) {
             const uint32_t sl = i0 + ln;
             if (sl < se) {
-                const uint32_t fx = a.t_fexp[base + sl];
+                const uint32_t fx = tfx[sl];
                 const uint32_t ne = fx >> 28U;
                 if (ne != 0) {
                     const uint32_t f = fx & kSlotNone;
                     double m8[8];
 #pragma unroll
                     for (uint32_t c = 0; c < 8; c++) {
-                        m8[c] = c < ne ? a.t_mass[base + f + c] : 0.0;
+                        m8[c] = c < ne ? tms[f + c] : 0.0;
                     }
                     double sum = 0.0;
 #pragma unroll
@@ -261,24 +266,27 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
                             sum += m8[c];
                         }
                     }
-                    a.t_mass[base + sl] += sum;
+                    tms[sl] += sum;
                 }
             }
         }
         __syncthreads();
     }
-    r.total = a.t_mass[base];
+    r.total = tms[0];
     return r;
 }
 
 // Resolves the draws of the active groups top-down through the slot table (one draw per group).
 // Returns the picked node in every lane of the group, kNone when the list had no positive mass.
-template <bool kMixed>
+template <bool kMixed, bool kMu0>
 __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool gact, uint64_t base, uint32_t* tests) {
     const SearchArgs& s = a.s;
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
     const uint32_t g0 = 8U * q;
+    const uint32_t* const tmy = a.t_my + base;
+    const uint32_t* const tfx = a.t_fexp + base;
+    const double* const tms = a.t_mass + base;
     uint32_t slot = 0;
     bool done = !gact;
     uint32_t res = kNone;
@@ -287,8 +295,8 @@ __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool g
             break;
         }
         // slot < cap always holds: it only advances to a checked child slot
-        uint32_t fx = a.t_fexp[base + slot];
-        uint32_t my = a.t_my[base + slot];
+        uint32_t fx = tfx[slot];
+        uint32_t my = tmy[slot];
         if (done) {
             fx = 0;
             my = 0;
@@ -299,14 +307,14 @@ __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool g
         const uint32_t nexp = fx >> 28U;
         const bool act = !done && j < nc;
         // the children and the masses of the expanded ones, loaded together
-        const NodeData nd = load_node<kMixed>(L, fc + j, act);
+        const NodeData nd = load_node<kMixed, kMu0>(L, fc + j, act);
         const bool mact = !done && j < nexp && f != kSlotNone && f + j < a.cap;
-        const double mj_raw = a.t_mass[mact ? base + f + j : base];
+        const double mj_raw = tms[mact ? f + j : 0U];
         const double mj = mact ? mj_raw : 0.0;
         int st = kDiscard;
         double p = 0.0;
         if (act) {
-            st = eval_node<kMixed>(s, L, nd, fc + j, &p);
+            st = eval_node<kMixed, kMu0>(s, L, nd, fc + j, &p);
         }
         *tests += static_cast<uint32_t>(__popcll(__ballot(act)));
         const uint64_t em = __ballot(act && st == kExpand);
@@ -365,14 +373,14 @@ struct ListResult {
 // Candidate list of the subtree of `root`; a list whose Gaussian sum underflows to 0 needs the
 // reference's fallbacks (Kernel::create_probability_interval, Kernel.h:109-126), which the per-lane
 // kernel implements, so the source is handed over.
-template <bool kMixed>
+template <bool kMixed, bool kMu0>
 __device__ ListResult list_of(const CoopArgs& a, const Lane& L, uint32_t root, CoopShared& sh, uint64_t base) {
     ListResult lr{ 0.0, 0, 0, false };
     const uint2 mr = L.meta[root];
     if (mr.x == 0) {
         return lr;
     }
-    const Bfs b = bfs<kMixed>(a, L, mr.y, sh, base);
+    const Bfs b = bfs<kMixed, kMu0>(a, L, mr.y, sh, base);
     lr.tests = b.tests;
     lr.total = b.total;
     lr.count = b.count;
@@ -383,7 +391,7 @@ __device__ ListResult list_of(const CoopArgs& a, const Lane& L, uint32_t root, C
 // Per-draw descent state, identical in the 8 lanes of the draw's group.
 enum : int { kDrawActive = 0, kDrawDone = 1, kDrawFailed = 2, kDrawSlow = 3, kDrawIdle = 4 };
 
-template <bool kMixed>
+template <bool kMixed, bool kMu0>
 __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
     __shared__ CoopShared sh;
     const SearchArgs& s = a.s;
@@ -428,7 +436,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
         uint64_t exec = 0, ref = 0;
         bool overflow = false;
         uint32_t found = 0, searched = 0;
-        const ListResult l1 = list_of<kMixed>(a, L, 0, sh, base);
+        const ListResult l1 = list_of<kMixed, kMu0>(a, L, 0, sh, base);
         exec += l1.tests;
         if (l1.overflow) {
             overflow = true;
@@ -445,7 +453,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
             for (uint32_t j0 = 0; j0 < na; j0 += 8) {
                 const bool gact = j0 + q < na;
                 const double r = gact ? draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + q, 1U), next1) : 0.0;
-                const uint32_t x = navigate<kMixed>(a, L, r, gact, base, &nav_tests);
+                const uint32_t x = navigate<kMixed, kMu0>(a, L, r, gact, base, &nav_tests);
                 if (gact && j == 0) {
                     s.slot_target[slot0 + j0 + q] = x;
                 }
@@ -490,9 +498,9 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                     const bool act = act_g && j < nc;
                     int st = kDiscard;
                     double p = 0.0;
-                    const NodeData cd = load_node<kMixed>(L, fc + j, act);
+                    const NodeData cd = load_node<kMixed, kMu0>(L, fc + j, act);
                     if (act) {
-                        st = eval_node<kMixed>(s, L, cd, fc + j, &p);
+                        st = eval_node<kMixed, kMu0>(s, L, cd, fc + j, &p);
                     }
                     exec += static_cast<uint32_t>(__popcll(__ballot(act)));
                     const uint32_t gexp = static_cast<uint32_t>((__ballot(act && st == kExpand) >> g0) & 0xFFULL);
@@ -556,7 +564,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                             res = kDrawDone;
                             break;
                         }
-                        const ListResult ls = list_of<kMixed>(a, L, nd, sh, base);
+                        const ListResult ls = list_of<kMixed, kMu0>(a, L, nd, sh, base);
                         exec += ls.tests;
                         td += ls.tests;
                         if (ls.overflow) {
@@ -570,7 +578,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                         const double r = draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + g, lv),
                                                          nextafter(ls.total, ls.total + 0.00001));
                         uint32_t nt = 0;
-                        nd = navigate<kMixed>(a, L, r, q == 0, base, &nt);
+                        nd = navigate<kMixed, kMu0>(a, L, r, q == 0, base, &nt);
                         nd = __shfl(nd, 0, kWave);
                         exec += nt;
                         lv++;
@@ -655,9 +663,13 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow)
     a.n_overflow = reinterpret_cast<unsigned int*>(c->coop_cursor + kXcds);
     if (nwork > 0) {
         if (c->cfg.precision == RL_PREC_FP32_MIXED) {
-            k_search_coop<true><<<c->coop_blocks, kWave, 0, s>>>(a);
+            if (c->cfg.mu == 0.0) {
+                k_search_coop<true, true><<<c->coop_blocks, kWave, 0, s>>>(a);
+            } else {
+                k_search_coop<true, false><<<c->coop_blocks, kWave, 0, s>>>(a);
+            }
         } else {
-            k_search_coop<false><<<c->coop_blocks, kWave, 0, s>>>(a);
+            k_search_coop<false, false><<<c->coop_blocks, kWave, 0, s>>>(a);
         }
         RL_HIP(c, hipGetLastError());
     }

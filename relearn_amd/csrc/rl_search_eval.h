@@ -127,7 +127,8 @@ __device__ __forceinline__ Eval eval_mixed(const SearchArgs& a, const Lane& L, u
     return eval_mixed_rec(a, L, L.rec32[node], node, leaf);
 }
 
-template <bool kMixed>
+// kMuZero: the caller guarantees mu == 0 (the default kernel), so the mixed path needs no square root.
+template <bool kMixed, bool kMuZero = false>
 __device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e, uint32_t cnt, int kind) {
 #pragma clang fp contract(off)
     if (kind == kInverse) {
@@ -141,7 +142,7 @@ __device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e
         return (exp(exponent) * 1.0) * static_cast<double>(cnt);
     }
     float x;
-    if (a.muf == 0.0f) {
+    if (kMuZero || a.muf == 0.0f) {
         x = e.d2f * a.k2f;
     } else {
         const float d = sqrtf(e.d2f) - a.muf;
