@@ -1,5 +1,6 @@
 // C ABI of librelearn_bh.so: context management and the Algorithm-shaped entry points.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -154,9 +155,14 @@ int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
         delete c;
         return RL_EHIP;
     }
+    // one fixed-point scale for all axes (the largest extent spans 2^31 - 1 units), so squared
+    // distances and squared cell sides compare in the same units
+    double ext_max = 0.0;
     for (int d = 0; d < 3; d++) {
-        const double ext = cfg->box_max[d] - cfg->box_min[d];
-        c->qscale[d] = ext > 0.0 ? 2147483647.0 / ext : 0.0;
+        ext_max = std::max(ext_max, cfg->box_max[d] - cfg->box_min[d]);
+    }
+    for (int d = 0; d < 3; d++) {
+        c->qscale[d] = ext_max > 0.0 ? 2147483647.0 / ext_max : 0.0;
     }
     *out = c;
     return RL_OK;

@@ -458,11 +458,10 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     a.mu = c->cfg.mu;
     a.theta2f = static_cast<float>(c->cfg.theta * c->cfg.theta);
     a.k2f = static_cast<float>(-1.4426950408889634 / (c->cfg.sigma * c->cfg.sigma));
-    a.sigma_inv2f = static_cast<float>(1.0 / (c->cfg.sigma * c->cfg.sigma));
+    const double inv_q = c->qscale[0] > 0.0 ? 1.0 / c->qscale[0] : 0.0;
+    a.k2q = static_cast<float>(-1.4426950408889634 / (c->cfg.sigma * c->cfg.sigma) * inv_q * inv_q);
     a.muf = static_cast<float>(c->cfg.mu);
-    for (int d = 0; d < 3; d++) {
-        a.inv_q[d] = c->qscale[d] > 0.0 ? static_cast<float>(1.0 / c->qscale[d]) : 0.0f;
-    }
+    a.inv_q = static_cast<float>(inv_q);
     a.guard = 2.0e-5f;
     a.seed = c->cfg.seed;
     a.step = step;

@@ -104,6 +104,20 @@ __device__ __forceinline__ double group_suffix8(double v) {
     return v;
 }
 
+// Element i of a wave-uniform array whose byte offsets fit 32 bits (slot tables; node arrays of
+// fewer than 2^28 nodes): the address is a scalar base plus a 32-bit lane offset.
+template <typename T>
+__device__ __forceinline__ T& at(T* base, uint32_t i) {
+    const uint32_t off = i * static_cast<uint32_t>(sizeof(T));
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
+}
+
+template <typename T>
+__device__ __forceinline__ const T& at(const T* base, uint32_t i) {
+    const uint32_t off = i * static_cast<uint32_t>(sizeof(T));
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+
 struct NodeData {
     uint2 m;
     uint4 r32;
@@ -117,11 +131,10 @@ struct NodeData {
 template <bool kMixed, bool kMu0>
 __device__ __forceinline__ NodeData load_node(const Lane& L, uint32_t node, bool active) {
     const uint32_t x = active ? node : 0U;
-    __builtin_assume(x < (1U << kLinkBits));   // node ids fit the link field: 32-bit byte offsets
     NodeData d;
-    d.m = L.meta[x];
+    d.m = at(L.meta, x);
     if (kMixed) {
-        d.r32 = L.rec32[x];
+        d.r32 = at(L.rec32, x);
         d.r64 = make_double4(0.0, 0.0, 0.0, 0.0);
     } else {
         d.r32 = make_uint4(0U, 0U, 0U, 0U);
@@ -131,23 +144,62 @@ __device__ __forceinline__ NodeData load_node(const Lane& L, uint32_t node, bool
 }
 
 // Acceptance test and probability of a loaded node (test_acceptance_criterion, BarnesHutBase.h:72-114;
-// Kernel::calculate_attractiveness_to_connect, Kernel.h:61-75).
+// Kernel::calculate_attractiveness_to_connect, Kernel.h:61-75) for the lanes with `act`; other lanes
+// return kDiscard and p = 0.  The FP32 path is branch-free; exact coincidences and decisions inside
+// the guard band are redone in FP64 under one wave-uniform branch.
 template <bool kMixed, bool kMu0>
-__device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, const NodeData& d, uint32_t node,
+__device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, const NodeData& d, uint32_t node, bool act,
                                          double* p_out) {
-    *p_out = 0.0;
-    if (d.m.x == 0) {
-        return kDiscard;
-    }
+    const bool valid = act && d.m.x != 0;
     const bool leaf = (d.m.y >> kLinkBits) == 0;
-    const Eval e = kMixed ? eval_mixed_rec(a, L, d.r32, node, leaf) : eval_exact_rec(a, L, d.r64, leaf);
-    if (e.status == kAccept) {
-        const bool self = leaf && (d.m.y & kLinkMask) == L.src;
-        if (!self) {
-            *p_out = probability<kMixed, kMu0>(a, e, d.m.x, kGauss);
+    const bool self = leaf && (d.m.y & kLinkMask) == L.src;
+    if (kMixed) {
+        const int32_t ix = static_cast<int32_t>(d.r32.x) - L.qx;
+        const int32_t iy = static_cast<int32_t>(d.r32.y) - L.qy;
+        const int32_t iz = static_cast<int32_t>(d.r32.z) - L.qz;
+        const float dx = static_cast<float>(ix);
+        const float dy = static_cast<float>(iy);
+        const float dz = static_cast<float>(iz);
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        const float s2 = __uint_as_float(d.r32.w);
+        const float g = __builtin_fmaf(a.theta2f, d2, -s2);
+        const bool coinc = (ix | iy | iz) == 0;
+        const bool near = !leaf && fabsf(g) <= a.guard * s2;
+        int st = (leaf || g > 0.0f) ? kAccept : kExpand;
+        float x;
+        if (kMu0) {
+            x = d2 * a.k2q;
+        } else {
+            const float dd = sqrtf(d2) * a.inv_q - a.muf;
+            x = dd * dd * a.k2f;
         }
+        const float w = __builtin_amdgcn_exp2f(x) * static_cast<float>(d.m.x);
+        double p = static_cast<double>(w);
+        st = valid ? st : kDiscard;
+        p = (st == kAccept && !self) ? p : 0.0;
+        const bool need = valid && (coinc || near);
+        if (__ballot(need) != 0) {
+            if (need) {
+                const Eval e = eval_exact(a, L, node, 0, leaf);
+                st = e.status;
+                p = (st == kAccept && !self) ? probability<false>(a, e, d.m.x, kGauss) : 0.0;
+            }
+        }
+        *p_out = p;
+        return st;
+    } else {
+        double p = 0.0;
+        int st = kDiscard;
+        if (valid) {
+            const Eval e = eval_exact_rec(a, L, d.r64, leaf);
+            st = e.status;
+            if (st == kAccept && !self) {
+                p = probability<false>(a, e, d.m.x, kGauss);
+            }
+        }
+        *p_out = p;
+        return st;
     }
-    return e.status;
 }
 
 struct Bfs {
@@ -157,21 +209,83 @@ struct Bfs {
     bool overflow;
 };
 
-// Breadth-first traversal below `root` (inner, with vacant dendrites) filling the slot table at `base`.
-// The loads of the next step's 64 children are issued before the current step is evaluated.
+
+// One BFS step's input: the group's node from the level list and this lane's child (loaded).
+struct StepIn {
+    uint32_t k;      // level-list index of the group's node
+    uint32_t child;
+    bool gact;       // the group has a node
+    bool act;        // the lane has a child
+    NodeData nd;
+};
+
 template <bool kMixed, bool kMu0>
-__device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopShared& sh, uint64_t base) {
-    const SearchArgs& s = a.s;
+__device__ __forceinline__ StepIn fetch_step(const Lane& L, const CoopShared& sh, uint32_t cur, uint32_t lb, uint32_t le,
+                                             uint32_t k) {
+    const uint32_t j = lane() & 7U;
+    StepIn x;
+    x.k = k;
+    x.gact = k < le;
+    const uint32_t packed = sh.lvl[cur][x.gact ? k - lb : kLevelCap];
+    x.child = (packed & kLinkMask) + j;
+    x.act = x.gact && j < (packed >> kLinkBits);
+    x.nd = load_node<kMixed, kMu0>(L, x.child, x.act);
+    return x;
+}
+
+struct SlotTable {
+    uint32_t* my;
+    uint32_t* fx;
+    double* ms;
+    uint32_t dump;   // last slot: takes the stores of lanes with nothing to store
+};
+
+// Tests the 64 children of one step, sums the accepted probabilities per group, appends the
+// expanded children to the next level and records the group nodes' masses.  Returns false when the
+// slot table or the level list would overflow.
+template <bool kMixed, bool kMu0>
+__device__ __forceinline__ bool bfs_step(const CoopArgs& a, const Lane& L, CoopShared& sh, const SlotTable& t,
+                                         const StepIn& x, uint32_t cur, uint32_t nb, uint32_t& nslots, Bfs& r) {
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
+    double p;
+    const int st = eval_node<kMixed, kMu0>(a.s, L, x.nd, x.child, x.act, &p);
+    r.tests += static_cast<uint32_t>(__popcll(__builtin_amdgcn_ballot_w64(x.act)));
+    r.count += static_cast<uint32_t>(__popcll(__builtin_amdgcn_ballot_w64(st == kAccept)));
+    const double gsum = group_sum8(p);
+    const bool ex = st == kExpand;
+    const uint64_t em = __builtin_amdgcn_ballot_w64(ex);
+    const uint32_t ne = static_cast<uint32_t>(__popcll(em));
+    if (nslots + ne > t.dump || nslots + ne - nb > kLevelCap) {
+        return false;
+    }
+    const uint32_t slot = nslots + rank_below(em, ln);
+    at(t.my, ex ? slot : t.dump) = x.nd.m.y;
+    sh.lvl[cur ^ 1U][ex ? slot - nb : kLevelCap] = x.nd.m.y;
+    const bool gw = x.gact && j == 0;
+    const uint32_t gem = static_cast<uint32_t>((em >> (8U * q)) & 0xFFULL);
+    const uint32_t first = nslots + rank_below(em, 8U * q);
+    at(t.ms, gw ? x.k : t.dump) = gsum;
+    at(t.fx, gw ? x.k : t.dump) = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
+    nslots += ne;
+    return true;
+}
+
+// Breadth-first traversal below `root` (inner, with vacant dendrites) filling the slot table at `base`.
+// The loads of the next step's 64 children are issued before the current step is evaluated (two
+// alternating step buffers, so no registers are copied).
+template <bool kMixed, bool kMu0>
+__device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopShared& sh, uint64_t base) {
+    const uint32_t ln = lane();
+    const uint32_t q = ln >> 3U;
     Bfs r{ 0.0, 0, 0, false };
-    // the last slot of the table takes the stores of lanes with nothing to store (unconditional stores)
-    const uint32_t dump = a.cap - 1;
-    uint32_t* const tmy = a.t_my + base;
-    uint32_t* const tfx = a.t_fexp + base;
-    double* const tms = a.t_mass + base;
+    SlotTable t;
+    t.my = a.t_my + base;
+    t.fx = a.t_fexp + base;
+    t.ms = a.t_mass + base;
+    t.dump = a.cap - 1;
     if (ln == 0) {
-        tmy[0] = root_meta_y;
+        t.my[0] = root_meta_y;
         sh.lvl[0][0] = root_meta_y;
         sh.lvl_start[0] = 0;
     }
@@ -179,56 +293,34 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
     uint32_t lb = 0, le = 1, nslots = 1, lev = 0, cur = 0;
     while (lb < le) {
         const uint32_t nb = nslots;
-        uint32_t k = lb + q;
-        bool gact = k < le;
-        uint32_t packed = sh.lvl[cur][gact ? k - lb : kLevelCap];
-        uint32_t child = (packed & kLinkMask) + j;
-        bool act = gact && j < (packed >> kLinkBits);
-        NodeData nd = load_node<kMixed, kMu0>(L, child, act);
+        StepIn xa = fetch_step<kMixed, kMu0>(L, sh, cur, lb, le, lb + q);
         // Wait for the first step's data here, so that the loop header carries no pending load and
         // the stores of the previous step are never waited for.
         if (kMixed) {
-            __asm__ volatile("" ::"v"(nd.m.x), "v"(nd.m.y), "v"(nd.r32.x), "v"(nd.r32.y), "v"(nd.r32.z), "v"(nd.r32.w));
+            __asm__ volatile("" ::"v"(xa.nd.m.x), "v"(xa.nd.m.y), "v"(xa.nd.r32.x), "v"(xa.nd.r32.y), "v"(xa.nd.r32.z),
+                             "v"(xa.nd.r32.w));
         } else {
-            __asm__ volatile("" ::"v"(nd.m.x), "v"(nd.m.y), "v"(nd.r64.x), "v"(nd.r64.y), "v"(nd.r64.z), "v"(nd.r64.w));
+            __asm__ volatile("" ::"v"(xa.nd.m.x), "v"(xa.nd.m.y), "v"(xa.nd.r64.x), "v"(xa.nd.r64.y), "v"(xa.nd.r64.z),
+                             "v"(xa.nd.r64.w));
         }
-        for (uint32_t kb = lb; kb < le; kb += 8) {
-            // issue the next step's loads before evaluating this step
-            const uint32_t k2 = kb + 8 + q;
-            const bool gact2 = k2 < le;
-            const uint32_t packed2 = sh.lvl[cur][gact2 ? k2 - lb : kLevelCap];
-            const uint32_t child2 = (packed2 & kLinkMask) + j;
-            const bool act2 = gact2 && j < (packed2 >> kLinkBits);
-            const NodeData nd2 = load_node<kMixed, kMu0>(L, child2, act2);
-            int st = kDiscard;
-            double p = 0.0;
-            if (act) {
-                st = eval_node<kMixed, kMu0>(s, L, nd, child, &p);
-            }
-            r.tests += static_cast<uint32_t>(__popcll(__ballot(act)));
-            r.count += static_cast<uint32_t>(__popcll(__ballot(act && st == kAccept)));
-            const double gsum = group_sum8(p);
-            const bool ex = act && st == kExpand;
-            const uint64_t em = __ballot(ex);
-            const uint32_t ne = static_cast<uint32_t>(__popcll(em));
-            if (nslots + ne > a.cap - 1 || nslots + ne - nb > kLevelCap) {
+        for (uint32_t kb = lb; kb < le;) {
+            const StepIn xb = fetch_step<kMixed, kMu0>(L, sh, cur, lb, le, kb + 8 + q);
+            __builtin_amdgcn_sched_barrier(0);   // keep the prefetch ahead of this step's work
+            if (!bfs_step<kMixed, kMu0>(a, L, sh, t, xa, cur, nb, nslots, r)) {
                 r.overflow = true;
                 return r;
             }
-            const uint32_t slot = nslots + rank_below(em, ln);
-            tmy[ex ? slot : dump] = nd.m.y;
-            sh.lvl[cur ^ 1U][ex ? slot - nb : kLevelCap] = nd.m.y;
-            const bool gw = gact && j == 0;
-            const uint32_t gem = static_cast<uint32_t>((em >> (8U * q)) & 0xFFULL);
-            const uint32_t first = nslots + rank_below(em, 8U * q);
-            tms[gw ? k : dump] = gsum;
-            tfx[gw ? k : dump] = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
-            nslots += ne;
-            k = k2;
-            gact = gact2;
-            child = child2;
-            act = act2;
-            nd = nd2;
+            kb += 8;
+            if (kb >= le) {
+                break;
+            }
+            xa = fetch_step<kMixed, kMu0>(L, sh, cur, lb, le, kb + 8 + q);
+            __builtin_amdgcn_sched_barrier(0);
+            if (!bfs_step<kMixed, kMu0>(a, L, sh, t, xb, cur, nb, nslots, r)) {
+                r.overflow = true;
+                return r;
+            }
+            kb += 8;
         }
         __syncthreads();
         lb = nb;
@@ -245,19 +337,19 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
     }
     __syncthreads();
     // pull the masses of expanded children into their parents, deepest parents first
-    for (int t = static_cast<int>(lev) - 2; t >= 0; t--) {
-        const uint32_t sb = sh.lvl_start[t], se = sh.lvl_start[t + 1];
+    for (int lv = static_cast<int>(lev) - 2; lv >= 0; lv--) {
+        const uint32_t sb = sh.lvl_start[lv], se = sh.lvl_start[lv + 1];
         for (uint32_t i0 = sb; i0 < se; i0 += kWave) {
             const uint32_t sl = i0 + ln;
             if (sl < se) {
-                const uint32_t fx = tfx[sl];
+                const uint32_t fx = at(t.fx, sl);
                 const uint32_t ne = fx >> 28U;
                 if (ne != 0) {
                     const uint32_t f = fx & kSlotNone;
                     double m8[8];
 #pragma unroll
                     for (uint32_t c = 0; c < 8; c++) {
-                        m8[c] = c < ne ? tms[f + c] : 0.0;
+                        m8[c] = at(t.ms, f + (c < ne ? c : 0U));
                     }
                     double sum = 0.0;
 #pragma unroll
@@ -266,13 +358,13 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
                             sum += m8[c];
                         }
                     }
-                    tms[sl] += sum;
+                    at(t.ms, sl) += sum;
                 }
             }
         }
         __syncthreads();
     }
-    r.total = tms[0];
+    r.total = t.ms[0];
     return r;
 }
 
@@ -295,8 +387,8 @@ __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool g
             break;
         }
         // slot < cap always holds: it only advances to a checked child slot
-        uint32_t fx = tfx[slot];
-        uint32_t my = tmy[slot];
+        uint32_t fx = at(tfx, slot);
+        uint32_t my = at(tmy, slot);
         if (done) {
             fx = 0;
             my = 0;
@@ -309,13 +401,10 @@ __device__ uint32_t navigate(const CoopArgs& a, const Lane& L, double rr, bool g
         // the children and the masses of the expanded ones, loaded together
         const NodeData nd = load_node<kMixed, kMu0>(L, fc + j, act);
         const bool mact = !done && j < nexp && f != kSlotNone && f + j < a.cap;
-        const double mj_raw = tms[mact ? f + j : 0U];
+        const double mj_raw = at(tms, mact ? f + j : 0U);
         const double mj = mact ? mj_raw : 0.0;
-        int st = kDiscard;
-        double p = 0.0;
-        if (act) {
-            st = eval_node<kMixed, kMu0>(s, L, nd, fc + j, &p);
-        }
+        double p;
+        const int st = eval_node<kMixed, kMu0>(s, L, nd, fc + j, act, &p);
         *tests += static_cast<uint32_t>(__popcll(__ballot(act)));
         const uint64_t em = __ballot(act && st == kExpand);
         const uint32_t gem = static_cast<uint32_t>((em >> g0) & 0xFFULL);
@@ -496,12 +585,9 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
                     const uint32_t fc = my & kLinkMask;
                     const uint32_t nc = act_g ? (my >> kLinkBits) : 0U;
                     const bool act = act_g && j < nc;
-                    int st = kDiscard;
-                    double p = 0.0;
+                    double p;
                     const NodeData cd = load_node<kMixed, kMu0>(L, fc + j, act);
-                    if (act) {
-                        st = eval_node<kMixed, kMu0>(s, L, cd, fc + j, &p);
-                    }
+                    const int st = eval_node<kMixed, kMu0>(s, L, cd, fc + j, act, &p);
                     exec += static_cast<uint32_t>(__popcll(__ballot(act)));
                     const uint32_t gexp = static_cast<uint32_t>((__ballot(act && st == kExpand) >> g0) & 0xFFULL);
                     const uint32_t gacc = static_cast<uint32_t>((__ballot(act && st == kAccept) >> g0) & 0xFFULL);

@@ -26,10 +26,10 @@ struct SearchArgs {
     double scale64;      // -1 / sigma^2
     double mu;
     float theta2f;       // theta^2
-    float k2f;           // -log2(e) / sigma^2
-    float sigma_inv2f;   // 1 / sigma^2
+    float k2f;           // -log2(e) / sigma^2 (micrometres)
+    float k2q;           // -log2(e) / sigma^2 in squared fixed-point units
     float muf;
-    float inv_q[3];      // micrometres per fixed-point unit
+    float inv_q;         // micrometres per fixed-point unit (one scale for all axes)
     float guard;         // relative guard band of the FP32 acceptance test
     uint64_t seed;
     uint32_t step;
@@ -91,8 +91,10 @@ __device__ __forceinline__ Eval eval_exact(const SearchArgs& a, const Lane& L, u
     return eval_exact_rec(a, L, L.rec64[node], leaf);
 }
 
-// FP32 test on the fixed-point record; decisions inside the guard band (and exact coincidences)
-// are taken by the FP64 test.
+// FP32 test on the fixed-point record (x, y, z in fixed-point units, w = squared side in squared
+// units): squared distance d2 = dx^2 + dy^2 + dz^2 of the exact integer differences, accept iff
+// theta^2 d2 > side^2.  Decisions inside the guard band (and exact coincidences) are taken by the
+// FP64 test.
 __device__ __forceinline__ Eval eval_mixed_rec(const SearchArgs& a, const Lane& L, uint4 q, uint32_t node, bool leaf) {
     const int32_t ix = static_cast<int32_t>(q.x) - L.qx;
     const int32_t iy = static_cast<int32_t>(q.y) - L.qy;
@@ -100,9 +102,9 @@ __device__ __forceinline__ Eval eval_mixed_rec(const SearchArgs& a, const Lane& 
     if ((ix | iy | iz) == 0) {
         return eval_exact(a, L, node, 0, leaf);
     }
-    const float dx = static_cast<float>(ix) * a.inv_q[0];
-    const float dy = static_cast<float>(iy) * a.inv_q[1];
-    const float dz = static_cast<float>(iz) * a.inv_q[2];
+    const float dx = static_cast<float>(ix);
+    const float dy = static_cast<float>(iy);
+    const float dz = static_cast<float>(iz);
     const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
     Eval e;
     e.exact = false;
@@ -112,8 +114,7 @@ __device__ __forceinline__ Eval eval_mixed_rec(const SearchArgs& a, const Lane& 
         e.status = kAccept;
         return e;
     }
-    const float side = __uint_as_float(q.w);
-    const float s2 = side * side;
+    const float s2 = __uint_as_float(q.w);
     const float g = __builtin_fmaf(a.theta2f, d2, -s2);
     if (fabsf(g) <= a.guard * s2) {
         return eval_exact(a, L, node, 0, leaf);
@@ -143,9 +144,9 @@ __device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e
     }
     float x;
     if (kMuZero || a.muf == 0.0f) {
-        x = e.d2f * a.k2f;
+        x = e.d2f * a.k2q;
     } else {
-        const float d = sqrtf(e.d2f) - a.muf;
+        const float d = sqrtf(e.d2f) * a.inv_q - a.muf;
         x = d * d * a.k2f;
     }
     return static_cast<double>(exp2f(x)) * static_cast<double>(cnt);

@@ -330,6 +330,14 @@ struct QuantArgs {
     int enabled;
 };
 
+// Squared cell side in squared fixed-point units (the FP32 acceptance test compares it with the
+// squared fixed-point distance).
+__device__ __forceinline__ uint32_t quant_side2(double side, double scale) {
+#pragma clang fp contract(off)
+    const double sq = side * scale;
+    return __float_as_uint(static_cast<float>(sq * sq));
+}
+
 __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __restrict__ nbox,
                                double4* __restrict__ rec0, double4* __restrict__ rec1, uint2* __restrict__ meta0,
                                uint2* __restrict__ meta1, uint4* __restrict__ q0, uint4* __restrict__ q1,
@@ -383,7 +391,7 @@ __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __re
         if (qa.enabled) {
             uint4* q = s == 0 ? q0 : q1;
             q[i] = make_uint4(quantize(r.x, qa.lo[0], qa.scale[0]), quantize(r.y, qa.lo[1], qa.scale[1]),
-                              quantize(r.z, qa.lo[2], qa.scale[2]), __float_as_uint(static_cast<float>(r.w)));
+                              quantize(r.z, qa.lo[2], qa.scale[2]), quant_side2(r.w, qa.scale[0]));
         }
     }
 }
@@ -396,7 +404,7 @@ __global__ void k_quant_leaves(uint64_t m, const double4* __restrict__ rec, cons
     }
     const double4 r = rec[i];
     const uint4 v = make_uint4(quantize(r.x, qa.lo[0], qa.scale[0]), quantize(r.y, qa.lo[1], qa.scale[1]),
-                               quantize(r.z, qa.lo[2], qa.scale[2]), __float_as_uint(static_cast<float>(r.w)));
+                               quantize(r.z, qa.lo[2], qa.scale[2]), quant_side2(r.w, qa.scale[0]));
     q0[i] = v;
     q1[i] = v;
     (void)meta;
