@@ -156,7 +156,8 @@ def main(argv=None):
     ap.add_argument("--cpu-sample", type=int, default=20_000, help="source neurons searched by the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp64", "fp32_mixed"], default=None)
+    ap.add_argument("--only", choices=["fp64", "fp32_mixed", "fp64_plain"], default=None)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 (one lane per source) run")
     ap.add_argument("--kernel", choices=["coop", "lane"], default="coop",
                     help="search kernel: wavefront-cooperative (default) or one lane per source in list order")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -183,24 +184,28 @@ def main(argv=None):
     d_vac = [int(a.data_ptr()) for a in d_arrays]
 
     res = {}
-    precs = [("fp32_mixed", RL_PREC_FP32_MIXED), ("fp64", RL_PREC_FP64)]
+    kflags = RL_FLAG_EXACT_ORDER if args.kernel == "lane" else 0
+    # fp64_plain: the FP64 path with one lane per source and the CPU path's list order (the plain
+    # FP64 GPU method the FP32-mixed path is compared with)
+    precs = [("fp32_mixed", RL_PREC_FP32_MIXED, kflags), ("fp64", RL_PREC_FP64, kflags)]
+    if not args.no_plain:
+        precs.append(("fp64_plain", RL_PREC_FP64, RL_FLAG_EXACT_ORDER))
     if args.only:
         precs = [p for p in precs if p[0] == args.only]
-    for name, prec in precs:
-        res[name] = run_precision(net, prec, args.steps, args.warmup, args.seed + 1, d_vac,
-                                  flags=RL_FLAG_EXACT_ORDER if args.kernel == "lane" else 0)
+    for name, prec, flags in precs:
+        res[name] = run_precision(net, prec, args.steps, args.warmup, args.seed + 1, d_vac, flags=flags)
         r = res[name]
         log(f"{name}: {r['searches_per_s']:.4g} searches/s whole step, {r['ms_per_step']:.1f} ms/step, "
             f"search kernel {r['search_kernel_ms']:.1f} ms, phases {['%.3f' % x for x in r['phase_s']]} s")
 
-    main_name = "fp32_mixed" if "fp32_mixed" in res else "fp64"
+    main_name = "fp32_mixed" if "fp32_mixed" in res else ("fp64" if "fp64" in res else "fp64_plain")
     main_r = res[main_name]
-    b_node = B_NODE[main_name]
+    b_node = B_NODE["fp32_mixed" if main_name == "fp32_mixed" else "fp64"]
     # algorithmic (logical) bytes: every acceptance test of the reference's traversals reads one node
     # record, every search reads its source and writes its result
     bytes_per_launch = main_r["node_tests_ref_per_step"] * b_node + main_r["searches_per_step"] * B_IO
     achieved = bytes_per_launch / (1e-3 * main_r["search_kernel_ms"]) / 1e9
-    kname = "k_search_coop" if args.kernel == "coop" else "k_search"
+    kname = "k_level1" if args.kernel == "coop" and main_name != "fp64_plain" else "k_search"
     traffic = read_traffic(kname, main_name, args.neurons)
     out = {
         "metric": METRIC,
@@ -252,6 +257,15 @@ def main(argv=None):
             "requests_fp32": int(b["counts"].sum()),
             "mean_synapses_per_neuron_fp64": a["created0"] / args.neurons,
             "mean_synapses_per_neuron_fp32": b["created0"] / args.neurons,
+            "synapses_rel_diff": abs(b["created0"] - a["created0"]) / max(1, a["created0"]),
+        }
+    if "fp64_plain" in res and "fp32_mixed" in res:
+        a, b = res["fp64_plain"], res["fp32_mixed"]
+        valid = a["targets"] != 0xFFFFFFFF
+        out["fp32_vs_fp64_plain"] = {
+            "speedup_whole_step": a["ms_per_step"] / b["ms_per_step"],
+            "speedup_search": a["search_kernel_ms"] / b["search_kernel_ms"],
+            "pick_divergence": float(np.mean(a["targets"][valid] != b["targets"][valid])),
             "synapses_rel_diff": abs(b["created0"] - a["created0"]) / max(1, a["created0"]),
         }
     if not args.no_cpu:

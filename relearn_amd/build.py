@@ -53,6 +53,25 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
+HOST_DIR = os.path.join(HERE, "host")
+HOST_EXAMPLE = os.path.join(HOST_DIR, "update_example")
+
+
+def build_host_example(verbose: bool = False) -> str:
+    """Compiles the C++ adapter example (host code only, g++) against librelearn_bh.so."""
+    cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
+    tmp = HOST_EXAMPLE + ".tmp"
+    cmd = [cxx, "-std=c++20", "-O2", "-Wall", "-Wextra", os.path.join(HOST_DIR, "update_example.cpp"), "-o", tmp,
+           f"-L{HERE}", "-lrelearn_bh", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host example build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, HOST_EXAMPLE)
+    return HOST_EXAMPLE
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4, 8)
@@ -66,6 +85,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    build_host_example(verbose)
     return LIB
 
 

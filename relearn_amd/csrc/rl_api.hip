@@ -86,7 +86,8 @@ void free_all(rl_bh_ctx* c) {
     dfree(&c->coop_overflow);
     dfree(&c->redo);
     dfree(&c->coop_node);
-    dfree(&c->coop_parent);
+    dfree(&c->slow_items);
+    dfree(&c->pipe);
     dfree(&c->coop_fexp);
     dfree(&c->coop_mass);
     c->coop_cap = 0;
@@ -242,6 +243,7 @@ int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signa
     RL_TRY(dalloc(c, &c->coop_cursor, 10));
     RL_TRY(dalloc(c, &c->coop_overflow, n));
     RL_TRY(dalloc(c, &c->redo, n));
+    RL_TRY(dalloc(c, &c->pipe, 8));
     {
         int cus = 0;
         RL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
@@ -249,7 +251,6 @@ int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signa
         c->coop_cap = 2048;
         const size_t slots = static_cast<size_t>(c->coop_blocks) * c->coop_cap;
         RL_TRY(dalloc(c, &c->coop_node, slots));
-        RL_TRY(dalloc(c, &c->coop_parent, slots));
         RL_TRY(dalloc(c, &c->coop_fexp, slots));
         RL_TRY(dalloc(c, &c->coop_mass, slots));
     }

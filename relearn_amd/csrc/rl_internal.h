@@ -25,7 +25,8 @@ struct Status {
 // Device octree in breadth-first order (level-major, Morton order inside a level).  Children of a
 // node are contiguous.  Per dendrite type s (0 excitatory, 1 inhibitory) the search reads
 //   rec64[s][i] = (centroid x, y, z, max edge)          FP64 path, 32 B
-//   rec32[s][i] = (centroid x, y, z as 31-bit fixed point, max edge as float bits)  FP32 path, 16 B
+//   rec32[s][i] = (centroid x, y, z as 31-bit fixed point, squared max edge in squared fixed-point
+//                  units as float bits)                 FP32 path, 16 B
 //   meta[s][i]  = (vacant count, link | nchild << 28)  8 B
 // link is the first child for inner nodes and the neuron id for leaves.
 struct Tree {
@@ -98,6 +99,7 @@ struct rl_bh_ctx {
     uint64_t last_tests = 0;
     uint64_t last_tests_ref = 0;
     uint64_t last_lane_sources = 0;  // sources handled by the per-lane kernel in the last search
+    uint64_t last_slow_descents = 0; // descents handed to the per-lane general traversal
 
     // persistent scratch (grown on demand, never per call)
     uint64_t* wide = nullptr;        // n + 1
@@ -120,11 +122,13 @@ struct rl_bh_ctx {
     uint32_t* coop_overflow = nullptr;  // n
     uint32_t* redo = nullptr;           // n
     uint32_t* coop_node = nullptr;   // blocks * cap
-    uint32_t* coop_parent = nullptr;
     uint32_t* coop_fexp = nullptr;
     double* coop_mass = nullptr;
     uint32_t coop_cap = 0;
     uint32_t coop_blocks = 0;
+    // (slot, source, node, level) of descents that need the general per-lane traversal (slots_cap)
+    uint4* slow_items = nullptr;
+    unsigned long long* pipe = nullptr;  // [0] slow items
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
     rl::Neurons neurons;
@@ -173,6 +177,7 @@ int export_tree(rl_bh_ctx* c, uint8_t* level, uint8_t* nchild, uint32_t* link, u
 int find_targets(rl_bh_ctx* c, uint32_t step);
 struct SearchArgs;
 int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow);
+int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, uint64_t count);
 int compact_active(rl_bh_ctx* c, uint64_t* nwork);
 
 // commit (rl_commit.hip)

@@ -345,6 +345,72 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
     atomicAdd(&a.counters[2], static_cast<unsigned long long>(searched));
 }
 
+// Descents that the cooperative kernels hand over (a level that needs expansion or the reference's
+// fallbacks): one lane continues the search of one slot from (node, level) with the general
+// traversal, as the descent loop of k_search does.
+template <bool kMixed>
+__global__ void __launch_bounds__(kBlock) k_descend_lane(SearchArgs a, const uint4* __restrict__ items, uint64_t count) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= count) {
+        return;
+    }
+    uint32_t* stk = lds_stack + threadIdx.x;
+    const uint32_t stride = blockDim.x;
+    const uint4 it = items[t];
+    const uint32_t slot = it.x, src = it.y;
+    uint32_t node = it.z;
+    Lane L;
+    L.src = src;
+    L.sig = a.signal[src] == 0 ? 0 : 1;
+    lane_bind(a, L);
+    L.px = a.pos[src];
+    L.py = a.pos[a.n + src];
+    L.pz = a.pos[2 * a.n + src];
+    if (kMixed) {
+        L.qx = static_cast<int32_t>(a.qpos[src]);
+        L.qy = static_cast<int32_t>(a.qpos[a.n + src]);
+        L.qz = static_cast<int32_t>(a.qpos[2 * a.n + src]);
+    } else {
+        L.qx = L.qy = L.qz = 0;
+    }
+    const uint32_t j = slot - static_cast<uint32_t>(a.slot_off[src]);
+    uint64_t exec = 0;
+    uint32_t tests = 0;
+    bool ok = true;
+    for (uint32_t level = it.w;; level++) {
+        if (node >= a.m || level > kMaxLevels + 2) {
+            ok = false;
+            break;
+        }
+        const uint2 m = L.meta[node];
+        if ((m.y >> kLinkBits) == 0) {
+            node = m.y & kLinkMask;
+            break;
+        }
+        const ListSum ls = list_sum<kMixed, true>(a, L, node, stk, stride);
+        exec += ls.tests_exec;
+        tests += ls.tests_ref;
+        if (ls.count == 0 || ls.sum == 0.0) {
+            ok = false;
+            break;
+        }
+        PickVisitor pv;
+        pv.nd = 1;
+#pragma unroll
+        for (int k = 0; k < kDrawsPerPass; k++) {
+            pv.pick[k] = kNone;
+            pv.r[k] = 0.0;
+        }
+        pv.r[0] = draw_to_uniform(pick_draw(a.seed, src, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
+        exec += list_pick<kMixed, true>(a, L, node, ls, pv, stk, stride);
+        node = pv.pick[0];
+    }
+    a.slot_tests[slot] += tests;
+    a.slot_target[slot] = ok ? node : kNone;
+    atomicAdd(&a.counters[0], static_cast<unsigned long long>(exec));
+}
+
 __global__ void k_widen(uint64_t n, const uint32_t* __restrict__ v, uint64_t* __restrict__ out) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i < n) {
@@ -383,6 +449,18 @@ __global__ void k_active_flags(uint64_t n, const uint32_t* __restrict__ order, c
 }
 
 }  // namespace
+
+int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, uint64_t count) {
+    hipStream_t s = c->stream;
+    const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
+    if (c->cfg.precision == RL_PREC_FP32_MIXED) {
+        k_descend_lane<true><<<grid_for(count), kBlock, lds, s>>>(sa, items, count);
+    } else {
+        k_descend_lane<false><<<grid_for(count), kBlock, lds, s>>>(sa, items, count);
+    }
+    RL_HIP(c, hipGetLastError());
+    return RL_OK;
+}
 
 // Active sources (enabled, vacant axons > 0) in Morton order into c->coop_work.
 int compact_active(rl_bh_ctx* c, uint64_t* nwork) {
@@ -425,9 +503,13 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     uint64_t slots = 0;
     RL_HIP(c, hipMemcpyAsync(&slots, c->slot_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     RL_HIP(c, hipStreamSynchronize(s));
+    if (slots >= (1ULL << 32)) {
+        return fail(c, RL_ELIMIT, "more than 2^32 - 1 vacant axons in one update");
+    }
     if (slots > c->slots_cap) {
         RL_TRY(dalloc(c, &c->slot_target, slots));
         RL_TRY(dalloc(c, &c->slot_tests, slots));
+        RL_TRY(dalloc(c, &c->slow_items, slots));
         c->slots_cap = slots;
     }
     c->slots = slots;

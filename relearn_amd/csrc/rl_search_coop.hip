@@ -53,6 +53,8 @@ struct CoopArgs {
     uint32_t cap;
     uint32_t* overflow;
     unsigned int* n_overflow;
+    uint4* slow;                  // (slot, source, node, level) of descents for the per-lane kernel
+    unsigned long long* n_slow;
 };
 
 struct CoopShared {
@@ -477,20 +479,101 @@ __device__ ListResult list_of(const CoopArgs& a, const Lane& L, uint32_t root, C
     return lr;
 }
 
+constexpr uint32_t kOverflowMark = 0xFFFFFFFFU;   // found[src] of a source handed to the per-lane kernel
+
 // Per-draw descent state, identical in the 8 lanes of the draw's group.
 enum : int { kDrawActive = 0, kDrawDone = 1, kDrawFailed = 2, kDrawSlow = 3, kDrawIdle = 4 };
 
+// Levels >= 2 of the groups' draws (BarnesHutBase::find_target_neuron, :223-255), one draw per group
+// of 8 lanes, each lane testing one child of the draw's current node.  Below an accepted node the
+// candidate list is normally the node's children, all accepted; the group then sums, draws and picks
+// on its own.  A level that needs expansion, or whose Gaussian sum is 0 (the reference's fallbacks),
+// ends in kDrawSlow with (node, level) where the general per-lane traversal takes over.
 template <bool kMixed, bool kMu0>
-__global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
+__device__ __forceinline__ int descend_fast(const SearchArgs& s, const Lane& L, uint32_t jd, int state, uint32_t& node,
+                                            uint32_t& level, uint32_t& tests, unsigned long long& exec) {
+    const uint32_t ln = lane();
+    const uint32_t j = ln & 7U;
+    const uint32_t g0 = ln & ~7U;
+    for (uint32_t guard = 0; guard < kMaxLevels + 2; guard++) {
+        uint32_t my = 0;
+        if (state == kDrawActive) {
+            if (node >= s.m) {
+                state = kDrawFailed;
+            } else {
+                const uint2 mn = L.meta[node];
+                my = mn.y;
+                if ((my >> kLinkBits) == 0) {
+                    node = my & kLinkMask;  // leaf: the target neuron
+                    state = kDrawDone;
+                } else if (mn.x == 0) {
+                    state = kDrawFailed;  // no vacant dendrites: empty list
+                }
+            }
+        }
+        if (__builtin_amdgcn_ballot_w64(state == kDrawActive) == 0) {
+            break;
+        }
+        const bool act_g = state == kDrawActive;
+        const uint32_t fc = my & kLinkMask;
+        const uint32_t nc = act_g ? (my >> kLinkBits) : 0U;
+        const bool act = act_g && j < nc;
+        double p;
+        const NodeData cd = load_node<kMixed, kMu0>(L, fc + j, act);
+        const int st = eval_node<kMixed, kMu0>(s, L, cd, fc + j, act, &p);
+        exec += static_cast<uint32_t>(__popcll(__builtin_amdgcn_ballot_w64(act)));
+        const uint32_t gexp = static_cast<uint32_t>((__builtin_amdgcn_ballot_w64(st == kExpand) >> g0) & 0xFFULL);
+        const uint32_t gacc = static_cast<uint32_t>((__builtin_amdgcn_ballot_w64(st == kAccept) >> g0) & 0xFFULL);
+        const double gsum = __shfl(group_sum8(p), static_cast<int>(g0), kWave);
+        if (act_g) {
+            if (gexp != 0 || (gacc != 0 && gsum == 0.0)) {
+                state = kDrawSlow;
+            } else {
+                tests += nc;
+                if (gacc == 0) {
+                    state = kDrawFailed;
+                }
+            }
+        }
+        const bool pick = state == kDrawActive;
+        const double r = pick ? draw_to_uniform(pick_draw(s.seed, L.src, s.step, jd, level), nextafter(gsum, gsum + 0.00001))
+                              : 0.0;
+        const double pa = st == kAccept ? p : 0.0;
+        const double cum = group_suffix8(pa);
+        const uint32_t gcm = static_cast<uint32_t>((__builtin_amdgcn_ballot_w64(pa > 0.0 && cum >= r) >> g0) & 0xFFULL);
+        const uint32_t gpm = static_cast<uint32_t>((__builtin_amdgcn_ballot_w64(pa > 0.0) >> g0) & 0xFFULL);
+        if (pick) {
+            int jstar = -1;
+            if (gcm != 0) {
+                jstar = 31 - __clz(static_cast<int>(gcm));
+            } else if (gpm != 0) {
+                jstar = __ffs(static_cast<int>(gpm)) - 1;
+            }
+            if (jstar < 0) {
+                state = kDrawFailed;
+            } else {
+                node = fc + static_cast<uint32_t>(jstar);
+                level++;
+            }
+        }
+    }
+    return state;
+}
+
+// One wave per source builds the candidate list of the root once (BarnesHutBase::find_target_neuron
+// re-derives it for every axon, BarnesHutBase.h:223-255), resolves the source's level-1 draws against
+// it 8 at a time, and continues each draw's descent in its group.  Results go to slot_target; descents
+// that need the general traversal are queued for the per-lane kernel.
+template <bool kMixed, bool kMu0>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(5, 8))) k_level1(CoopArgs a) {
     __shared__ CoopShared sh;
     const SearchArgs& s = a.s;
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
-    const uint32_t g0 = 8U * q;
     const uint64_t base = static_cast<uint64_t>(blockIdx.x) * a.cap;
     int seg = static_cast<int>(blockIdx.x % kXcds);
     int tried = 0;
-    unsigned long long acc_exec = 0, acc_ref = 0, acc_searches = 0;
+    unsigned long long acc_exec = 0;
     while (true) {
         uint64_t w = 0;
         if (ln == 0) {
@@ -506,7 +589,7 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
         }
         const uint32_t src = __builtin_amdgcn_readfirstlane(a.work[w]);
         const uint32_t na = __builtin_amdgcn_readfirstlane(s.vac_axon[src]);
-        const uint64_t slot0 = s.slot_off[src];
+        const uint32_t slot0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s.slot_off[src]));
         // source data is wave-uniform: keep it in scalar registers
         Lane L;
         L.src = src;
@@ -522,223 +605,117 @@ __global__ void __launch_bounds__(kWave) k_search_coop(CoopArgs a) {
         } else {
             L.qx = L.qy = L.qz = 0;
         }
-        uint64_t exec = 0, ref = 0;
-        bool overflow = false;
-        uint32_t found = 0, searched = 0;
         const ListResult l1 = list_of<kMixed, kMu0>(a, L, 0, sh, base);
-        exec += l1.tests;
+        acc_exec += l1.tests;
         if (l1.overflow) {
-            overflow = true;
+            if (ln == 0) {
+                const unsigned int k = atomicAdd(a.n_overflow, 1U);
+                a.overflow[k] = src;
+                s.found[src] = kOverflowMark;
+            }
         } else if (l1.count == 0) {
             if (ln == 0) {
-                s.slot_tests[slot0] = l1.tests;
+                s.slot_tests[slot0] = l1.tests;   // slot_target[slot0] stays kNone: one failed search
             }
-            ref += l1.tests;
-            searched = 1;
         } else {
             const double next1 = nextafter(l1.total, l1.total + 0.00001);
-            // level 1: all draws against the one slot table, 8 at a time
             uint32_t nav_tests = 0;
             for (uint32_t j0 = 0; j0 < na; j0 += 8) {
                 const bool gact = j0 + q < na;
                 const double r = gact ? draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + q, 1U), next1) : 0.0;
-                const uint32_t x = navigate<kMixed, kMu0>(a, L, r, gact, base, &nav_tests);
-                if (gact && j == 0) {
-                    s.slot_target[slot0 + j0 + q] = x;
+                uint32_t node = navigate<kMixed, kMu0>(a, L, r, gact, base, &nav_tests);
+                uint32_t level = 2, tests = l1.tests;
+                int state = !gact ? kDrawIdle : (node == kNone ? kDrawFailed : kDrawActive);
+                state = descend_fast<kMixed, kMu0>(s, L, j0 + q, state, node, level, tests, acc_exec);
+                const bool lead = gact && j == 0;
+                const uint64_t sm = __builtin_amdgcn_ballot_w64(lead && state == kDrawSlow);
+                unsigned long long at0 = 0;
+                if (ln == 0 && sm != 0) {
+                    at0 = atomicAdd(a.n_slow, static_cast<unsigned long long>(__popcll(sm)));
+                }
+                at0 = __shfl(at0, 0, kWave);
+                if (lead) {
+                    const uint32_t slot = slot0 + j0 + q;
+                    s.slot_tests[slot] = tests;
+                    if (state == kDrawSlow) {
+                        a.slow[at0 + rank_below(sm, ln)] = make_uint4(slot, src, node, level);
+                    } else {
+                        s.slot_target[slot] = state == kDrawDone ? node : kNone;
+                    }
                 }
             }
-            exec += nav_tests;
-            __syncthreads();
-            // descents, 8 axons at a time; draw j0+q is handled by group q
-            bool stop = false;
-            for (uint32_t j0 = 0; j0 < na && !stop && !overflow; j0 += 8) {
-                const uint32_t jd = j0 + q;
-                const bool gact = jd < na;
-                uint32_t node = gact ? s.slot_target[slot0 + jd] : kNone;
-                int state = gact ? kDrawActive : kDrawIdle;
-                uint32_t level = 2;
-                uint32_t tests_d = l1.tests;
-                if (gact && node == kNone) {
-                    state = kDrawFailed;
-                }
-                // fast levels: the group tests the children of its node; no expansion expected
-                for (uint32_t guard = 0; guard < kMaxLevels + 2; guard++) {
-                    uint32_t my = 0;
-                    if (state == kDrawActive) {
-                        if (node >= s.m) {
-                            state = kDrawFailed;
-                        } else {
-                            const uint2 mn = L.meta[node];
-                            my = mn.y;
-                            if ((my >> kLinkBits) == 0) {
-                                node = my & kLinkMask;  // leaf: the target neuron
-                                state = kDrawDone;
-                            } else if (mn.x == 0) {
-                                state = kDrawFailed;  // no vacant dendrites: empty list
-                            }
-                        }
-                    }
-                    if (__ballot(state == kDrawActive) == 0) {
-                        break;
-                    }
-                    const bool act_g = state == kDrawActive;
-                    const uint32_t fc = my & kLinkMask;
-                    const uint32_t nc = act_g ? (my >> kLinkBits) : 0U;
-                    const bool act = act_g && j < nc;
-                    double p;
-                    const NodeData cd = load_node<kMixed, kMu0>(L, fc + j, act);
-                    const int st = eval_node<kMixed, kMu0>(s, L, cd, fc + j, act, &p);
-                    exec += static_cast<uint32_t>(__popcll(__ballot(act)));
-                    const uint32_t gexp = static_cast<uint32_t>((__ballot(act && st == kExpand) >> g0) & 0xFFULL);
-                    const uint32_t gacc = static_cast<uint32_t>((__ballot(act && st == kAccept) >> g0) & 0xFFULL);
-                    const double gsum = __shfl(group_sum8(p), static_cast<int>(g0), kWave);
-                    if (act_g) {
-                        if (gacc != 0 && gexp == 0 && gsum == 0.0) {
-                            overflow = true;  // Gaussian sum of 0: the per-lane kernel applies the fallback
-                            state = kDrawIdle;
-                        } else if (gexp != 0) {
-                            state = kDrawSlow;  // needs the general traversal
-                        } else {
-                            tests_d += nc;
-                            if (gacc == 0) {
-                                state = kDrawFailed;
-                            }
-                        }
-                    }
-                    const bool pick = state == kDrawActive;
-                    const double r = pick ? draw_to_uniform(pick_draw(s.seed, src, s.step, jd, level),
-                                                            nextafter(gsum, gsum + 0.00001))
-                                          : 0.0;
-                    const double pa = (act && st == kAccept) ? p : 0.0;
-                    const double cum = group_suffix8(pa);
-                    const uint32_t gcm = static_cast<uint32_t>((__ballot(pa > 0.0 && cum >= r) >> g0) & 0xFFULL);
-                    const uint32_t gpm = static_cast<uint32_t>((__ballot(pa > 0.0) >> g0) & 0xFFULL);
-                    if (pick) {
-                        int jstar = -1;
-                        if (gcm != 0) {
-                            jstar = 31 - __clz(static_cast<int>(gcm));
-                        } else if (gpm != 0) {
-                            jstar = __ffs(static_cast<int>(gpm)) - 1;
-                        }
-                        if (jstar < 0) {
-                            state = kDrawFailed;
-                        } else {
-                            node = fc + static_cast<uint32_t>(jstar);
-                            level++;
-                        }
-                    }
-                }
-                overflow = __ballot(overflow) != 0;
-                // general traversal for the draws that need it, in axon order
-                for (uint32_t g = 0; g < 8 && !overflow; g++) {
-                    const int sg = __shfl(state, static_cast<int>(8U * g), kWave);
-                    if (sg != kDrawSlow) {
-                        continue;
-                    }
-                    uint32_t nd = __shfl(node, static_cast<int>(8U * g), kWave);
-                    uint32_t lv = __shfl(level, static_cast<int>(8U * g), kWave);
-                    uint32_t td = __shfl(tests_d, static_cast<int>(8U * g), kWave);
-                    int res = kDrawSlow;
-                    for (uint32_t guard = 0; guard < kMaxLevels + 2 && res == kDrawSlow; guard++) {
-                        if (nd >= s.m) {
-                            res = kDrawFailed;
-                            break;
-                        }
-                        const uint2 m = L.meta[nd];
-                        if ((m.y >> kLinkBits) == 0) {
-                            nd = m.y & kLinkMask;
-                            res = kDrawDone;
-                            break;
-                        }
-                        const ListResult ls = list_of<kMixed, kMu0>(a, L, nd, sh, base);
-                        exec += ls.tests;
-                        td += ls.tests;
-                        if (ls.overflow) {
-                            overflow = true;
-                            break;
-                        }
-                        if (ls.count == 0) {
-                            res = kDrawFailed;
-                            break;
-                        }
-                        const double r = draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + g, lv),
-                                                         nextafter(ls.total, ls.total + 0.00001));
-                        uint32_t nt = 0;
-                        nd = navigate<kMixed, kMu0>(a, L, r, q == 0, base, &nt);
-                        nd = __shfl(nd, 0, kWave);
-                        exec += nt;
-                        lv++;
-                        if (nd == kNone) {
-                            res = kDrawFailed;
-                        }
-                    }
-                    if (q == g) {
-                        node = nd;
-                        state = res;
-                        tests_d = td;
-                    }
-                }
-                if (overflow) {
-                    break;
-                }
-                // results in axon order, stopping at the first failure (find_target_neurons, :283-289)
-                for (uint32_t g = 0; g < 8 && j0 + g < na; g++) {
-                    const int sg = __shfl(state, static_cast<int>(8U * g), kWave);
-                    const uint32_t ng = __shfl(node, static_cast<int>(8U * g), kWave);
-                    const uint32_t tg = __shfl(tests_d, static_cast<int>(8U * g), kWave);
-                    searched++;
-                    ref += tg;
-                    if (ln == 0) {
-                        s.slot_tests[slot0 + j0 + g] = tg;
-                        s.slot_target[slot0 + j0 + g] = sg == kDrawDone ? ng : kNone;
-                    }
-                    if (sg != kDrawDone) {
-                        stop = true;
-                        break;
-                    }
-                    found++;
-                }
-            }
-        }
-        if (overflow) {
-            if (ln == 0) {
-                const unsigned int k = atomicAdd(a.n_overflow, 1U);
-                a.overflow[k] = src;
-            }
-        } else {
-            if (ln == 0) {
-                s.found[src] = found;
-                s.searched[src] = searched;
-            }
-            acc_exec += exec;
-            acc_ref += ref;
-            acc_searches += searched;
+            acc_exec += nav_tests;
         }
         __syncthreads();
     }
     if (ln == 0) {
         atomicAdd(&s.counters[0], acc_exec);
-        atomicAdd(&s.counters[1], acc_ref);
-        atomicAdd(&s.counters[2], acc_searches);
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, kWave);
+    }
+    return v;
+}
+
+// Per source, in axon order: a search that fails ends the source's searches (find_target_neurons,
+// BarnesHutBase.h:283-289).  Counts the searches, the successful ones and the reference's tests.
+__global__ void __launch_bounds__(256) k_finalize(SearchArgs s, const uint32_t* __restrict__ work, uint64_t nwork) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    unsigned long long ref = 0, searched_n = 0;
+    if (t < nwork) {
+        const uint32_t src = work[t];
+        if (s.found[src] != kOverflowMark) {
+            const uint32_t na = s.vac_axon[src];
+            const uint64_t slot0 = s.slot_off[src];
+            uint32_t found = 0, searched = 0;
+            uint32_t jj = 0;
+            for (; jj < na; jj++) {
+                searched++;
+                ref += s.slot_tests[slot0 + jj];
+                if (s.slot_target[slot0 + jj] == kNone) {
+                    break;
+                }
+                found++;
+            }
+            // draws after the first failure were resolved speculatively: the reference never makes them
+            for (jj = jj + 1; jj < na; jj++) {
+                s.slot_target[slot0 + jj] = kNone;
+                s.slot_tests[slot0 + jj] = 0;
+            }
+            s.found[src] = found;
+            s.searched[src] = searched;
+            searched_n = searched;
+        }
+    }
+    ref = wave_sum(ref);
+    searched_n = wave_sum(searched_n);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicAdd(&s.counters[1], ref);
+        atomicAdd(&s.counters[2], searched_n);
     }
 }
 
 }  // namespace
 
-// Launches the cooperative search over the active sources; sources that overflow the slot table
-// are returned in c->coop_overflow (count in *n_overflow) for the per-lane kernel.
+// Cooperative search over the active sources: the search kernel, the per-lane general descents it
+// hands over, then the per-source finalisation.  Sources that overflow the slot table (or whose root
+// list sums to 0) are returned in c->coop_overflow (count in *n_overflow) for the per-lane kernel.
 int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow) {
     hipStream_t s = c->stream;
     uint64_t nw = 0;
     RL_TRY(compact_active(c, &nw));
-    const int nwork = static_cast<int>(nw);
     RL_HIP(c, hipMemsetAsync(c->coop_cursor, 0, 9 * sizeof(unsigned long long), s));
+    RL_HIP(c, hipMemsetAsync(c->pipe, 0, 8 * sizeof(unsigned long long), s));
     CoopArgs a{};
     a.s = sa;
     a.work = c->coop_work;
     for (int x = 0; x < kXcds; x++) {
-        a.seg_begin[x] = static_cast<uint64_t>(nwork) * x / kXcds;
-        a.seg_end[x] = static_cast<uint64_t>(nwork) * (x + 1) / kXcds;
+        a.seg_begin[x] = nw * x / kXcds;
+        a.seg_end[x] = nw * (x + 1) / kXcds;
     }
     a.cursor = c->coop_cursor;
     a.t_my = c->coop_node;
@@ -747,16 +724,27 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow)
     a.cap = c->coop_cap;
     a.overflow = c->coop_overflow;
     a.n_overflow = reinterpret_cast<unsigned int*>(c->coop_cursor + kXcds);
-    if (nwork > 0) {
-        if (c->cfg.precision == RL_PREC_FP32_MIXED) {
-            if (c->cfg.mu == 0.0) {
-                k_search_coop<true, true><<<c->coop_blocks, kWave, 0, s>>>(a);
-            } else {
-                k_search_coop<true, false><<<c->coop_blocks, kWave, 0, s>>>(a);
-            }
+    a.slow = c->slow_items;
+    a.n_slow = c->pipe;
+    const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
+    const bool mu0 = c->cfg.mu == 0.0;
+    if (nw > 0) {
+        if (mixed && mu0) {
+            k_level1<true, true><<<c->coop_blocks, kWave, 0, s>>>(a);
+        } else if (mixed) {
+            k_level1<true, false><<<c->coop_blocks, kWave, 0, s>>>(a);
         } else {
-            k_search_coop<false, false><<<c->coop_blocks, kWave, 0, s>>>(a);
+            k_level1<false, false><<<c->coop_blocks, kWave, 0, s>>>(a);
         }
+        RL_HIP(c, hipGetLastError());
+        unsigned long long nslow = 0;
+        RL_HIP(c, hipMemcpyAsync(&nslow, c->pipe, sizeof(nslow), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipStreamSynchronize(s));
+        c->last_slow_descents = nslow;
+        if (nslow > 0) {
+            RL_TRY(launch_descend_lane(c, sa, c->slow_items, nslow));
+        }
+        k_finalize<<<static_cast<unsigned>((nw + 255) / 256), 256, 0, s>>>(sa, c->coop_work, nw);
         RL_HIP(c, hipGetLastError());
     }
     unsigned int nov = 0;
