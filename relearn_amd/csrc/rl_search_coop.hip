@@ -37,7 +37,9 @@ namespace rl {
 namespace {
 
 constexpr int kWave = 64;
-constexpr uint32_t kLevelCap = 1024;   // expanded nodes per level held in LDS
+constexpr uint32_t kQueueCap = 1024;   // LDS ring of queued / not yet flushed slots (power of two)
+constexpr uint32_t kStageCap = 256;    // LDS ring of slot records not yet flushed (power of two)
+constexpr uint32_t kFlushAt = 192;     // flush when this many processed slots are staged
 constexpr int kXcds = 8;
 constexpr uint32_t kSlotNone = 0x0FFFFFFFU;
 
@@ -57,9 +59,14 @@ struct CoopArgs {
     unsigned long long* n_slow;
 };
 
+// Slot records are built in LDS during the traversal and copied to the wave's global slot table in
+// batches: a global store inside the step loop would make every step wait for the previous step's
+// stores (loads and stores share one wait counter).  The last entry of each ring takes the writes
+// of lanes with nothing to write.
 struct CoopShared {
-    uint32_t lvl[2][kLevelCap + 1];    // packed child range (meta.y) of the slots of a level; the
-                                       // last entry takes the writes of lanes with nothing to write
+    uint32_t queue[kQueueCap + 1];     // packed child range (meta.y) of slot i at i % kQueueCap
+    uint32_t stage_fx[kStageCap + 1];  // first expanded child slot | count << 28 of slot i at i % kStageCap
+    double stage_ms[kStageCap + 1];    // probability mass of slot i at i % kStageCap
     uint32_t lvl_start[kMaxLevels + 4];
 };
 
@@ -152,9 +159,10 @@ __device__ __forceinline__ NodeData load_node(const Lane& L, uint32_t node, bool
 template <bool kMixed, bool kMu0>
 __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, const NodeData& d, uint32_t node, bool act,
                                          double* p_out) {
+    // The source's own leaf lies at distance 0 and is discarded by the test, so the kernel's
+    // self-exclusion (Kernel.h:61-75) never applies here.
     const bool valid = act && d.m.x != 0;
     const bool leaf = (d.m.y >> kLinkBits) == 0;
-    const bool self = leaf && (d.m.y & kLinkMask) == L.src;
     if (kMixed) {
         const int32_t ix = static_cast<int32_t>(d.r32.x) - L.qx;
         const int32_t iy = static_cast<int32_t>(d.r32.y) - L.qy;
@@ -178,13 +186,13 @@ __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, con
         const float w = __builtin_amdgcn_exp2f(x) * static_cast<float>(d.m.x);
         double p = static_cast<double>(w);
         st = valid ? st : kDiscard;
-        p = (st == kAccept && !self) ? p : 0.0;
+        p = st == kAccept ? p : 0.0;
         const bool need = valid && (coinc || near);
-        if (__ballot(need) != 0) {
+        if (__builtin_amdgcn_ballot_w64(need) != 0) {
             if (need) {
                 const Eval e = eval_exact(a, L, node, 0, leaf);
                 st = e.status;
-                p = (st == kAccept && !self) ? probability<false>(a, e, d.m.x, kGauss) : 0.0;
+                p = st == kAccept ? probability<false>(a, e, d.m.x, kGauss) : 0.0;
             }
         }
         *p_out = p;
@@ -195,7 +203,7 @@ __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, con
         if (valid) {
             const Eval e = eval_exact_rec(a, L, d.r64, leaf);
             st = e.status;
-            if (st == kAccept && !self) {
+            if (st == kAccept) {
                 p = probability<false>(a, e, d.m.x, kGauss);
             }
         }
@@ -212,23 +220,29 @@ struct Bfs {
 };
 
 
-// One BFS step's input: the group's node from the level list and this lane's child (loaded).
+// One BFS step's input: up to 8 queue entries [kb, end) (one per group) and this lane's child of
+// its group's node (loaded).
 struct StepIn {
-    uint32_t k;      // level-list index of the group's node
+    uint32_t kb;     // first queue entry (= slot) of the step
+    uint32_t end;    // one past the last entry the step covers
+    uint32_t k;      // this group's entry
     uint32_t child;
-    bool gact;       // the group has a node
+    bool gact;       // the group has an entry
     bool act;        // the lane has a child
     NodeData nd;
 };
 
+// Reads the entries [kb, min(kb + 8, tail)) of the queue and issues the loads of their children.
 template <bool kMixed, bool kMu0>
-__device__ __forceinline__ StepIn fetch_step(const Lane& L, const CoopShared& sh, uint32_t cur, uint32_t lb, uint32_t le,
-                                             uint32_t k) {
-    const uint32_t j = lane() & 7U;
+__device__ __forceinline__ StepIn fetch_step(const Lane& L, const CoopShared& sh, uint32_t kb, uint32_t tail) {
+    const uint32_t ln = lane();
+    const uint32_t q = ln >> 3U, j = ln & 7U;
     StepIn x;
-    x.k = k;
-    x.gact = k < le;
-    const uint32_t packed = sh.lvl[cur][x.gact ? k - lb : kLevelCap];
+    x.kb = kb;
+    x.end = min(kb + 8U, tail);
+    x.k = kb + q;
+    x.gact = x.k < x.end;
+    const uint32_t packed = sh.queue[x.gact ? (x.k & (kQueueCap - 1U)) : kQueueCap];
     x.child = (packed & kLinkMask) + j;
     x.act = x.gact && j < (packed >> kLinkBits);
     x.nd = load_node<kMixed, kMu0>(L, x.child, x.act);
@@ -242,12 +256,29 @@ struct SlotTable {
     uint32_t dump;   // last slot: takes the stores of lanes with nothing to store
 };
 
-// Tests the 64 children of one step, sums the accepted probabilities per group, appends the
-// expanded children to the next level and records the group nodes' masses.  Returns false when the
-// slot table or the level list would overflow.
+struct BfsState {
+    uint32_t nslots;   // queue tail = slots allocated
+    uint32_t lvl_end;  // one past the last slot of the level being processed
+    uint32_t lev;      // that level
+    uint32_t flushed;  // slots below this are in the global table
+};
+
+// Copies the records of slots [from, to) from the LDS rings to the global slot table.
+__device__ __forceinline__ void flush_slots(const SlotTable& t, const CoopShared& sh, uint32_t from, uint32_t to) {
+    for (uint32_t sl = from + lane(); sl < to; sl += kWave) {
+        at(t.my, sl) = sh.queue[sl & (kQueueCap - 1U)];
+        at(t.fx, sl) = sh.stage_fx[sl & (kStageCap - 1U)];
+        at(t.ms, sl) = sh.stage_ms[sl & (kStageCap - 1U)];
+    }
+}
+
+// Tests the children of one step's entries, sums the accepted probabilities per group, appends the
+// expanded children to the queue and records the entries' masses.  Entries of two consecutive levels
+// can share a step; the level boundary is tracked for the upward mass pass.  Returns false when the
+// slot table, the queue or the level table would overflow.
 template <bool kMixed, bool kMu0>
 __device__ __forceinline__ bool bfs_step(const CoopArgs& a, const Lane& L, CoopShared& sh, const SlotTable& t,
-                                         const StepIn& x, uint32_t cur, uint32_t nb, uint32_t& nslots, Bfs& r) {
+                                         const StepIn& x, BfsState& b, Bfs& r) {
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
     double p;
@@ -258,28 +289,51 @@ __device__ __forceinline__ bool bfs_step(const CoopArgs& a, const Lane& L, CoopS
     const bool ex = st == kExpand;
     const uint64_t em = __builtin_amdgcn_ballot_w64(ex);
     const uint32_t ne = static_cast<uint32_t>(__popcll(em));
-    if (nslots + ne > t.dump || nslots + ne - nb > kLevelCap) {
+    if (b.nslots + ne > t.dump || b.nslots + ne - b.flushed > kQueueCap) {
         return false;
     }
-    const uint32_t slot = nslots + rank_below(em, ln);
-    at(t.my, ex ? slot : t.dump) = x.nd.m.y;
-    sh.lvl[cur ^ 1U][ex ? slot - nb : kLevelCap] = x.nd.m.y;
+    // lane 8q (the one that writes group q's record) has rank_below(em, ln) = the group's first slot
+    const uint32_t slot = b.nslots + rank_below(em, ln);
+    sh.queue[ex ? (slot & (kQueueCap - 1U)) : kQueueCap] = x.nd.m.y;
     const bool gw = x.gact && j == 0;
     const uint32_t gem = static_cast<uint32_t>((em >> (8U * q)) & 0xFFULL);
-    const uint32_t first = nslots + rank_below(em, 8U * q);
-    at(t.ms, gw ? x.k : t.dump) = gsum;
-    at(t.fx, gw ? x.k : t.dump) = (gem != 0 ? first : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
-    nslots += ne;
+    const uint32_t gi = gw ? (x.k & (kStageCap - 1U)) : kStageCap;
+    sh.stage_ms[gi] = gsum;
+    sh.stage_fx[gi] = (gem != 0 ? slot : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
+    // Levels that end inside this step: the level after each is the children of the groups before its
+    // end.  A step can close two levels when the second was queued entirely before the step (the
+    // last level of the traversal); an empty next level ends the traversal.
+    while (x.kb < b.lvl_end && b.lvl_end <= x.end) {
+        const uint32_t gb = b.lvl_end - x.kb;
+        const uint64_t before = gb >= 8U ? ~0ULL : ((1ULL << (8U * gb)) - 1ULL);
+        b.lev++;
+        if (b.lev + 1 >= kMaxLevels + 4) {
+            return false;
+        }
+        if (ln == 0) {
+            sh.lvl_start[b.lev] = b.lvl_end;
+        }
+        const uint32_t next_end = b.nslots + static_cast<uint32_t>(__popcll(em & before));
+        if (next_end == b.lvl_end) {
+            break;
+        }
+        b.lvl_end = next_end;
+    }
+    b.nslots += ne;
+    if (x.end - b.flushed >= kFlushAt) {
+        flush_slots(t, sh, b.flushed, x.end);
+        b.flushed = x.end;
+    }
     return true;
 }
 
 // Breadth-first traversal below `root` (inner, with vacant dendrites) filling the slot table at `base`.
-// The loads of the next step's 64 children are issued before the current step is evaluated (two
-// alternating step buffers, so no registers are copied).
+// The queue is consumed 8 entries per step across level boundaries; the next step's child records are
+// loaded before the current step is evaluated (two alternating step buffers, so no registers are
+// copied).
 template <bool kMixed, bool kMu0>
 __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopShared& sh, uint64_t base) {
     const uint32_t ln = lane();
-    const uint32_t q = ln >> 3U;
     Bfs r{ 0.0, 0, 0, false };
     SlotTable t;
     t.my = a.t_my + base;
@@ -287,56 +341,43 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
     t.ms = a.t_mass + base;
     t.dump = a.cap - 1;
     if (ln == 0) {
-        t.my[0] = root_meta_y;
-        sh.lvl[0][0] = root_meta_y;
+        sh.queue[0] = root_meta_y;
         sh.lvl_start[0] = 0;
     }
     __syncthreads();
-    uint32_t lb = 0, le = 1, nslots = 1, lev = 0, cur = 0;
-    while (lb < le) {
-        const uint32_t nb = nslots;
-        StepIn xa = fetch_step<kMixed, kMu0>(L, sh, cur, lb, le, lb + q);
-        // Wait for the first step's data here, so that the loop header carries no pending load and
-        // the stores of the previous step are never waited for.
-        if (kMixed) {
-            __asm__ volatile("" ::"v"(xa.nd.m.x), "v"(xa.nd.m.y), "v"(xa.nd.r32.x), "v"(xa.nd.r32.y), "v"(xa.nd.r32.z),
-                             "v"(xa.nd.r32.w));
-        } else {
-            __asm__ volatile("" ::"v"(xa.nd.m.x), "v"(xa.nd.m.y), "v"(xa.nd.r64.x), "v"(xa.nd.r64.y), "v"(xa.nd.r64.z),
-                             "v"(xa.nd.r64.w));
-        }
-        for (uint32_t kb = lb; kb < le;) {
-            const StepIn xb = fetch_step<kMixed, kMu0>(L, sh, cur, lb, le, kb + 8 + q);
-            __builtin_amdgcn_sched_barrier(0);   // keep the prefetch ahead of this step's work
-            if (!bfs_step<kMixed, kMu0>(a, L, sh, t, xa, cur, nb, nslots, r)) {
-                r.overflow = true;
-                return r;
-            }
-            kb += 8;
-            if (kb >= le) {
-                break;
-            }
-            xa = fetch_step<kMixed, kMu0>(L, sh, cur, lb, le, kb + 8 + q);
-            __builtin_amdgcn_sched_barrier(0);
-            if (!bfs_step<kMixed, kMu0>(a, L, sh, t, xb, cur, nb, nslots, r)) {
-                r.overflow = true;
-                return r;
-            }
-            kb += 8;
-        }
-        __syncthreads();
-        lb = nb;
-        le = nslots;
-        cur ^= 1U;
-        lev++;
-        if (lev + 1 >= kMaxLevels + 4) {
+    BfsState b{ 1U, 1U, 0U, 0U };
+    StepIn xa = fetch_step<kMixed, kMu0>(L, sh, 0U, 1U);
+    // Wait for the first step's data here, so that the loop header carries no pending load and the
+    // stores of the previous step are never waited for.
+    if (kMixed) {
+        __asm__ volatile("" ::"v"(xa.nd.m.x), "v"(xa.nd.m.y), "v"(xa.nd.r32.x), "v"(xa.nd.r32.y), "v"(xa.nd.r32.z),
+                         "v"(xa.nd.r32.w));
+    } else {
+        __asm__ volatile("" ::"v"(xa.nd.m.x), "v"(xa.nd.m.y), "v"(xa.nd.r64.x), "v"(xa.nd.r64.y), "v"(xa.nd.r64.z),
+                         "v"(xa.nd.r64.w));
+    }
+    while (true) {
+        StepIn xb = fetch_step<kMixed, kMu0>(L, sh, xa.end, b.nslots);
+        __builtin_amdgcn_sched_barrier(0);   // keep the prefetch ahead of this step's work
+        if (!bfs_step<kMixed, kMu0>(a, L, sh, t, xa, b, r)) {
             r.overflow = true;
             return r;
         }
-        if (ln == 0) {
-            sh.lvl_start[lev] = lb;
+        if (xb.kb >= b.nslots) {
+            break;
+        }
+        xa = fetch_step<kMixed, kMu0>(L, sh, xb.end, b.nslots);
+        __builtin_amdgcn_sched_barrier(0);
+        if (!bfs_step<kMixed, kMu0>(a, L, sh, t, xb, b, r)) {
+            r.overflow = true;
+            return r;
+        }
+        if (xa.kb >= b.nslots) {
+            break;
         }
     }
+    const uint32_t lev = b.lev;
+    flush_slots(t, sh, b.flushed, b.nslots);
     __syncthreads();
     // pull the masses of expanded children into their parents, deepest parents first
     for (int lv = static_cast<int>(lev) - 2; lv >= 0; lv--) {
@@ -346,8 +387,8 @@ __device__ Bfs bfs(const CoopArgs& a, const Lane& L, uint32_t root_meta_y, CoopS
             if (sl < se) {
                 const uint32_t fx = at(t.fx, sl);
                 const uint32_t ne = fx >> 28U;
-                if (ne != 0) {
-                    const uint32_t f = fx & kSlotNone;
+                const uint32_t f = fx & kSlotNone;
+                if (ne != 0 && f + ne <= t.dump) {
                     double m8[8];
 #pragma unroll
                     for (uint32_t c = 0; c < 8; c++) {
