@@ -238,8 +238,9 @@ __device__ __forceinline__ StepIn fetch_step(const Lane& L, const CoopShared& sh
     const uint32_t ln = lane();
     const uint32_t q = ln >> 3U, j = ln & 7U;
     StepIn x;
-    x.kb = kb;
-    x.end = min(kb + 8U, tail);
+    // kb and end are wave-uniform; keeping them in scalar registers keeps the step's control flow scalar
+    x.kb = __builtin_amdgcn_readfirstlane(kb);
+    x.end = __builtin_amdgcn_readfirstlane(min(kb + 8U, tail));
     x.k = kb + q;
     x.gact = x.k < x.end;
     const uint32_t packed = sh.queue[x.gact ? (x.k & (kQueueCap - 1U)) : kQueueCap];
@@ -302,27 +303,38 @@ __device__ __forceinline__ bool bfs_step(const CoopArgs& a, const Lane& L, CoopS
     sh.stage_fx[gi] = (gem != 0 ? slot : kSlotNone) | (static_cast<uint32_t>(__popc(gem)) << 28U);
     // Levels that end inside this step: the level after each is the children of the groups before its
     // end.  A step can close two levels when the second was queued entirely before the step (the
-    // last level of the traversal); an empty next level ends the traversal.
-    while (x.kb < b.lvl_end && b.lvl_end <= x.end) {
-        const uint32_t gb = b.lvl_end - x.kb;
+    // last level of the traversal); an empty next level ends the traversal.  All of this is
+    // wave-uniform scalar work.
+    const uint32_t kb = __builtin_amdgcn_readfirstlane(x.kb);
+    const uint32_t end = __builtin_amdgcn_readfirstlane(x.end);
+    const uint32_t ns = __builtin_amdgcn_readfirstlane(b.nslots);
+    uint32_t le = __builtin_amdgcn_readfirstlane(b.lvl_end);
+    uint32_t lev = __builtin_amdgcn_readfirstlane(b.lev);
+    for (int pass = 0; pass < 2; pass++) {
+        if (!(kb < le && le <= end)) {
+            break;
+        }
+        const uint32_t gb = le - kb;
         const uint64_t before = gb >= 8U ? ~0ULL : ((1ULL << (8U * gb)) - 1ULL);
-        b.lev++;
-        if (b.lev + 1 >= kMaxLevels + 4) {
+        lev++;
+        if (lev + 1 >= kMaxLevels + 4) {
             return false;
         }
         if (ln == 0) {
-            sh.lvl_start[b.lev] = b.lvl_end;
+            sh.lvl_start[lev] = le;
         }
-        const uint32_t next_end = b.nslots + static_cast<uint32_t>(__popcll(em & before));
-        if (next_end == b.lvl_end) {
+        const uint32_t next_end = ns + static_cast<uint32_t>(__popcll(em & before));
+        if (next_end == le) {
             break;
         }
-        b.lvl_end = next_end;
+        le = next_end;
     }
-    b.nslots += ne;
+    b.lvl_end = le;
+    b.lev = lev;
+    b.nslots = ns + ne;
     if (x.end - b.flushed >= kFlushAt) {
         flush_slots(t, sh, b.flushed, x.end);
-        b.flushed = x.end;
+        b.flushed = __builtin_amdgcn_readfirstlane(x.end);
     }
     return true;
 }
@@ -606,7 +618,7 @@ __device__ __forceinline__ int descend_fast(const SearchArgs& s, const Lane& L, 
 // it 8 at a time, and continues each draw's descent in its group.  Results go to slot_target; descents
 // that need the general traversal are queued for the per-lane kernel.
 template <bool kMixed, bool kMu0>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(5, 8))) k_level1(CoopArgs a) {
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMixed ? 5 : 4, 8))) k_level1(CoopArgs a) {
     __shared__ CoopShared sh;
     const SearchArgs& s = a.s;
     const uint32_t ln = lane();
