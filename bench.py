@@ -69,6 +69,7 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     tests = sum(r[0]["node_tests"] for r in recs)
     tests_ref = sum(r[0]["node_tests_ref"] for r in recs)
     kernel_ms = [r[0]["ms_search"] for r in recs]
+    dominant_ms = [r[0]["ms_search_kernel"] for r in recs]
     # pick results of step 0 for the precision comparison
     g.prepare_update_connectivity_device(*d_vac)
     g.find_target_neurons(0)
@@ -84,8 +85,9 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
         "ms_per_step": 1e3 * wall / steps,
         "searches_per_step": searches / steps,
         "searches_per_s": searches / wall,
-        "search_kernel_ms": float(np.mean(kernel_ms)),
-        "search_kernel_searches_per_s": searches / (1e-3 * sum(kernel_ms)),
+        "search_ms": float(np.mean(kernel_ms)),
+        "dominant_kernel_ms": float(np.mean(dominant_ms)),
+        "search_phase_searches_per_s": searches / (1e-3 * sum(kernel_ms)),
         "node_tests_per_step": tests / steps,
         "node_tests_ref_per_step": tests_ref / steps,
         "phase_s": [float(np.mean([r[2][i] for r in recs])) for i in range(3)],
@@ -196,7 +198,7 @@ def main(argv=None):
         res[name] = run_precision(net, prec, args.steps, args.warmup, args.seed + 1, d_vac, flags=flags)
         r = res[name]
         log(f"{name}: {r['searches_per_s']:.4g} searches/s whole step, {r['ms_per_step']:.1f} ms/step, "
-            f"search kernel {r['search_kernel_ms']:.1f} ms, phases {['%.3f' % x for x in r['phase_s']]} s")
+            f"search {r['search_ms']:.1f} ms (kernel {r['dominant_kernel_ms']:.1f} ms), phases {['%.3f' % x for x in r['phase_s']]} s")
 
     main_name = "fp32_mixed" if "fp32_mixed" in res else ("fp64" if "fp64" in res else "fp64_plain")
     main_r = res[main_name]
@@ -204,7 +206,7 @@ def main(argv=None):
     # algorithmic (logical) bytes: every acceptance test of the reference's traversals reads one node
     # record, every search reads its source and writes its result
     bytes_per_launch = main_r["node_tests_ref_per_step"] * b_node + main_r["searches_per_step"] * B_IO
-    achieved = bytes_per_launch / (1e-3 * main_r["search_kernel_ms"]) / 1e9
+    achieved = bytes_per_launch / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9
     kname = "k_level1" if args.kernel == "coop" and main_name != "fp64_plain" else "k_search"
     traffic = read_traffic(kname, main_name, args.neurons)
     out = {
@@ -240,7 +242,9 @@ def main(argv=None):
             "bytes_model": f"reference acceptance tests x {b_node} B + searches x {B_IO} B (logical bytes; "
                            "node records are re-read from L2/MALL, so the frac can exceed 1)",
             "achieved_executed_tests": (main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO)
-            / (1e-3 * main_r["search_kernel_ms"]) / 1e9,
+            / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9,
+            "kernel_ms": main_r["dominant_kernel_ms"],
+            "traffic_gbs": None if traffic is None else traffic / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9,
         },
     }
     for name, r in res.items():
@@ -251,7 +255,7 @@ def main(argv=None):
         div = float(np.mean(a["targets"][valid] != b["targets"][valid]))
         out["fp32_vs_fp64"] = {
             "speedup_whole_step": a["ms_per_step"] / b["ms_per_step"],
-            "speedup_search_kernel": a["search_kernel_ms"] / b["search_kernel_ms"],
+            "speedup_search": a["search_ms"] / b["search_ms"],
             "pick_divergence": div,
             "requests_fp64": int(a["counts"].sum()),
             "requests_fp32": int(b["counts"].sum()),
@@ -264,7 +268,7 @@ def main(argv=None):
         valid = a["targets"] != 0xFFFFFFFF
         out["fp32_vs_fp64_plain"] = {
             "speedup_whole_step": a["ms_per_step"] / b["ms_per_step"],
-            "speedup_search": a["search_kernel_ms"] / b["search_kernel_ms"],
+            "speedup_search": a["search_ms"] / b["search_ms"],
             "pick_divergence": float(np.mean(a["targets"][valid] != b["targets"][valid])),
             "synapses_rel_diff": abs(b["created0"] - a["created0"]) / max(1, a["created0"]),
         }

@@ -70,6 +70,7 @@ typedef struct {
     double ms_prepare;          /* leaf fill + upward pass */
     double ms_search;           /* target search */
     double ms_commit;           /* conflict resolution + vacancy commit */
+    double ms_search_kernel;    /* the dominant search kernel alone (cooperative or per-lane) */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a
@@ -157,8 +158,8 @@ int rl_bh_export_tree(rl_bh_ctx* ctx, uint8_t* level, uint8_t* nchild, uint32_t*
 /* Milliseconds of the last search kernel(s), measured with HIP events on the context's stream. */
 int rl_bh_last_search_ms(rl_bh_ctx* ctx, double* ms);
 
-/* Counters of the last search: searches, requests, node_tests, node_tests_ref and ms_search are
- * filled (created and the other times are 0). */
+/* Counters of the last search: searches, requests, node_tests, node_tests_ref, ms_search and
+ * ms_search_kernel are filled (created and the other times are 0). */
 int rl_bh_last_search_stats(rl_bh_ctx* ctx, rl_bh_step_stats* stats);
 
 /* ------------------------------------------------------------------ neuron step */

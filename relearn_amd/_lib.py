@@ -59,6 +59,7 @@ class StepStats(C.Structure):
         ("ms_prepare", C.c_double),
         ("ms_search", C.c_double),
         ("ms_commit", C.c_double),
+        ("ms_search_kernel", C.c_double),
     ]
 
     def as_dict(self):

@@ -151,7 +151,8 @@ int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
     auto* c = new rl_bh_ctx();
     c->cfg = *cfg;
     if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
         g_create_error = "rl_bh_create: stream/event creation failed";
         delete c;
         return RL_EHIP;
@@ -180,6 +181,12 @@ int rl_bh_destroy(rl_bh_ctx* c) {
     free_all(c);
     if (c->ev0 != nullptr) {
         (void)hipEventDestroy(c->ev0);
+    }
+    if (c->ev2 != nullptr) {
+        (void)hipEventDestroy(c->ev2);
+    }
+    if (c->ev3 != nullptr) {
+        (void)hipEventDestroy(c->ev3);
     }
     if (c->ev1 != nullptr) {
         (void)hipEventDestroy(c->ev1);
@@ -473,6 +480,7 @@ int rl_bh_update_connectivity(rl_bh_ctx* c, uint32_t step, rl_bh_step_stats* sta
             stats->ms_prepare = t01;
             stats->ms_search = t12;
             stats->ms_commit = t23;
+            stats->ms_search_kernel = c->last_kernel_ms;
         }
     } while (false);
     for (auto& ev : e) {
@@ -537,6 +545,7 @@ int rl_bh_last_search_stats(rl_bh_ctx* c, rl_bh_step_stats* st) {
     st->node_tests = c->last_tests;
     st->node_tests_ref = c->last_tests_ref;
     st->ms_search = c->last_search_ms;
+    st->ms_search_kernel = c->last_kernel_ms;
     return RL_OK;
 }
 

@@ -62,8 +62,9 @@ struct rl_bh_ctx {
     rl_bh_config cfg{};
     rl::Status st;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     double last_search_ms = 0.0;
+    double last_kernel_ms = 0.0;    // the dominant search kernel alone
 
     uint64_t n = 0;
     double* pos = nullptr;          // SoA x[n], y[n], z[n]

@@ -567,17 +567,25 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
         RL_TRY(compact_active(c, &nlane));
         a.order = c->coop_work;
         a.nwork = nlane;
+        c->last_kernel_ms = 0.0;
         if (nlane > 0) {
+            RL_HIP(c, hipEventRecord(c->ev2, s));
             if (mixed) {
                 k_search<true, false><<<grid_for(nlane), kBlock, lds, s>>>(a);
             } else {
                 k_search<false, false><<<grid_for(nlane), kBlock, lds, s>>>(a);
             }
             RL_HIP(c, hipGetLastError());
+            RL_HIP(c, hipEventRecord(c->ev3, s));
         }
         unsigned int nredo = 0;
         RL_HIP(c, hipMemcpyAsync(&nredo, a.n_redo, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
         RL_HIP(c, hipStreamSynchronize(s));
+        if (nlane > 0) {
+            float kms = 0.0f;
+            RL_HIP(c, hipEventElapsedTime(&kms, c->ev2, c->ev3));
+            c->last_kernel_ms = kms;
+        }
         nlane = nredo;
         a.order = c->redo;
     } else {

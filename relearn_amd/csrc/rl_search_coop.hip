@@ -781,7 +781,9 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow)
     a.n_slow = c->pipe;
     const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
     const bool mu0 = c->cfg.mu == 0.0;
+    c->last_kernel_ms = 0.0;
     if (nw > 0) {
+        RL_HIP(c, hipEventRecord(c->ev2, s));
         if (mixed && mu0) {
             k_level1<true, true><<<c->coop_blocks, kWave, 0, s>>>(a);
         } else if (mixed) {
@@ -790,9 +792,13 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow)
             k_level1<false, false><<<c->coop_blocks, kWave, 0, s>>>(a);
         }
         RL_HIP(c, hipGetLastError());
+        RL_HIP(c, hipEventRecord(c->ev3, s));
         unsigned long long nslow = 0;
         RL_HIP(c, hipMemcpyAsync(&nslow, c->pipe, sizeof(nslow), hipMemcpyDeviceToHost, s));
         RL_HIP(c, hipStreamSynchronize(s));
+        float kms = 0.0f;
+        RL_HIP(c, hipEventElapsedTime(&kms, c->ev2, c->ev3));
+        c->last_kernel_ms = kms;
         c->last_slow_descents = nslow;
         if (nslow > 0) {
             RL_TRY(launch_descend_lane(c, sa, c->slow_items, nslow));
