@@ -89,6 +89,20 @@ const char* rl_bh_last_error(const rl_bh_ctx* ctx);
  * (SynapticElements::get_signal_types), RL_SIGNAL_*. */
 int rl_bh_init(rl_bh_ctx* ctx, uint64_t n, const double* xyz, const uint8_t* signal);
 
+/* Ensemble of independent networks in one context, advanced together by every call below (batched
+ * plasticity updates; no counterpart in the reference, which runs one network per process).
+ * Network g owns the neurons [offsets[g], offsets[g+1]) (offsets[0] = 0, every network non-empty)
+ * and the simulation box boxes[6g .. 6g+5] (min xyz, max xyz).  Each network gets its own octree
+ * (one forest, built in one pass), its own fixed-point scale, and its own draw stream: seed
+ * rl_bh_network_seed(cfg.seed, g) addressed by neuron ids local to the network.  Network g therefore
+ * reproduces a one-network context created with that seed on its neurons.  rl_bh_init is the case
+ * num_networks = 1 with the box of the configuration. */
+int rl_bh_init_networks(rl_bh_ctx* ctx, uint32_t num_networks, const uint64_t* offsets, const double* boxes,
+                        const double* xyz, const uint8_t* signal);
+
+/* Draw-stream seed of network g of an ensemble (network 0 keeps the configured seed). */
+uint64_t rl_bh_network_seed(uint64_t seed, uint32_t network);
+
 /* Algorithm::disable_neurons / NeuronsExtraInfo update flags: enabled[i] = 0 disables neuron i
  * (its leaf gets zero vacancies and it searches no targets).  NULL enables all. */
 int rl_bh_set_enabled(rl_bh_ctx* ctx, const uint8_t* enabled);
@@ -193,6 +207,11 @@ int rl_neurons_step(rl_bh_ctx* ctx, uint32_t step, const double* input);
  * deltas to grown/vacant/connected counts at a plasticity step.  deleted (3*n, may be NULL) gets the
  * number of bound elements each neuron must delete per type. */
 int rl_neurons_commit_elements(rl_bh_ctx* ctx, uint32_t* deleted);
+
+/* Growth rates per network of an ensemble (nu[3g + t] for element type t of network g: axons,
+ * excitatory, inhibitory dendrites), replacing rl_step_params.nu for those networks; NULL restores
+ * rl_step_params.nu everywhere (SynapticElements growth rate, SynapticElementsBase.h:617-625). */
+int rl_neurons_set_network_growth(rl_bh_ctx* ctx, const double* nu);
 
 /* Copies the neuron state: any pointer may be NULL; delta/grown are 3*n (type-major). */
 int rl_neurons_get(rl_bh_ctx* ctx, double* x, double* u, double* ca, uint8_t* fired, double* delta,

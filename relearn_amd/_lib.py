@@ -24,13 +24,14 @@ _u64p = C.POINTER(C.c_uint64)
 _f64p = C.POINTER(C.c_double)
 
 EXPORTED = [
-    "rl_bh_create", "rl_bh_destroy", "rl_bh_last_error", "rl_bh_init", "rl_bh_set_enabled",
+    "rl_bh_create", "rl_bh_destroy", "rl_bh_last_error", "rl_bh_init", "rl_bh_init_networks", "rl_bh_network_seed",
+    "rl_bh_set_enabled",
     "rl_bh_prepare_update_connectivity", "rl_bh_prepare_resident", "rl_bh_prepare_update_connectivity_device",
     "rl_bh_find_target_neurons",
     "rl_bh_get_requests", "rl_bh_get_search_slots", "rl_bh_commit", "rl_bh_get_commit_result",
     "rl_bh_update_connectivity", "rl_bh_get_vacancies", "rl_bh_tree_info", "rl_bh_export_tree",
     "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
-    "rl_neurons_get",
+    "rl_neurons_get", "rl_neurons_set_network_growth",
 ]
 
 
@@ -86,6 +87,21 @@ class StepParams(C.Structure):
         ("nu", C.c_double * 3),
     ]
 
+    @classmethod
+    def izhikevich(cls, input_mode=0, input_mean=5.0, input_stddev=2.0, nu=1e-4):
+        """The reference's defaults: Izhikevich a=0.02, b=0.2, c=-65, d=8, V_spike=30, k1=0.04, k2=5,
+        k3=140, h=10 substeps; calcium tau 1e4, beta 1e-3; eta 0.4/0.1/0.0; growth rate nu."""
+        p = cls()
+        p.a, p.b, p.c, p.d = 0.02, 0.2, -65.0, 8.0
+        p.v_spike, p.k1, p.k2, p.k3 = 30.0, 0.04, 5.0, 140.0
+        p.h = 10
+        p.input_mode = input_mode
+        p.input_mean, p.input_stddev = input_mean, input_stddev
+        p.tau_ca, p.beta_ca = 10000.0, 0.001
+        p.eta[0], p.eta[1], p.eta[2] = 0.4, 0.1, 0.0
+        p.nu[0] = p.nu[1] = p.nu[2] = nu
+        return p
+
 
 def _ptr(a, t):
     if a is None:
@@ -110,6 +126,9 @@ def load():
     L.rl_bh_last_error.argtypes = [vp]
     L.rl_bh_last_error.restype = C.c_char_p
     L.rl_bh_init.argtypes = [vp, C.c_uint64, _f64p, _u8p]
+    L.rl_bh_init_networks.argtypes = [vp, C.c_uint32, _u64p, _f64p, _f64p, _u8p]
+    L.rl_bh_network_seed.argtypes = [C.c_uint64, C.c_uint32]
+    L.rl_neurons_set_network_growth.argtypes = [vp, _f64p]
     L.rl_bh_set_enabled.argtypes = [vp, _u8p]
     L.rl_bh_prepare_update_connectivity.argtypes = [vp, _u32p, _u32p, _u32p]
     L.rl_bh_prepare_resident.argtypes = [vp]
@@ -132,8 +151,9 @@ def load():
     L.rl_neurons_get.argtypes = [vp, _f64p, _f64p, _f64p, _u8p, _f64p, _f64p, _u32p]
     for name in EXPORTED:
         f = getattr(L, name)
-        if name != "rl_bh_last_error":
+        if name not in ("rl_bh_last_error", "rl_bh_network_seed"):
             f.restype = C.c_int
+    L.rl_bh_network_seed.restype = C.c_uint64
     _lib = L
     return L
 
@@ -200,6 +220,17 @@ class BarnesHutHIP:
         sig = np.ascontiguousarray(signal, dtype=np.uint8)
         self.n = xyz.shape[0]
         self._check(self.L.rl_bh_init(self.h, self.n, _ptr(xyz, _f64p), _ptr(sig, _u8p)))
+
+    def init_networks(self, offsets, boxes, xyz, signal):
+        """Ensemble of independent networks: network g owns neurons offsets[g]:offsets[g+1] and the
+        box boxes[g] = (min x, min y, min z, max x, max y, max z)."""
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        bx = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        sig = np.ascontiguousarray(signal, dtype=np.uint8)
+        self.n = xyz.shape[0]
+        self._check(self.L.rl_bh_init_networks(self.h, len(off) - 1, _ptr(off, _u64p), _ptr(bx, _f64p),
+                                               _ptr(xyz, _f64p), _ptr(sig, _u8p)))
 
     def set_enabled(self, enabled):
         en = None if enabled is None else np.ascontiguousarray(enabled, dtype=np.uint8)
@@ -307,6 +338,10 @@ class BarnesHutHIP:
         con = None if connected is None else np.ascontiguousarray(connected, dtype=np.uint32)
         self._check(self.L.rl_neurons_init(self.h, C.byref(params), *[_ptr(a, _f64p) for a in arrs], _ptr(con, _u32p)))
 
+    def neurons_set_network_growth(self, nu):
+        a = None if nu is None else np.ascontiguousarray(nu, dtype=np.float64).ravel()
+        self._check(self.L.rl_neurons_set_network_growth(self.h, _ptr(a, _f64p)))
+
     def neurons_step(self, step, inp=None):
         i = None if inp is None else np.ascontiguousarray(inp, dtype=np.float64)
         self._check(self.L.rl_neurons_step(self.h, step, _ptr(i, _f64p)))
@@ -328,3 +363,8 @@ class BarnesHutHIP:
         out["grown"] = out["grown"].reshape(3, n)
         out["connected"] = out["connected"].reshape(3, n)
         return out
+
+
+def network_seed(seed, network):
+    """Draw-stream seed of network `network` of an ensemble."""
+    return int(load().rl_bh_network_seed(seed, network))

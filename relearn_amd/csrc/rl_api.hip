@@ -1,6 +1,7 @@
 // C ABI of librelearn_bh.so: context management and the Algorithm-shaped entry points.
-#include <cmath>
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <vector>
@@ -90,6 +91,17 @@ void free_all(rl_bh_ctx* c) {
     dfree(&c->pipe);
     dfree(&c->coop_fexp);
     dfree(&c->coop_mass);
+    dfree(&c->net_off);
+    dfree(&c->net_box);
+    dfree(&c->net_scale);
+    dfree(&c->net_seed);
+    dfree(&c->net_k2q);
+    dfree(&c->net_invq);
+    dfree(&c->net_of);
+    c->num_networks = 0;
+    c->h_net_off.clear();
+    c->h_net_box.clear();
+    c->h_net_scale.clear();
     c->coop_cap = 0;
     c->coop_blocks = 0;
     c->commit_cap = 0;
@@ -157,15 +169,6 @@ int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
         delete c;
         return RL_EHIP;
     }
-    // one fixed-point scale for all axes (the largest extent spans 2^31 - 1 units), so squared
-    // distances and squared cell sides compare in the same units
-    double ext_max = 0.0;
-    for (int d = 0; d < 3; d++) {
-        ext_max = std::max(ext_max, cfg->box_max[d] - cfg->box_min[d]);
-    }
-    for (int d = 0; d < 3; d++) {
-        c->qscale[d] = ext_max > 0.0 ? 2147483647.0 / ext_max : 0.0;
-    }
     *out = c;
     return RL_OK;
 }
@@ -205,32 +208,82 @@ const char* rl_bh_last_error(const rl_bh_ctx* c) {
     return c->st.msg.c_str();
 }
 
-int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signal) {
-    if (c == nullptr) {
-        return RL_EINVAL;
-    }
+uint64_t rl_bh_network_seed(uint64_t seed, uint32_t network) {
+    // network 0 keeps the context seed, so a one-network ensemble equals rl_bh_init
+    return network == 0 ? seed : seed ^ (static_cast<uint64_t>(network) * 0x9E3779B97F4A7C15ULL);
+}
+
+namespace {
+
+// Shared body of rl_bh_init and rl_bh_init_networks: validates, uploads and builds the forest.
+int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const double* boxes, const double* xyz,
+                  const uint8_t* signal) {
+    const uint64_t n = offsets[G];
     if (n > 0 && (xyz == nullptr || signal == nullptr)) {
         return fail(c, RL_EINVAL, "rl_bh_init: null positions or signal types");
     }
     if (n > kLinkMask) {
         return fail(c, RL_ELIMIT, "rl_bh_init: more than 2^28 neurons");
     }
+    if (offsets[0] != 0) {
+        return fail(c, RL_EINVAL, "rl_bh_init_networks: offsets[0] must be 0");
+    }
+    std::vector<uint32_t> net_of(n);
+    std::vector<double> scale(G);
+    for (uint32_t g = 0; g < G; g++) {
+        if (n > 0 && offsets[g + 1] <= offsets[g]) {
+            return fail(c, RL_EINVAL, "rl_bh_init_networks: every network needs at least one neuron");
+        }
+        const double* b = boxes + 6 * static_cast<uint64_t>(g);
+        double ext_max = 0.0;
+        for (int d = 0; d < 3; d++) {
+            if (!std::isfinite(b[d]) || !std::isfinite(b[3 + d]) || !(b[d] <= b[3 + d]) ||
+                std::fabs(b[d]) > DBL_MAX / 4 || std::fabs(b[3 + d]) > DBL_MAX / 4) {
+                return fail(c, RL_EINVAL, "rl_bh_init_networks: network box must be finite with min <= max");
+            }
+            ext_max = std::max(ext_max, b[3 + d] - b[d]);
+        }
+        // one fixed-point scale for all axes of the network (the largest extent spans 2^31 - 1
+        // units), so squared distances and squared cell sides compare in the same units
+        scale[g] = ext_max > 0.0 ? 2147483647.0 / ext_max : 0.0;
+        for (uint64_t i = offsets[g]; i < offsets[g + 1]; i++) {
+            net_of[i] = g;
+            for (int d = 0; d < 3; d++) {
+                const double v = xyz[3 * i + d];
+                if (!(b[d] <= v && v <= b[3 + d])) {
+                    return fail(c, RL_EOUTSIDE, "rl_bh_init: position outside the simulation box");
+                }
+            }
+            if (signal[i] > 1) {
+                return fail(c, RL_EINVAL, "rl_bh_init: signal type must be 0 or 1");
+            }
+        }
+    }
     RL_TRY(set_device(c));
     free_all(c);
     c->n = n;
     c->prepared = false;
     c->searched_valid = false;
-    for (uint64_t i = 0; i < n; i++) {
-        for (int d = 0; d < 3; d++) {
-            const double v = xyz[3 * i + d];
-            if (!(c->cfg.box_min[d] <= v && v <= c->cfg.box_max[d])) {
-                return fail(c, RL_EOUTSIDE, "rl_bh_init: position outside the simulation box");
-            }
-        }
-        if (signal[i] > 1) {
-            return fail(c, RL_EINVAL, "rl_bh_init: signal type must be 0 or 1");
-        }
+    c->num_networks = G;
+    c->h_net_off.assign(offsets, offsets + G + 1);
+    c->h_net_box.assign(boxes, boxes + 6 * static_cast<uint64_t>(G));
+    c->h_net_scale = scale;
+    std::vector<uint64_t> seeds(G);
+    std::vector<float> k2q(G), invq(G);
+    const double k2 = -1.4426950408889634 / (c->cfg.sigma * c->cfg.sigma);
+    for (uint32_t g = 0; g < G; g++) {
+        seeds[g] = rl_bh_network_seed(c->cfg.seed, g);
+        const double iq = scale[g] > 0.0 ? 1.0 / scale[g] : 0.0;
+        k2q[g] = static_cast<float>(k2 * iq * iq);
+        invq[g] = static_cast<float>(iq);
     }
+    RL_TRY(dalloc(c, &c->net_off, G + 1));
+    RL_TRY(dalloc(c, &c->net_box, 6 * static_cast<size_t>(G)));
+    RL_TRY(dalloc(c, &c->net_scale, G));
+    RL_TRY(dalloc(c, &c->net_seed, G));
+    RL_TRY(dalloc(c, &c->net_k2q, G));
+    RL_TRY(dalloc(c, &c->net_invq, G));
+    RL_TRY(dalloc(c, &c->net_of, n));
     RL_TRY(dalloc(c, &c->signal, n));
     RL_TRY(dalloc(c, &c->enabled, n));
     for (int t = 0; t < 3; t++) {
@@ -262,7 +315,14 @@ int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signa
         RL_TRY(dalloc(c, &c->coop_mass, slots));
     }
     hipStream_t s = c->stream;
+    RL_HIP(c, hipMemcpyAsync(c->net_off, c->h_net_off.data(), (G + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    RL_HIP(c, hipMemcpyAsync(c->net_box, c->h_net_box.data(), 6 * G * sizeof(double), hipMemcpyHostToDevice, s));
+    RL_HIP(c, hipMemcpyAsync(c->net_scale, scale.data(), G * sizeof(double), hipMemcpyHostToDevice, s));
+    RL_HIP(c, hipMemcpyAsync(c->net_seed, seeds.data(), G * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    RL_HIP(c, hipMemcpyAsync(c->net_k2q, k2q.data(), G * sizeof(float), hipMemcpyHostToDevice, s));
+    RL_HIP(c, hipMemcpyAsync(c->net_invq, invq.data(), G * sizeof(float), hipMemcpyHostToDevice, s));
     if (n > 0) {
+        RL_HIP(c, hipMemcpyAsync(c->net_of, net_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
         RL_HIP(c, hipMemcpyAsync(c->signal, signal, n, hipMemcpyHostToDevice, s));
         RL_HIP(c, hipMemsetAsync(c->enabled, 1, n, s));
         for (int t = 0; t < 3; t++) {
@@ -271,11 +331,33 @@ int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signa
         }
     }
     RL_HIP(c, hipStreamSynchronize(s));
-    const int st = build_tree(c, xyz);
-    if (st != RL_OK) {
-        return st;
+    return build_tree(c, xyz);
+}
+
+}  // namespace
+
+int rl_bh_init(rl_bh_ctx* c, uint64_t n, const double* xyz, const uint8_t* signal) {
+    if (c == nullptr) {
+        return RL_EINVAL;
     }
-    return RL_OK;
+    const uint64_t off[2] = { 0, n };
+    const double box[6] = { c->cfg.box_min[0], c->cfg.box_min[1], c->cfg.box_min[2],
+                            c->cfg.box_max[0], c->cfg.box_max[1], c->cfg.box_max[2] };
+    return init_networks(c, 1, off, box, xyz, signal);
+}
+
+int rl_bh_init_networks(rl_bh_ctx* c, uint32_t num_networks, const uint64_t* offsets, const double* boxes,
+                        const double* xyz, const uint8_t* signal) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (num_networks == 0 || offsets == nullptr || boxes == nullptr) {
+        return fail(c, RL_EINVAL, "rl_bh_init_networks: need at least one network, offsets and boxes");
+    }
+    if (num_networks > kLinkMask) {
+        return fail(c, RL_ELIMIT, "rl_bh_init_networks: too many networks");
+    }
+    return init_networks(c, num_networks, offsets, boxes, xyz, signal);
 }
 
 int rl_bh_set_enabled(rl_bh_ctx* c, const uint8_t* enabled) {
@@ -556,6 +638,14 @@ int rl_neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, cons
     }
     RL_TRY(set_device(c));
     return neurons_init(c, p, x, u, ca, target_ca, grown, connected);
+}
+
+int rl_neurons_set_network_growth(rl_bh_ctx* c, const double* nu) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    return neurons_set_network_growth(c, nu);
 }
 
 int rl_neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {

@@ -23,11 +23,17 @@ inline unsigned grid_for(uint64_t n, int block = kBlock) {
     return static_cast<unsigned>(g == 0 ? 1 : g);
 }
 
+// Counter-based commit priority of request i, addressed like the draws: the network's seed and the
+// source's id inside its network.
 __global__ void k_priority(uint64_t m, const uint32_t* __restrict__ src, const uint32_t* __restrict__ rj,
-                           uint64_t seed, uint32_t step, uint64_t* __restrict__ prio, uint32_t* __restrict__ idx) {
+                           const uint32_t* __restrict__ net_of, const uint64_t* __restrict__ net_off,
+                           const uint64_t* __restrict__ net_seed, uint32_t step, uint64_t* __restrict__ prio,
+                           uint32_t* __restrict__ idx) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i < m) {
-        prio[i] = commit_priority(seed, src[i], step, rj[i]);
+        const uint32_t s = src[i];
+        const uint32_t g = net_of[s];
+        prio[i] = commit_priority(net_seed[g], s - static_cast<uint32_t>(net_off[g]), step, rj[i]);
         idx[i] = static_cast<uint32_t>(i);
     }
 }
@@ -149,7 +155,8 @@ int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* creat
             RL_STEP(hipMemsetAsync(prio, 0xFF, m * sizeof(uint64_t), s));
             k_rank_from_order<<<grid_for(m), kBlock, 0, s>>>(m, order, prio, idx, err);
         } else {
-            k_priority<<<grid_for(m), kBlock, 0, s>>>(m, c->req_source, c->req_j, c->cfg.seed, step, prio, idx);
+            k_priority<<<grid_for(m), kBlock, 0, s>>>(m, c->req_source, c->req_j, c->net_of, c->net_off, c->net_seed,
+                                                      step, prio, idx);
         }
         RL_STEP(hipGetLastError());
         size_t tb = 0, tb2 = 0;

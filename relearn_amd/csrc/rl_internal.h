@@ -39,6 +39,7 @@ struct Tree {
     double4* rec64[2] = { nullptr, nullptr };
     uint4* rec32[2] = { nullptr, nullptr };
     uint2* meta[2] = { nullptr, nullptr };
+    uint32_t* node_net = nullptr;       // network of each node (the roots are nodes 0..networks-1)
 };
 
 struct Neurons {
@@ -52,6 +53,7 @@ struct Neurons {
     double* delta = nullptr;      // 3n
     uint32_t* connected = nullptr;  // 3n
     uint32_t* deleted = nullptr;  // 3n
+    double* net_nu = nullptr;     // growth rates per network (3 each), null: p.nu everywhere
     rl_step_params p{};
     bool ready = false;
 };
@@ -76,8 +78,23 @@ struct rl_bh_ctx {
     uint32_t* leaf_of = nullptr;    // neuron -> leaf node
     uint32_t* order = nullptr;      // neurons in Morton order (search order)
     rl::Tree tree;
-    double qscale[3] = { 0, 0, 0 }; // fixed-point units per micrometre
     bool prepared = false;
+
+    // Networks of the context: one for rl_bh_init, an ensemble of independent networks for
+    // rl_bh_init_networks.  Network g owns neurons [net_off[g], net_off[g+1]), the box
+    // net_box[6g..6g+5] (lo xyz, hi xyz), the octree rooted at node g, its own fixed-point scale and
+    // its own draw stream.
+    uint32_t num_networks = 0;
+    std::vector<uint64_t> h_net_off;
+    std::vector<double> h_net_box;
+    std::vector<double> h_net_scale;
+    uint64_t* net_off = nullptr;    // device copies
+    double* net_box = nullptr;
+    double* net_scale = nullptr;    // fixed-point units per micrometre (2^31 - 1 across the largest extent)
+    uint64_t* net_seed = nullptr;
+    float* net_k2q = nullptr;       // -log2(e) / sigma^2 in squared fixed-point units
+    float* net_invq = nullptr;      // micrometres per fixed-point unit
+    uint32_t* net_of = nullptr;     // n
 
     // search results (slot layout: vacant axons per neuron at the exclusive prefix sum)
     uint64_t slots = 0;
@@ -191,6 +208,7 @@ int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input);
 int neurons_commit(rl_bh_ctx* c, uint32_t* deleted);
 int neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
                 uint32_t* connected);
+int neurons_set_network_growth(rl_bh_ctx* c, const double* nu);
 void neurons_free(rl_bh_ctx* c);
 
 }  // namespace rl

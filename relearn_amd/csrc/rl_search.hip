@@ -86,7 +86,7 @@ __device__ uint32_t traverse(const SearchArgs& a, const Lane& L, uint32_t root, 
         const Eval e = kMixed ? eval_mixed(a, L, root, mr.x, true) : eval_exact(a, L, root, mr.x, true);
         if (e.status != kDiscard) {
             const bool self = (mr.y & kLinkMask) == L.src;
-            v(root, self ? 0.0 : probability<kMixed>(a, e, mr.x, kind));
+            v(root, self ? 0.0 : probability<kMixed>(a, L, e, mr.x, kind));
         }
         return tests;
     }
@@ -128,7 +128,7 @@ __device__ uint32_t traverse(const SearchArgs& a, const Lane& L, uint32_t root, 
         }
         if (e.status == kAccept) {
             const bool self = leaf && (m.y & kLinkMask) == L.src;
-            v(node, self ? 0.0 : probability<kMixed>(a, e, m.x, kind));
+            v(node, self ? 0.0 : probability<kMixed>(a, L, e, m.x, kind));
             continue;
         }
         if (top != 0) {
@@ -222,22 +222,10 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
         return;
     }
     Lane L;
-    L.src = src;
-    L.sig = a.signal[src] == 0 ? 0 : 1;
-    lane_bind(a, L);
-    L.px = a.pos[src];
-    L.py = a.pos[a.n + src];
-    L.pz = a.pos[2 * a.n + src];
-    if (kMixed) {
-        L.qx = static_cast<int32_t>(a.qpos[src]);
-        L.qy = static_cast<int32_t>(a.qpos[a.n + src]);
-        L.qz = static_cast<int32_t>(a.qpos[2 * a.n + src]);
-    } else {
-        L.qx = L.qy = L.qz = 0;
-    }
+    lane_source(a, L, src, kMixed);
     uint64_t exec = 0;
     uint64_t ref = 0;
-    const uint2 mroot = L.meta[0];
+    const uint2 mroot = L.meta[L.root];
     if ((mroot.y >> kLinkBits) == 0) {
         // the root is a leaf (one neuron): it holds the source, so no target (BarnesHutBase.h:231-237)
         const uint32_t nid = mroot.y & kLinkMask;
@@ -257,7 +245,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
         return;
     }
     // level 1: one list for all axons of the neuron
-    const ListSum l1 = list_sum<kMixed, kFallback>(a, L, 0, stk, stride);
+    const ListSum l1 = list_sum<kMixed, kFallback>(a, L, L.root, stk, stride);
     if (l1.redo) {
         a.redo[atomicAdd(a.n_redo, 1U)] = src;
         return;
@@ -279,9 +267,9 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
 #pragma unroll
         for (int k = 0; k < kDrawsPerPass; k++) {
             pv.pick[k] = kNone;
-            pv.r[k] = k < pv.nd ? draw_to_uniform(pick_draw(a.seed, src, a.step, j0 + k, 1U), next1) : 0.0;
+            pv.r[k] = k < pv.nd ? draw_to_uniform(pick_draw(L.seed, L.local, a.step, j0 + k, 1U), next1) : 0.0;
         }
-        exec += list_pick<kMixed, kFallback>(a, L, 0, l1, pv, stk, stride);
+        exec += list_pick<kMixed, kFallback>(a, L, L.root, l1, pv, stk, stride);
 #pragma unroll
         for (int k = 0; k < kDrawsPerPass; k++) {
             if (k < pv.nd) {
@@ -325,7 +313,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
                 pv.pick[k] = kNone;
                 pv.r[k] = 0.0;
             }
-            pv.r[0] = draw_to_uniform(pick_draw(a.seed, src, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
+            pv.r[0] = draw_to_uniform(pick_draw(L.seed, L.local, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
             exec += list_pick<kMixed, kFallback>(a, L, node, ls, pv, stk, stride);
             node = pv.pick[0];
         }
@@ -361,19 +349,7 @@ __global__ void __launch_bounds__(kBlock) k_descend_lane(SearchArgs a, const uin
     const uint32_t slot = it.x, src = it.y;
     uint32_t node = it.z;
     Lane L;
-    L.src = src;
-    L.sig = a.signal[src] == 0 ? 0 : 1;
-    lane_bind(a, L);
-    L.px = a.pos[src];
-    L.py = a.pos[a.n + src];
-    L.pz = a.pos[2 * a.n + src];
-    if (kMixed) {
-        L.qx = static_cast<int32_t>(a.qpos[src]);
-        L.qy = static_cast<int32_t>(a.qpos[a.n + src]);
-        L.qz = static_cast<int32_t>(a.qpos[2 * a.n + src]);
-    } else {
-        L.qx = L.qy = L.qz = 0;
-    }
+    lane_source(a, L, src, kMixed);
     const uint32_t j = slot - static_cast<uint32_t>(a.slot_off[src]);
     uint64_t exec = 0;
     uint32_t tests = 0;
@@ -402,7 +378,7 @@ __global__ void __launch_bounds__(kBlock) k_descend_lane(SearchArgs a, const uin
             pv.pick[k] = kNone;
             pv.r[k] = 0.0;
         }
-        pv.r[0] = draw_to_uniform(pick_draw(a.seed, src, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
+        pv.r[0] = draw_to_uniform(pick_draw(L.seed, L.local, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
         exec += list_pick<kMixed, true>(a, L, node, ls, pv, stk, stride);
         node = pv.pick[0];
     }
@@ -540,10 +516,12 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     a.mu = c->cfg.mu;
     a.theta2f = static_cast<float>(c->cfg.theta * c->cfg.theta);
     a.k2f = static_cast<float>(-1.4426950408889634 / (c->cfg.sigma * c->cfg.sigma));
-    const double inv_q = c->qscale[0] > 0.0 ? 1.0 / c->qscale[0] : 0.0;
-    a.k2q = static_cast<float>(-1.4426950408889634 / (c->cfg.sigma * c->cfg.sigma) * inv_q * inv_q);
     a.muf = static_cast<float>(c->cfg.mu);
-    a.inv_q = static_cast<float>(inv_q);
+    a.net_of = c->net_of;
+    a.net_off = c->net_off;
+    a.net_seed = c->net_seed;
+    a.net_k2q = c->net_k2q;
+    a.net_invq = c->net_invq;
     a.guard = 2.0e-5f;
     a.seed = c->cfg.seed;
     a.step = step;

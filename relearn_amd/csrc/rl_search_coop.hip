@@ -178,9 +178,9 @@ __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, con
         int st = (leaf || g > 0.0f) ? kAccept : kExpand;
         float x;
         if (kMu0) {
-            x = d2 * a.k2q;
+            x = d2 * L.k2q;
         } else {
-            const float dd = sqrtf(d2) * a.inv_q - a.muf;
+            const float dd = sqrtf(d2) * L.inv_q - a.muf;
             x = dd * dd * a.k2f;
         }
         const float w = __builtin_amdgcn_exp2f(x) * static_cast<float>(d.m.x);
@@ -192,7 +192,7 @@ __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, con
             if (need) {
                 const Eval e = eval_exact(a, L, node, 0, leaf);
                 st = e.status;
-                p = st == kAccept ? probability<false>(a, e, d.m.x, kGauss) : 0.0;
+                p = st == kAccept ? probability<false>(a, L, e, d.m.x, kGauss) : 0.0;
             }
         }
         *p_out = p;
@@ -204,7 +204,7 @@ __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, con
             const Eval e = eval_exact_rec(a, L, d.r64, leaf);
             st = e.status;
             if (st == kAccept) {
-                p = probability<false>(a, e, d.m.x, kGauss);
+                p = probability<false>(a, L, e, d.m.x, kGauss);
             }
         }
         *p_out = p;
@@ -589,7 +589,7 @@ __device__ __forceinline__ int descend_fast(const SearchArgs& s, const Lane& L, 
             }
         }
         const bool pick = state == kDrawActive;
-        const double r = pick ? draw_to_uniform(pick_draw(s.seed, L.src, s.step, jd, level), nextafter(gsum, gsum + 0.00001))
+        const double r = pick ? draw_to_uniform(pick_draw(L.seed, L.local, s.step, jd, level), nextafter(gsum, gsum + 0.00001))
                               : 0.0;
         const double pa = st == kAccept ? p : 0.0;
         const double cum = group_suffix8(pa);
@@ -648,6 +648,13 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
         L.src = src;
         L.sig = __builtin_amdgcn_readfirstlane(s.signal[src] == 0 ? 0 : 1);
         lane_bind(s, L);
+        const uint32_t net = __builtin_amdgcn_readfirstlane(s.net_of[src]);
+        L.root = net;
+        L.local = src - __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s.net_off[net]));
+        L.seed = (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s.net_seed[net] >> 32U)))) << 32U) |
+                 static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s.net_seed[net] & 0xFFFFFFFFULL)));
+        L.k2q = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s.net_k2q[net])));
+        L.inv_q = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s.net_invq[net])));
         L.px = uniform_f64(s.pos[src]);
         L.py = uniform_f64(s.pos[s.n + src]);
         L.pz = uniform_f64(s.pos[2 * s.n + src]);
@@ -658,7 +665,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
         } else {
             L.qx = L.qy = L.qz = 0;
         }
-        const ListResult l1 = list_of<kMixed, kMu0>(a, L, 0, sh, base);
+        const ListResult l1 = list_of<kMixed, kMu0>(a, L, L.root, sh, base);
         acc_exec += l1.tests;
         if (l1.overflow) {
             if (ln == 0) {
@@ -675,7 +682,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
             uint32_t nav_tests = 0;
             for (uint32_t j0 = 0; j0 < na; j0 += 8) {
                 const bool gact = j0 + q < na;
-                const double r = gact ? draw_to_uniform(pick_draw(s.seed, src, s.step, j0 + q, 1U), next1) : 0.0;
+                const double r = gact ? draw_to_uniform(pick_draw(L.seed, L.local, s.step, j0 + q, 1U), next1) : 0.0;
                 uint32_t node = navigate<kMixed, kMu0>(a, L, r, gact, base, &nav_tests);
                 uint32_t level = 2, tests = l1.tests;
                 int state = !gact ? kDrawIdle : (node == kNone ? kDrawFailed : kDrawActive);

@@ -27,10 +27,15 @@ struct SearchArgs {
     double mu;
     float theta2f;       // theta^2
     float k2f;           // -log2(e) / sigma^2 (micrometres)
-    float k2q;           // -log2(e) / sigma^2 in squared fixed-point units
     float muf;
-    float inv_q;         // micrometres per fixed-point unit (one scale for all axes)
     float guard;         // relative guard band of the FP32 acceptance test
+    // networks: network of each neuron, first neuron of each network, and per network the draw seed
+    // and the fixed-point constants (-log2(e) / sigma^2 in squared units, micrometres per unit)
+    const uint32_t* net_of;
+    const uint64_t* net_off;
+    const uint64_t* net_seed;
+    const float* net_k2q;
+    const float* net_invq;
     uint64_t seed;
     uint32_t step;
     uint32_t* slot_target;
@@ -47,7 +52,11 @@ struct SearchArgs {
 struct Lane {
     double px, py, pz;
     int32_t qx, qy, qz;
-    uint32_t src;
+    uint32_t src;        // neuron id in the context
+    uint32_t local;      // neuron id inside its network (the draw-stream address)
+    uint32_t root;       // root node of the network
+    uint64_t seed;       // draw-stream seed of the network
+    float k2q, inv_q;    // fixed-point constants of the network
     int sig;
     const uint2* meta;
     const double4* rec64;
@@ -58,6 +67,29 @@ __device__ __forceinline__ void lane_bind(const SearchArgs& a, Lane& L) {
     L.meta = L.sig ? a.meta[1] : a.meta[0];
     L.rec64 = L.sig ? a.rec64[1] : a.rec64[0];
     L.rec32 = L.sig ? a.rec32[1] : a.rec32[0];
+}
+
+// Everything a search needs about source neuron src (one lane's copy).
+__device__ __forceinline__ void lane_source(const SearchArgs& a, Lane& L, uint32_t src, bool mixed) {
+    L.src = src;
+    L.sig = a.signal[src] == 0 ? 0 : 1;
+    lane_bind(a, L);
+    const uint32_t g = a.net_of[src];
+    L.root = g;
+    L.local = src - static_cast<uint32_t>(a.net_off[g]);
+    L.seed = a.net_seed[g];
+    L.k2q = a.net_k2q[g];
+    L.inv_q = a.net_invq[g];
+    L.px = a.pos[src];
+    L.py = a.pos[a.n + src];
+    L.pz = a.pos[2 * a.n + src];
+    if (mixed) {
+        L.qx = static_cast<int32_t>(a.qpos[src]);
+        L.qy = static_cast<int32_t>(a.qpos[a.n + src]);
+        L.qz = static_cast<int32_t>(a.qpos[2 * a.n + src]);
+    } else {
+        L.qx = L.qy = L.qz = 0;
+    }
 }
 
 enum : int { kDiscard = 0, kAccept = 1, kExpand = 2 };
@@ -130,7 +162,7 @@ __device__ __forceinline__ Eval eval_mixed(const SearchArgs& a, const Lane& L, u
 
 // kMuZero: the caller guarantees mu == 0 (the default kernel), so the mixed path needs no square root.
 template <bool kMixed, bool kMuZero = false>
-__device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e, uint32_t cnt, int kind) {
+__device__ __forceinline__ double probability(const SearchArgs& a, const Lane& L, const Eval& e, uint32_t cnt, int kind) {
 #pragma clang fp contract(off)
     if (kind == kInverse) {
         return static_cast<double>(cnt) / e.d64;
@@ -144,9 +176,9 @@ __device__ __forceinline__ double probability(const SearchArgs& a, const Eval& e
     }
     float x;
     if (kMuZero || a.muf == 0.0f) {
-        x = e.d2f * a.k2q;
+        x = e.d2f * L.k2q;
     } else {
-        const float d = sqrtf(e.d2f) * a.inv_q - a.muf;
+        const float d = sqrtf(e.d2f) * L.inv_q - a.muf;
         x = d * d * a.k2f;
     }
     return static_cast<double>(exp2f(x)) * static_cast<double>(cnt);

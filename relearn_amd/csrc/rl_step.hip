@@ -7,6 +7,8 @@
 //                                       (source/neurons/synaptic_elements/SynapticElementsCPU.cpp:25-49,
 //                                        GrowthCurves.h:30-54)
 // and, at plasticity steps, ElementBaseBase::update (ElementBaseBase.h:420-487) per element type.
+#include <cmath>
+
 #include "rl_device.h"
 #include "rl_internal.h"
 
@@ -48,7 +50,10 @@ __device__ __forceinline__ double normal_input(uint64_t seed, uint32_t neuron, u
 struct StepArgs {
     uint64_t n;
     rl_step_params p;
-    uint64_t seed;
+    const uint32_t* net_of;      // network of each neuron
+    const uint64_t* net_off;     // first neuron of each network
+    const uint64_t* net_seed;    // draw seed of each network
+    const double* net_nu;        // growth rates per network (3 each) or null: p.nu for all
     uint32_t step;
     double* x;
     double* u;
@@ -67,9 +72,10 @@ __global__ void k_step(StepArgs a) {
         return;
     }
     const rl_step_params& p = a.p;
+    const uint32_t g = a.net_of[i];
     const double input = p.input_mode == 0 ? a.input[i]
-                                           : normal_input(a.seed, static_cast<uint32_t>(i), a.step, p.input_mean,
-                                                          p.input_stddev);
+                                           : normal_input(a.net_seed[g], static_cast<uint32_t>(i - a.net_off[g]), a.step,
+                                                          p.input_mean, p.input_stddev);
     const double scale = 1.0 / static_cast<double>(p.h);
     const double ab = p.a * p.b;
     double xv = a.x[i];
@@ -101,7 +107,8 @@ __global__ void k_step(StepArgs a) {
     for (int t = 0; t < 3; t++) {
         const double clamped_target = tgt < p.eta[t] ? p.eta[t] : tgt;  // std::max(target, eta)
         const double dz = growth_curve(c, p.eta[t], clamped_target);
-        a.delta[t * a.n + i] += dz * p.nu[t];
+        const double nu = a.net_nu != nullptr ? a.net_nu[3 * static_cast<uint64_t>(g) + t] : p.nu[t];
+        a.delta[t * a.n + i] += dz * nu;
     }
 }
 
@@ -164,7 +171,29 @@ void neurons_free(rl_bh_ctx* c) {
     dfree(&s.grown);
     dfree(&s.delta);
     dfree(&s.deleted);
+    dfree(&s.net_nu);
     s.ready = false;
+}
+
+int neurons_set_network_growth(rl_bh_ctx* c, const double* nu) {
+    Neurons& s = c->neurons;
+    if (!s.ready) {
+        return fail(c, RL_ESTATE, "rl_neurons_set_network_growth before rl_neurons_init");
+    }
+    if (nu == nullptr) {
+        dfree(&s.net_nu);
+        return RL_OK;
+    }
+    const size_t cnt = 3 * static_cast<size_t>(c->num_networks);
+    for (size_t k = 0; k < cnt; k++) {
+        if (!std::isfinite(nu[k])) {
+            return fail(c, RL_EINVAL, "rl_neurons_set_network_growth: growth rates must be finite");
+        }
+    }
+    RL_TRY(dalloc(c, &s.net_nu, cnt));
+    RL_HIP(c, hipMemcpyAsync(s.net_nu, nu, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    return RL_OK;
 }
 
 int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const double* u, const double* ca,
@@ -226,7 +255,10 @@ int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
     StepArgs a{};
     a.n = n;
     a.p = s.p;
-    a.seed = c->cfg.seed;
+    a.net_of = c->net_of;
+    a.net_off = c->net_off;
+    a.net_seed = c->net_seed;
+    a.net_nu = s.net_nu;
     a.step = step;
     a.x = s.x;
     a.u = s.u;

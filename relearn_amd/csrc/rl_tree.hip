@@ -12,6 +12,11 @@
 // node.  Nodes are sorted by (level, first point) = breadth-first with Morton order inside a level,
 // so the children of a node are contiguous.
 //
+// Ensembles: every network gets its own key space (its own box); neurons are sorted by (network,
+// key) and the common prefix between the last point of one network and the first of the next is
+// taken as -1, so each network starts a root at level 0.  The forest is breadth-first over all
+// networks together, so the roots are nodes 0..networks-1 and children stay contiguous.
+//
 // Upward pass (per plasticity step): leaves get the vacant dendrite counts
 // (OctreeAlgorithm::update_leaf_nodes, OctreeAlgorithm.h:154-238), then one launch per level from the
 // deepest to the root sums children in octant order and forms the count-weighted centroid in FP64
@@ -35,10 +40,6 @@ inline unsigned grid_for(uint64_t n, int block = kBlock) {
     return static_cast<unsigned>(g == 0 ? 1 : g);
 }
 
-struct BoxArgs {
-    double lo[3];
-    double hi[3];
-};
 
 __device__ __forceinline__ uint32_t digit_of(uint64_t khi, uint64_t klo, uint32_t level) {
     // level 1..21 in khi (level 1 most significant), 22..42 in klo
@@ -48,17 +49,19 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t khi, uint64_t klo, uint32_
     return static_cast<uint32_t>((klo >> (3U * (42U - level))) & 7ULL);
 }
 
-// Key of every neuron by replaying the octant decisions level by level.
-__global__ void k_keys(uint64_t n, const double* __restrict__ pos, BoxArgs box, uint64_t* __restrict__ khi,
-                       uint64_t* __restrict__ klo, uint32_t* __restrict__ idx, int* __restrict__ err) {
+// Key of every neuron by replaying the octant decisions level by level in its network's box.
+__global__ void k_keys(uint64_t n, const double* __restrict__ pos, const uint32_t* __restrict__ net_of,
+                       const double* __restrict__ net_box, uint64_t* __restrict__ khi, uint64_t* __restrict__ klo,
+                       uint32_t* __restrict__ idx, int* __restrict__ err) {
 #pragma clang fp contract(off)
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
         return;
     }
     const double p[3] = { pos[i], pos[n + i], pos[2 * n + i] };
-    double lo[3] = { box.lo[0], box.lo[1], box.lo[2] };
-    double hi[3] = { box.hi[0], box.hi[1], box.hi[2] };
+    const double* box = net_box + 6 * static_cast<uint64_t>(net_of[i]);
+    double lo[3] = { box[0], box[1], box[2] };
+    double hi[3] = { box[3], box[4], box[5] };
     bool inside = true;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -114,11 +117,18 @@ __global__ void k_any_equal(uint64_t n, const uint64_t* __restrict__ k, int* __r
     }
 }
 
-// c(p) for p in [0, n-1): shared leading digits of sorted keys p and p+1 (42 = identical keys).
+constexpr uint8_t kNoPrefix = 0xFF;   // lcp of two points of different networks (-1)
+
+// c(p) for p in [0, n-1): shared leading digits of sorted keys p and p+1 (42 = identical keys), or
+// kNoPrefix when p and p+1 belong to different networks.
 __global__ void k_lcp(uint64_t n, const uint64_t* __restrict__ khi, const uint64_t* __restrict__ klo,
-                      uint8_t* __restrict__ lcp, int* __restrict__ err) {
+                      const uint32_t* __restrict__ netp, uint8_t* __restrict__ lcp, int* __restrict__ err) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i + 1 >= n) {
+        return;
+    }
+    if (netp[i] != netp[i + 1]) {
+        lcp[i] = kNoPrefix;
         return;
     }
     const uint64_t x = khi[i] ^ khi[i + 1];
@@ -136,7 +146,36 @@ __global__ void k_lcp(uint64_t n, const uint64_t* __restrict__ khi, const uint64
 }
 
 __device__ __forceinline__ int lcp_at(const uint8_t* lcp, uint64_t n, int64_t p) {
-    return (p < 0 || p + 1 >= static_cast<int64_t>(n)) ? -1 : static_cast<int>(lcp[p]);
+    if (p < 0 || p + 1 >= static_cast<int64_t>(n) || lcp[p] == kNoPrefix) {
+        return -1;
+    }
+    return static_cast<int>(lcp[p]);
+}
+
+// Network of every sorted position.
+__global__ void k_net_of_sorted(uint64_t n, const uint32_t* __restrict__ order, const uint32_t* __restrict__ net_of,
+                                uint32_t* __restrict__ netp) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        netp[i] = net_of[order[i]];
+    }
+}
+
+// out[i] = in[perm[i]] for 64-bit and 32-bit arrays
+__global__ void k_permute64(uint64_t n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ in,
+                            uint64_t* __restrict__ out) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        out[i] = in[perm[i]];
+    }
+}
+
+__global__ void k_permute32(uint64_t n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in,
+                            uint32_t* __restrict__ out) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        out[i] = in[perm[i]];
+    }
 }
 
 __global__ void k_node_counts(uint64_t n, const uint8_t* __restrict__ lcp, uint64_t* __restrict__ cnt) {
@@ -188,10 +227,12 @@ __global__ void k_level_starts(uint64_t m, const uint64_t* __restrict__ keys, ui
 // Per node: level, parent, box, max edge, octant; leaves get the neuron id and position.
 __global__ void k_link(uint64_t m, uint64_t n, const uint64_t* __restrict__ keys, const uint64_t* __restrict__ ls,
                        const uint8_t* __restrict__ lcp, const uint64_t* __restrict__ khi, const uint64_t* __restrict__ klo,
-                       const uint32_t* __restrict__ sorted, const double* __restrict__ pos, BoxArgs box,
+                       const uint32_t* __restrict__ sorted, const uint32_t* __restrict__ netp,
+                       const double* __restrict__ pos, const double* __restrict__ net_box,
                        uint8_t* __restrict__ level, uint8_t* __restrict__ octant, uint32_t* __restrict__ parent,
                        double* __restrict__ nbox, uint32_t* __restrict__ link, uint8_t* __restrict__ isleaf,
-                       uint32_t* __restrict__ leaf_of, double4* __restrict__ rec0, double4* __restrict__ rec1) {
+                       uint32_t* __restrict__ leaf_of, uint32_t* __restrict__ node_net, double4* __restrict__ rec0,
+                       double4* __restrict__ rec1) {
 #pragma clang fp contract(off)
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= m) {
@@ -202,8 +243,11 @@ __global__ void k_link(uint64_t m, uint64_t n, const uint64_t* __restrict__ keys
     const uint64_t p = key & 0xFFFFFFFFULL;
     level[i] = static_cast<uint8_t>(l);
     const uint64_t a = khi[p], b = klo[p];
-    double lo[3] = { box.lo[0], box.lo[1], box.lo[2] };
-    double hi[3] = { box.hi[0], box.hi[1], box.hi[2] };
+    const uint32_t g = netp[p];
+    node_net[i] = g;
+    const double* box = net_box + 6 * static_cast<uint64_t>(g);
+    double lo[3] = { box[0], box[1], box[2] };
+    double hi[3] = { box[3], box[4], box[5] };
     for (uint32_t k = 1; k <= l; k++) {
         const uint32_t dig = digit_of(a, b, k);
 #pragma unroll
@@ -325,8 +369,8 @@ __device__ __forceinline__ uint32_t quantize(double v, double lo, double scale) 
 }
 
 struct QuantArgs {
-    double lo[3];
-    double scale[3];
+    const double* net_box;     // per network: lo xyz, hi xyz
+    const double* net_scale;   // per network
     int enabled;
 };
 
@@ -341,7 +385,7 @@ __device__ __forceinline__ uint32_t quant_side2(double side, double scale) {
 __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __restrict__ nbox,
                                double4* __restrict__ rec0, double4* __restrict__ rec1, uint2* __restrict__ meta0,
                                uint2* __restrict__ meta1, uint4* __restrict__ q0, uint4* __restrict__ q1,
-                               QuantArgs qa) {
+                               const uint32_t* __restrict__ node_net, QuantArgs qa) {
 #pragma clang fp contract(off)
     const uint64_t i = first + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= last) {
@@ -390,42 +434,49 @@ __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __re
         rec[i] = r;
         if (qa.enabled) {
             uint4* q = s == 0 ? q0 : q1;
-            q[i] = make_uint4(quantize(r.x, qa.lo[0], qa.scale[0]), quantize(r.y, qa.lo[1], qa.scale[1]),
-                              quantize(r.z, qa.lo[2], qa.scale[2]), quant_side2(r.w, qa.scale[0]));
+            const uint32_t g = node_net[i];
+            const double* lo = qa.net_box + 6 * static_cast<uint64_t>(g);
+            const double sc = qa.net_scale[g];
+            q[i] = make_uint4(quantize(r.x, lo[0], sc), quantize(r.y, lo[1], sc), quantize(r.z, lo[2], sc),
+                              quant_side2(r.w, sc));
         }
     }
 }
 
-__global__ void k_quant_leaves(uint64_t m, const double4* __restrict__ rec, const uint2* __restrict__ meta,
+__global__ void k_quant_leaves(uint64_t m, const double4* __restrict__ rec, const uint32_t* __restrict__ node_net,
                                uint4* __restrict__ q0, uint4* __restrict__ q1, QuantArgs qa) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= m) {
         return;
     }
     const double4 r = rec[i];
-    const uint4 v = make_uint4(quantize(r.x, qa.lo[0], qa.scale[0]), quantize(r.y, qa.lo[1], qa.scale[1]),
-                               quantize(r.z, qa.lo[2], qa.scale[2]), quant_side2(r.w, qa.scale[0]));
+    const uint32_t g = node_net[i];
+    const double* lo = qa.net_box + 6 * static_cast<uint64_t>(g);
+    const double sc = qa.net_scale[g];
+    const uint4 v = make_uint4(quantize(r.x, lo[0], sc), quantize(r.y, lo[1], sc), quantize(r.z, lo[2], sc),
+                               quant_side2(r.w, sc));
     q0[i] = v;
     q1[i] = v;
-    (void)meta;
 }
 
-__global__ void k_quant_positions(uint64_t n, const double* __restrict__ pos, uint32_t* __restrict__ qpos, QuantArgs qa) {
+__global__ void k_quant_positions(uint64_t n, const double* __restrict__ pos, const uint32_t* __restrict__ net_of,
+                                  uint32_t* __restrict__ qpos, QuantArgs qa) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) {
         return;
     }
+    const uint32_t g = net_of[i];
+    const double* lo = qa.net_box + 6 * static_cast<uint64_t>(g);
+    const double sc = qa.net_scale[g];
     for (int d = 0; d < 3; d++) {
-        qpos[d * n + i] = quantize(pos[d * n + i], qa.lo[d], qa.scale[d]);
+        qpos[d * n + i] = quantize(pos[d * n + i], lo[d], sc);
     }
 }
 
 QuantArgs quant_args(const rl_bh_ctx* c) {
     QuantArgs q{};
-    for (int d = 0; d < 3; d++) {
-        q.lo[d] = c->cfg.box_min[d];
-        q.scale[d] = c->qscale[d];
-    }
+    q.net_box = c->net_box;
+    q.net_scale = c->net_scale;
     q.enabled = c->cfg.precision == RL_PREC_FP32_MIXED ? 1 : 0;
     return q;
 }
@@ -442,6 +493,7 @@ void free_tree(rl_bh_ctx* c) {
         dfree(&t.rec32[s]);
         dfree(&t.meta[s]);
     }
+    dfree(&t.node_net);
     t.m = 0;
     t.depth = 0;
     t.level_start.clear();
@@ -467,13 +519,9 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         RL_HIP(c, hipMemcpyAsync(c->pos, soa.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
         RL_HIP(c, hipStreamSynchronize(s));
     }
-    BoxArgs box{};
-    for (int d = 0; d < 3; d++) {
-        box.lo[d] = c->cfg.box_min[d];
-        box.hi[d] = c->cfg.box_max[d];
-    }
+    const uint32_t G = c->num_networks;
     uint64_t *khi = nullptr, *klo = nullptr, *khi2 = nullptr, *klo2 = nullptr;
-    uint32_t *idx = nullptr, *idx2 = nullptr;
+    uint32_t *idx = nullptr, *idx2 = nullptr, *netp = nullptr, *netp2 = nullptr, *perm = nullptr;
     int* err = nullptr;
     uint8_t* lcp = nullptr;
     uint64_t* cnt = nullptr;
@@ -488,6 +536,9 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         dfree(&klo2);
         dfree(&idx);
         dfree(&idx2);
+        dfree(&netp);
+        dfree(&netp2);
+        dfree(&perm);
         dfree(&err);
         dfree(&lcp);
         dfree(&cnt);
@@ -505,7 +556,8 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         if ((status = dalloc(c, &khi, n)) || (status = dalloc(c, &klo, n)) || (status = dalloc(c, &khi2, n)) ||
             (status = dalloc(c, &klo2, n)) || (status = dalloc(c, &idx, n)) || (status = dalloc(c, &idx2, n)) ||
             (status = dalloc(c, &err, 2)) || (status = dalloc(c, &lcp, n)) || (status = dalloc(c, &cnt, n + 1)) ||
-            (status = dalloc(c, &off, n + 1))) {
+            (status = dalloc(c, &off, n + 1)) || (status = dalloc(c, &netp, n)) || (status = dalloc(c, &netp2, n)) ||
+            (status = dalloc(c, &perm, n))) {
             break;
         }
 #define RL_STEP(expr)                                        \
@@ -517,7 +569,7 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         }                                                    \
     }
         RL_STEP(hipMemsetAsync(err, 0, 2 * sizeof(int), s));
-        k_keys<<<grid_for(n), kBlock, 0, s>>>(n, c->pos, box, khi, klo, idx, err);
+        k_keys<<<grid_for(n), kBlock, 0, s>>>(n, c->pos, c->net_of, c->net_box, khi, klo, idx, err);
         RL_STEP(hipGetLastError());
         // sort by the first 21 levels; a second LSD pass over levels 22..42 only when needed
         size_t tb = 0;
@@ -547,6 +599,34 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
             RL_STEP(hipMemcpyAsync(idx, idx2, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
             RL_STEP(hipcub::DeviceRadixSort::SortPairs(c->tmp, tb, khi, khi2, idx, idx2, static_cast<int>(n), 0, 63, s));
         }
+        // networks: a stable sort of the key order by network puts every network's points together,
+        // each in key order
+        k_net_of_sorted<<<grid_for(n), kBlock, 0, s>>>(n, idx2, c->net_of, netp);
+        RL_STEP(hipGetLastError());
+        if (G > 1) {
+            int netbits = 1;
+            while ((1ULL << netbits) < G) {
+                netbits++;
+            }
+            // perm = sorted positions reordered stably by network (netp2 gets the sorted network ids)
+            k_iota<<<grid_for(n), kBlock, 0, s>>>(n, idx);
+            RL_STEP(hipGetLastError());
+            size_t tbn = 0;
+            RL_STEP(hipcub::DeviceRadixSort::SortPairs(nullptr, tbn, netp, netp2, idx, perm, static_cast<int>(n), 0,
+                                                       netbits, s));
+            if ((status = ensure_tmp(c, tbn)) != RL_OK) {
+                break;
+            }
+            RL_STEP(hipcub::DeviceRadixSort::SortPairs(c->tmp, tbn, netp, netp2, idx, perm, static_cast<int>(n), 0,
+                                                       netbits, s));
+            k_permute32<<<grid_for(n), kBlock, 0, s>>>(n, perm, idx2, idx);
+            RL_STEP(hipGetLastError());
+            k_permute64<<<grid_for(n), kBlock, 0, s>>>(n, perm, khi2, khi);
+            RL_STEP(hipGetLastError());
+            RL_STEP(hipMemcpyAsync(idx2, idx, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+            RL_STEP(hipMemcpyAsync(khi2, khi, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+            RL_STEP(hipMemcpyAsync(netp, netp2, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        }
         // idx2 = neurons in key order; khi2 = sorted high keys; gather the low keys
         k_gather64<<<grid_for(n), kBlock, 0, s>>>(n, idx2, klo, klo2);
         RL_STEP(hipGetLastError());
@@ -554,7 +634,7 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
             break;
         }
         RL_STEP(hipMemcpyAsync(c->order, idx2, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-        k_lcp<<<grid_for(n), kBlock, 0, s>>>(n, khi2, klo2, lcp, err);
+        k_lcp<<<grid_for(n), kBlock, 0, s>>>(n, khi2, klo2, netp, lcp, err);
         RL_STEP(hipGetLastError());
         RL_STEP(hipMemsetAsync(cnt + n, 0, sizeof(uint64_t), s));
         k_node_counts<<<grid_for(n), kBlock, 0, s>>>(n, lcp, cnt);
@@ -609,6 +689,10 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
                 depth = l;
             }
         }
+        if (hls[1] != G) {
+            status = fail(c, RL_EDUPPOS, "octree construction produced a forest without one root per network (internal error)");
+            break;
+        }
         t.m = m;
         t.depth = depth;
         t.level_start.assign(hls.begin(), hls.begin() + depth + 2);
@@ -616,7 +700,7 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
             (status = dalloc(c, &t.rec64[0], m)) || (status = dalloc(c, &t.rec64[1], m)) ||
             (status = dalloc(c, &t.meta[0], m)) || (status = dalloc(c, &t.meta[1], m)) ||
             (status = dalloc(c, &parent, m)) || (status = dalloc(c, &link, m)) || (status = dalloc(c, &nchild, m)) ||
-            (status = dalloc(c, &isleaf, m)) || (status = dalloc(c, &c->leaf_of, n))) {
+            (status = dalloc(c, &isleaf, m)) || (status = dalloc(c, &c->leaf_of, n)) || (status = dalloc(c, &t.node_net, m))) {
             break;
         }
         if (c->cfg.precision == RL_PREC_FP32_MIXED) {
@@ -626,8 +710,9 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
             }
         }
         RL_STEP(hipMemsetAsync(nchild, 0, m * sizeof(uint32_t), s));
-        k_link<<<grid_for(m), kBlock, 0, s>>>(m, n, nkeys2, ls, lcp, khi2, klo2, idx2, c->pos, box, t.level, t.octant,
-                                               parent, t.box, link, isleaf, c->leaf_of, t.rec64[0], t.rec64[1]);
+        k_link<<<grid_for(m), kBlock, 0, s>>>(m, n, nkeys2, ls, lcp, khi2, klo2, idx2, netp, c->pos, c->net_box, t.level,
+                                               t.octant, parent, t.box, link, isleaf, c->leaf_of, t.node_net, t.rec64[0],
+                                               t.rec64[1]);
         RL_STEP(hipGetLastError());
         k_children<<<grid_for(m), kBlock, 0, s>>>(m, ls, t.level, parent, link, nchild);
         RL_STEP(hipGetLastError());
@@ -636,9 +721,9 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         RL_STEP(hipGetLastError());
         if (c->cfg.precision == RL_PREC_FP32_MIXED) {
             const QuantArgs qa = quant_args(c);
-            k_quant_leaves<<<grid_for(m), kBlock, 0, s>>>(m, t.rec64[0], t.meta[0], t.rec32[0], t.rec32[1], qa);
+            k_quant_leaves<<<grid_for(m), kBlock, 0, s>>>(m, t.rec64[0], t.node_net, t.rec32[0], t.rec32[1], qa);
             RL_STEP(hipGetLastError());
-            k_quant_positions<<<grid_for(n), kBlock, 0, s>>>(n, c->pos, c->qpos, qa);
+            k_quant_positions<<<grid_for(n), kBlock, 0, s>>>(n, c->pos, c->net_of, c->qpos, qa);
             RL_STEP(hipGetLastError());
         }
         RL_STEP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -672,7 +757,7 @@ int prepare_tree(rl_bh_ctx* c) {
             continue;
         }
         k_upward_level<<<grid_for(last - first), kBlock, 0, s>>>(first, last, t.box, t.rec64[0], t.rec64[1], t.meta[0],
-                                                                  t.meta[1], t.rec32[0], t.rec32[1], qa);
+                                                                  t.meta[1], t.rec32[0], t.rec32[1], t.node_net, qa);
         RL_HIP(c, hipGetLastError());
     }
     return RL_OK;
