@@ -99,6 +99,90 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     }
 
 
+def run_ensemble(args):
+    """Batched plasticity updates of a parameter-sweep ensemble (one context, one launch per phase)
+    next to the same networks updated one context at a time."""
+    import torch
+
+    from relearn_amd import RL_PREC_FP32_MIXED, RL_PREC_FP64, BarnesHutHIP
+    from relearn_amd._lib import network_seed
+    from relearn_amd.workloads import parameter_sweep
+
+    t0 = time.perf_counter()
+    ens = parameter_sweep(args.networks, seed=args.seed)
+    G, n = ens.num_networks, ens.n
+    log(f"ensemble: {G} networks, {n} neurons, generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda:0")
+    d_arrays = [torch.from_numpy(a.view(np.int32)).to(dev) for a in (ens.vac_axon, ens.vac_exc, ens.vac_inh)]
+    torch.cuda.synchronize()
+    d_vac = [int(a.data_ptr()) for a in d_arrays]
+    lo, hi = ens.boxes[:, :3].min(0), ens.boxes[:, 3:].max(0)
+    out = {"metric": "ensemble target searches/s (parameter sweep, one context, one launch per phase)",
+           "unit": "searches/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "data": "synthetic",
+           "config": {"workload": f"{G} networks, sizes log-uniform [2e3, 2e5] ({n} neurons), cube side "
+                                  "1000 (n/1000)^(1/3) um, theta 0.3, sigma 750 um, one plasticity update per step",
+                      "networks": G, "neurons": n, "seed": args.seed}}
+    for name, prec in (("fp32_mixed", RL_PREC_FP32_MIXED), ("fp64", RL_PREC_FP64)):
+        g = BarnesHutHIP(lo, hi, theta=0.3, sigma=750.0, seed=args.seed + 1, precision=prec)
+        t0 = time.perf_counter()
+        g.init_networks(ens.offsets, ens.boxes, ens.xyz, ens.signal)
+        build_s = time.perf_counter() - t0
+        for w in range(args.warmup):
+            g.prepare_update_connectivity_device(*d_vac)
+            g.find_target_neurons(1000 + w)
+            g.commit(1000 + w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        searches = 0
+        for k in range(args.steps):
+            g.prepare_update_connectivity_device(*d_vac)
+            g.find_target_neurons(k)
+            searches += g.last_search_stats()["searches"]
+            g.commit(k)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        g.close()
+        out[name] = {"build_s": build_s, "ms_per_update": 1e3 * wall / args.steps, "searches_per_s": searches / wall}
+        log(f"ensemble {name}: {1e3 * wall / args.steps:.1f} ms per update of all networks, "
+            f"{searches / wall:.4g} searches/s, forest build {build_s:.2f} s")
+    # the same networks, one context each, updated one after another (unbatched)
+    if not args.no_unbatched:
+        ctxs = []
+        t0 = time.perf_counter()
+        for k in range(G):
+            a, b = int(ens.offsets[k]), int(ens.offsets[k + 1])
+            c = BarnesHutHIP(ens.boxes[k, :3], ens.boxes[k, 3:], theta=0.3, sigma=750.0,
+                             seed=network_seed(args.seed + 1, k), precision=RL_PREC_FP32_MIXED)
+            c.init(ens.xyz[a:b], ens.signal[a:b])
+            ctxs.append((c, a))
+        init_s = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        searches = 0
+        for step in range(args.steps):
+            for c, a in ctxs:
+                c.prepare_update_connectivity_device(*[p + 4 * a for p in d_vac])
+                c.find_target_neurons(step)
+                searches += c.last_search_stats()["searches"]
+                c.commit(step)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        for c, _ in ctxs:
+            c.close()
+        out["fp32_mixed_unbatched"] = {"init_s": init_s, "ms_per_update": 1e3 * wall / args.steps,
+                                       "searches_per_s": searches / wall}
+        out["batching_speedup"] = out["fp32_mixed_unbatched"]["ms_per_update"] / out["fp32_mixed"]["ms_per_update"]
+        log(f"unbatched fp32_mixed: {1e3 * wall / args.steps:.1f} ms per update of all networks")
+    out["value"] = out["fp32_mixed"]["searches_per_s"]
+    out["ms_per_step"] = out["fp32_mixed"]["ms_per_update"]
+    out["dtype"] = "f32-mixed"
+    out["vs_baseline"] = None
+    out["scaling"] = "weak"
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def copy_calibration(mib):
     """Runs a plain device-to-device copy of `mib` MiB three times (reference kernel for the HBM counters)."""
     import torch
@@ -164,6 +248,10 @@ def main(argv=None):
                     help="search kernel: wavefront-cooperative (default) or one lane per source in list order")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (reference kernel for rocprofv3 byte counters)")
+    ap.add_argument("--workload", choices=["single", "ensemble"], default="single",
+                    help="single: one 16M-neuron network (default); ensemble: batched parameter-sweep networks")
+    ap.add_argument("--networks", type=int, default=256, help="networks of the ensemble workload")
+    ap.add_argument("--no-unbatched", action="store_true", help="ensemble: skip the one-context-per-network run")
     args = ap.parse_args(argv)
     if args.calibrate_copy:
         return copy_calibration(args.calibrate_copy)
@@ -172,6 +260,9 @@ def main(argv=None):
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the sharded design is documented, not built)", file=sys.stderr)
         return 2
+
+    if args.workload == "ensemble":
+        return run_ensemble(args)
 
     from relearn_amd import RL_FLAG_EXACT_ORDER, RL_PREC_FP32_MIXED, RL_PREC_FP64
     from relearn_amd.workloads import uniform_cube
