@@ -71,6 +71,7 @@ typedef struct {
     double ms_search;           /* target search */
     double ms_commit;           /* conflict resolution + vacancy commit */
     double ms_search_kernel;    /* the dominant search kernel alone (cooperative or per-lane) */
+    double ms_total;            /* the whole update, or the whole replayed period (rl_bh_graph_launch) */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a
@@ -216,6 +217,31 @@ int rl_neurons_set_network_growth(rl_bh_ctx* ctx, const double* nu);
 /* Copies the neuron state: any pointer may be NULL; delta/grown are 3*n (type-major). */
 int rl_neurons_get(rl_bh_ctx* ctx, double* x, double* u, double* ca, uint8_t* fired, double* delta,
                    double* grown, uint32_t* connected);
+
+/* ------------------------------------------------------------------ captured plasticity periods */
+
+/* Captures one plasticity period of the simulation loop (Simulation::simulate, Simulation.cpp:443-459)
+ * into a HIP graph on the context's stream: `steps` neuron steps (rl_neurons_step with counter-based
+ * input, so rl_neurons_init with input_mode 1), then, when steps > 0, the element commit
+ * (rl_neurons_commit_elements), then the connectivity update of rl_bh_update_connectivity on the
+ * resident vacancies (ForwardCPUAlgorithm::update_connectivity, ExchangingAlgorithm.h:58-87: leaf
+ * fill, upward pass, search, commit with counter priorities).  steps = 0 captures the connectivity
+ * update alone.  The slot buffers are sized for max_vacant_axons vacant axons per update; an update
+ * with more searches nothing, changes nothing and is reported as RL_ELIMIT.  Replays allocate
+ * nothing and synchronise with nothing.  rl_bh_init*, rl_neurons_init and
+ * rl_neurons_set_network_growth release the captured period, as does growing the slot buffers. */
+int rl_bh_graph_create(rl_bh_ctx* ctx, uint32_t steps, uint64_t max_vacant_axons);
+
+/* Replays the captured period: neuron steps first_step .. first_step + steps - 1 and connectivity
+ * update index `update` (the step argument of rl_bh_update_connectivity).  With stats, waits for the
+ * period, fills searches, requests, created, node_tests, node_tests_ref and ms_total (the other
+ * times are 0), and makes the requests and accept flags readable (rl_bh_get_requests,
+ * rl_bh_get_commit_result); returns RL_ELIMIT when this or an earlier replay without stats exceeded
+ * max_vacant_axons.  Without stats (NULL) the replay is asynchronous on the context's stream. */
+int rl_bh_graph_launch(rl_bh_ctx* ctx, uint32_t first_step, uint32_t update, rl_bh_step_stats* stats);
+
+/* Releases the captured period (rl_bh_destroy does too). */
+int rl_bh_graph_destroy(rl_bh_ctx* ctx);
 
 #ifdef __cplusplus
 }

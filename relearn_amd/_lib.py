@@ -31,7 +31,8 @@ EXPORTED = [
     "rl_bh_get_requests", "rl_bh_get_search_slots", "rl_bh_commit", "rl_bh_get_commit_result",
     "rl_bh_update_connectivity", "rl_bh_get_vacancies", "rl_bh_tree_info", "rl_bh_export_tree",
     "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
-    "rl_neurons_get", "rl_neurons_set_network_growth",
+    "rl_neurons_get", "rl_neurons_set_network_growth", "rl_bh_graph_create", "rl_bh_graph_launch",
+    "rl_bh_graph_destroy",
 ]
 
 
@@ -61,6 +62,7 @@ class StepStats(C.Structure):
         ("ms_search", C.c_double),
         ("ms_commit", C.c_double),
         ("ms_search_kernel", C.c_double),
+        ("ms_total", C.c_double),
     ]
 
     def as_dict(self):
@@ -149,6 +151,9 @@ def load():
     L.rl_neurons_step.argtypes = [vp, C.c_uint32, _f64p]
     L.rl_neurons_commit_elements.argtypes = [vp, _u32p]
     L.rl_neurons_get.argtypes = [vp, _f64p, _f64p, _f64p, _u8p, _f64p, _f64p, _u32p]
+    L.rl_bh_graph_create.argtypes = [vp, C.c_uint32, C.c_uint64]
+    L.rl_bh_graph_launch.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(StepStats)]
+    L.rl_bh_graph_destroy.argtypes = [vp]
     for name in EXPORTED:
         f = getattr(L, name)
         if name not in ("rl_bh_last_error", "rl_bh_network_seed"):
@@ -291,6 +296,24 @@ class BarnesHutHIP:
         st = StepStats()
         self._check(self.L.rl_bh_update_connectivity(self.h, step, C.byref(st)))
         return st.as_dict()
+
+    def graph_create(self, steps=0, max_vacant_axons=None):
+        """Captures a plasticity period (steps neuron steps, element commit, connectivity update)."""
+        if max_vacant_axons is None:
+            max_vacant_axons = 1
+        self._check(self.L.rl_bh_graph_create(self.h, int(steps), int(max_vacant_axons)))
+
+    def graph_launch(self, first_step=0, update=0, stats=True):
+        """Replays the captured period; returns its stats dict, or None for an asynchronous replay."""
+        if not stats:
+            self._check(self.L.rl_bh_graph_launch(self.h, int(first_step), int(update), None))
+            return None
+        st = StepStats()
+        self._check(self.L.rl_bh_graph_launch(self.h, int(first_step), int(update), C.byref(st)))
+        return st.as_dict()
+
+    def graph_destroy(self):
+        self._check(self.L.rl_bh_graph_destroy(self.h))
 
     def get_vacancies(self):
         va = np.zeros(max(self.n, 1), np.uint32)

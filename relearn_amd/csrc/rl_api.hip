@@ -23,6 +23,14 @@ int hip_fail(rl_bh_ctx* c, hipError_t e, const char* what) {
     return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void destroy_graph(rl_bh_ctx* c) {
+    if (c->graph_exec != nullptr) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+    }
+    c->graph_steps = 0;
+}
+
 int ensure_tmp(rl_bh_ctx* c, size_t bytes) {
     if (bytes <= c->tmp_bytes && c->tmp != nullptr) {
         return RL_OK;
@@ -47,6 +55,7 @@ namespace {
 thread_local std::string g_create_error;
 
 void free_all(rl_bh_ctx* c) {
+    destroy_graph(c);
     free_tree(c);
     neurons_free(c);
     dfree(&c->pos);
@@ -72,23 +81,21 @@ void free_all(rl_bh_ctx* c) {
     dfree(&c->wide);
     dfree(&c->req_off);
     dfree(&c->counters);
-    dfree(&c->flags);
+    dfree(&c->d_steps);
+    dfree(&c->scan_part);
+    dfree(&c->c_gcnt);
+    dfree(&c->c_goff);
     dfree(&c->c_prio);
-    dfree(&c->c_prio2);
     dfree(&c->c_order);
-    dfree(&c->c_idx);
-    dfree(&c->c_idx2);
-    dfree(&c->c_g);
-    dfree(&c->c_g2);
+    dfree(&c->c_sprio);
+    dfree(&c->c_sidx);
     dfree(&c->coop_flag);
     dfree(&c->coop_work);
-    dfree(&c->coop_nwork);
     dfree(&c->coop_cursor);
     dfree(&c->coop_overflow);
     dfree(&c->redo);
     dfree(&c->coop_node);
     dfree(&c->slow_items);
-    dfree(&c->pipe);
     dfree(&c->coop_fexp);
     dfree(&c->coop_mass);
     dfree(&c->net_off);
@@ -104,14 +111,15 @@ void free_all(rl_bh_ctx* c) {
     c->h_net_scale.clear();
     c->coop_cap = 0;
     c->coop_blocks = 0;
-    c->commit_cap = 0;
+    c->scan_part_cap = 0;
     if (c->tmp != nullptr) {
         (void)hipFree(c->tmp);
         c->tmp = nullptr;
         c->tmp_bytes = 0;
     }
     c->slots_cap = 0;
-    c->req_cap = 0;
+    c->slots = 0;
+    c->nreq = 0;
 }
 
 int set_device(rl_bh_ctx* c) {
@@ -293,17 +301,19 @@ int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const doubl
     RL_TRY(dalloc(c, &c->slot_off, n + 1));
     RL_TRY(dalloc(c, &c->wide, n + 1));
     RL_TRY(dalloc(c, &c->req_off, n + 1));
-    RL_TRY(dalloc(c, &c->counters, 8));
-    RL_TRY(dalloc(c, &c->flags, 4));
+    RL_TRY(dalloc(c, &c->counters, kCounters));
+    RL_TRY(dalloc(c, &c->d_steps, 2));
+    c->scan_part_cap = scan_scratch_entries(2 * n);
+    RL_TRY(dalloc(c, &c->scan_part, c->scan_part_cap));
+    RL_TRY(dalloc(c, &c->c_gcnt, 2 * n));
+    RL_TRY(dalloc(c, &c->c_goff, 2 * n + 1));
     RL_TRY(dalloc(c, &c->found, n));
     RL_TRY(dalloc(c, &c->searched, n));
     RL_TRY(dalloc(c, &c->coop_flag, n));
     RL_TRY(dalloc(c, &c->coop_work, n));
-    RL_TRY(dalloc(c, &c->coop_nwork, 1));
-    RL_TRY(dalloc(c, &c->coop_cursor, 10));
+    RL_TRY(dalloc(c, &c->coop_cursor, kCursorSlots));
     RL_TRY(dalloc(c, &c->coop_overflow, n));
     RL_TRY(dalloc(c, &c->redo, n));
-    RL_TRY(dalloc(c, &c->pipe, 8));
     {
         int cus = 0;
         RL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
@@ -321,6 +331,8 @@ int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const doubl
     RL_HIP(c, hipMemcpyAsync(c->net_seed, seeds.data(), G * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     RL_HIP(c, hipMemcpyAsync(c->net_k2q, k2q.data(), G * sizeof(float), hipMemcpyHostToDevice, s));
     RL_HIP(c, hipMemcpyAsync(c->net_invq, invq.data(), G * sizeof(float), hipMemcpyHostToDevice, s));
+    RL_HIP(c, hipMemsetAsync(c->counters, 0, kCounters * sizeof(unsigned long long), s));
+    RL_HIP(c, hipMemsetAsync(c->d_steps, 0, 2 * sizeof(uint32_t), s));
     if (n > 0) {
         RL_HIP(c, hipMemcpyAsync(c->net_of, net_of.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
         RL_HIP(c, hipMemcpyAsync(c->signal, signal, n, hipMemcpyHostToDevice, s));
@@ -563,6 +575,7 @@ int rl_bh_update_connectivity(rl_bh_ctx* c, uint32_t step, rl_bh_step_stats* sta
             stats->ms_search = t12;
             stats->ms_commit = t23;
             stats->ms_search_kernel = c->last_kernel_ms;
+            stats->ms_total = t01 + t12 + t23;
         }
     } while (false);
     for (auto& ev : e) {
@@ -672,6 +685,110 @@ int rl_neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fire
     }
     RL_TRY(set_device(c));
     return neurons_get(c, x, u, ca, fired, delta, grown, connected);
+}
+
+int rl_bh_graph_create(rl_bh_ctx* c, uint32_t steps, uint64_t max_vacant_axons) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    constexpr uint32_t kMaxCapturedSteps = 100000;
+    if (steps > kMaxCapturedSteps) {
+        return fail(c, RL_EINVAL, "rl_bh_graph_create: more than 100000 steps in one period");
+    }
+    if (c->n == 0 || c->tree.m == 0) {
+        return fail(c, RL_ESTATE, "rl_bh_graph_create: no neurons (rl_bh_init first)");
+    }
+    if (steps > 0 && (!c->neurons.ready || c->neurons.p.input_mode == 0)) {
+        return fail(c, RL_ESTATE, "rl_bh_graph_create: neuron steps need rl_neurons_init with input_mode 1");
+    }
+    RL_TRY(set_device(c));
+    destroy_graph(c);
+    RL_TRY(ensure_slots(c, std::max<uint64_t>(max_vacant_axons, 1)));
+    hipStream_t s = c->stream;
+    RL_HIP(c, hipStreamSynchronize(s));
+    RL_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int status = RL_OK;
+    if (steps > 0) {
+        status = enqueue_neuron_steps(c, steps);
+        if (status == RL_OK) status = enqueue_elements_commit(c);
+    }
+    if (status == RL_OK) status = prepare_tree(c);
+    if (status == RL_OK) status = enqueue_search(c);
+    if (status == RL_OK) status = enqueue_commit(c, nullptr);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (status != RL_OK || e != hipSuccess) {
+        if (g != nullptr) {
+            (void)hipGraphDestroy(g);
+        }
+        return status != RL_OK ? status : hip_fail(c, e, "hipStreamEndCapture");
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->graph_exec = nullptr;
+        return hip_fail(c, ei, "hipGraphInstantiate");
+    }
+    c->graph_steps = steps;
+    c->prepared = false;
+    c->searched_valid = false;
+    return RL_OK;
+}
+
+int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh_step_stats* stats) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (c->graph_exec == nullptr) {
+        return fail(c, RL_ESTATE, "rl_bh_graph_launch: no captured period (rl_bh_graph_create)");
+    }
+    RL_TRY(set_device(c));
+    hipStream_t s = c->stream;
+    RL_TRY(set_steps(c, first_step, update));
+    if (stats != nullptr) {
+        RL_HIP(c, hipEventRecord(c->ev0, s));
+    }
+    RL_HIP(c, hipGraphLaunch(c->graph_exec, s));
+    c->prepared = false;
+    c->searched_valid = false;
+    c->nreq = 0;
+    if (stats == nullptr) {
+        return RL_OK;
+    }
+    RL_HIP(c, hipEventRecord(c->ev1, s));
+    unsigned long long h[kCounters];
+    RL_TRY(read_counters(c, h));
+    float ms = 0.0f;
+    RL_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *stats = rl_bh_step_stats{};
+    stats->searches = h[kCtrSearches];
+    stats->requests = h[kCtrRequests];
+    stats->created = h[kCtrCreated];
+    stats->node_tests = h[kCtrTestsExec];
+    stats->node_tests_ref = h[kCtrTestsRef];
+    stats->ms_total = ms;
+    c->slots = h[kCtrSlots];
+    c->nreq = h[kCtrRequests];
+    c->last_searches = h[kCtrSearches];
+    c->last_tests = h[kCtrTestsExec];
+    c->last_tests_ref = h[kCtrTestsRef];
+    c->last_slow_descents = h[kCtrSlow];
+    c->searched_valid = true;
+    if (h[kCtrOverCap] != 0 || h[kCtrSticky] != 0) {
+        return fail(c, RL_ELIMIT, "rl_bh_graph_launch: an update had more vacant axons than max_vacant_axons");
+    }
+    return RL_OK;
+}
+
+int rl_bh_graph_destroy(rl_bh_ctx* c) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    destroy_graph(c);
+    return RL_OK;
 }
 
 }  // extern "C"

@@ -16,6 +16,26 @@ constexpr uint32_t kLinkBits = 28;           // node / neuron index bits in the 
 constexpr uint32_t kLinkMask = (1U << kLinkBits) - 1U;
 constexpr uint32_t kNoTarget = 0xFFFFFFFFU;
 constexpr int kDrawsPerPass = 8;             // draws resolved per pick traversal
+constexpr unsigned kStreamBlocks = 2048;     // grid of the grid-stride kernels sized on the device
+constexpr int kCursorSlots = 8;              // work cursors of the cooperative search, one per XCD
+
+// Device counters of one update (rl_bh_ctx::counters), zeroed at the start of every search and read
+// back with one copy.
+enum : int {
+    kCtrTestsExec = 0,   // node tests executed
+    kCtrTestsRef = 1,    // node tests the reference's traversal makes
+    kCtrSearches = 2,
+    kCtrCreated = 3,     // synapses created by the commit
+    kCtrRequests = 4,
+    kCtrSlots = 5,       // vacant axons of the update
+    kCtrOverCap = 6,     // 1: more vacant axons than slots_cap, nothing was searched
+    kCtrSlow = 7,        // descents queued for the per-lane traversal
+    kCtrOverflow = 8,    // sources whose root list overflowed the cooperative slot table
+    kCtrRedo = 9,        // sources of the per-lane path that need the fallback pass
+    kCtrActive = 10,     // sources with vacant axons
+    kCtrSticky = 11,     // over-capacity seen since the host last read the counters
+    kCounters = 16
+};
 
 struct Status {
     int code = RL_OK;
@@ -96,17 +116,18 @@ struct rl_bh_ctx {
     float* net_invq = nullptr;      // micrometres per fixed-point unit
     uint32_t* net_of = nullptr;     // n
 
-    // search results (slot layout: vacant axons per neuron at the exclusive prefix sum)
+    // search results (slot layout: vacant axons per neuron at the exclusive prefix sum).  One
+    // capacity, slots_cap, sizes every per-slot and per-request buffer: a neuron's requests are a
+    // prefix of its slots, so there are never more requests than slots.
     uint64_t slots = 0;
     uint64_t slots_cap = 0;
-    uint64_t* slot_off = nullptr;   // n
+    uint64_t* slot_off = nullptr;   // n + 1
     uint32_t* slot_target = nullptr;
     uint32_t* slot_tests = nullptr;
     uint32_t* found = nullptr;      // n
     uint32_t* searched = nullptr;   // n
-    // dense requests
+    // dense requests in (source, axon ordinal) order
     uint64_t nreq = 0;
-    uint64_t req_cap = 0;
     uint32_t* req_target = nullptr;
     uint32_t* req_source = nullptr;
     uint8_t* req_signal = nullptr;
@@ -119,24 +140,24 @@ struct rl_bh_ctx {
     uint64_t last_lane_sources = 0;  // sources handled by the per-lane kernel in the last search
     uint64_t last_slow_descents = 0; // descents handed to the per-lane general traversal
 
-    // persistent scratch (grown on demand, never per call)
-    uint64_t* wide = nullptr;        // n + 1
+    // persistent scratch, sized at init or with slots_cap (never inside an update)
+    uint64_t* wide = nullptr;        // n + 1: offsets of the active sources
     uint64_t* req_off = nullptr;     // n + 1
-    unsigned long long* counters = nullptr;  // 8
-    int* flags = nullptr;            // 4
-    uint64_t commit_cap = 0;
-    uint64_t* c_prio = nullptr;
-    uint64_t* c_prio2 = nullptr;
-    uint64_t* c_order = nullptr;
-    uint32_t* c_idx = nullptr;
-    uint32_t* c_idx2 = nullptr;
-    uint32_t* c_g = nullptr;
-    uint32_t* c_g2 = nullptr;
+    unsigned long long* counters = nullptr;  // rl::kCounters device counters of the current update
+    uint32_t* d_steps = nullptr;     // [0] first neuron step of a captured period, [1] plasticity update
+    uint64_t* scan_part = nullptr;   // tile sums of scan_counts
+    uint64_t scan_part_cap = 0;
+    // commit: requests grouped by (target, dendrite type) with a counting sort
+    uint32_t* c_gcnt = nullptr;      // 2n
+    uint64_t* c_goff = nullptr;      // 2n + 1
+    uint64_t* c_prio = nullptr;      // slots_cap: processing priority per request
+    uint64_t* c_order = nullptr;     // slots_cap: caller's processing order
+    uint64_t* c_sprio = nullptr;     // slots_cap: priorities in group order
+    uint32_t* c_sidx = nullptr;      // slots_cap: request indices in group order
     // cooperative search
     uint32_t* coop_flag = nullptr;   // n
     uint32_t* coop_work = nullptr;   // n
-    int* coop_nwork = nullptr;
-    unsigned long long* coop_cursor = nullptr;  // 10
+    unsigned long long* coop_cursor = nullptr;  // one work cursor per XCD
     uint32_t* coop_overflow = nullptr;  // n
     uint32_t* redo = nullptr;           // n
     uint32_t* coop_node = nullptr;   // blocks * cap
@@ -146,9 +167,12 @@ struct rl_bh_ctx {
     uint32_t coop_blocks = 0;
     // (slot, source, node, level) of descents that need the general per-lane traversal (slots_cap)
     uint4* slow_items = nullptr;
-    unsigned long long* pipe = nullptr;  // [0] slow items
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
+    // captured plasticity period (rl_bh_graph_create)
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t graph_steps = 0;
+    bool capturing = false;
     rl::Neurons neurons;
 };
 
@@ -191,20 +215,35 @@ int export_tree(rl_bh_ctx* c, uint8_t* level, uint8_t* nchild, uint32_t* link, u
                 double* box_hi, double* side, uint32_t* cnt_exc, uint32_t* cnt_inh, uint8_t* has_exc, uint8_t* has_inh,
                 double* pos_exc, double* pos_inh);
 
+// captured plasticity periods (rl_api.hip)
+void destroy_graph(rl_bh_ctx* c);
+
+// scans and fills (rl_scan.hip)
+uint64_t scan_scratch_entries(uint64_t n);
+int scan_counts(rl_bh_ctx* c, const uint32_t* in, uint64_t n, uint64_t* out);  // out[0..n], out[n] = total
+int fill_u32(rl_bh_ctx* c, uint32_t* p, uint32_t value, uint64_t n, const uint64_t* n_dev, uint64_t cap);
+int set_steps(rl_bh_ctx* c, uint32_t neuron_step, uint32_t update);
+
 // search (rl_search.hip, rl_search_coop.hip)
+int ensure_slots(rl_bh_ctx* c, uint64_t want);
+int search_capacity(rl_bh_ctx* c);
+int enqueue_search(rl_bh_ctx* c);
 int find_targets(rl_bh_ctx* c, uint32_t step);
+int read_counters(rl_bh_ctx* c, unsigned long long* h);
 struct SearchArgs;
-int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow);
-int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, uint64_t count);
-int compact_active(rl_bh_ctx* c, uint64_t* nwork);
+int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa);
+int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, const unsigned long long* count);
 
 // commit (rl_commit.hip)
+int enqueue_commit(rl_bh_ctx* c, const uint64_t* h_order);
 int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* created);
 
 // neuron step (rl_step.hip)
 int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const double* u, const double* ca,
                  const double* target_ca, const double* grown, const uint32_t* connected);
 int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input);
+int enqueue_neuron_steps(rl_bh_ctx* c, uint32_t steps);  // steps d_steps[0] + k, k < steps
+int enqueue_elements_commit(rl_bh_ctx* c);
 int neurons_commit(rl_bh_ctx* c, uint32_t* deleted);
 int neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
                 uint32_t* connected);

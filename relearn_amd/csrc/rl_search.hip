@@ -25,7 +25,7 @@
 //               exp2; FP64 sums.  Acceptance decisions within a relative guard band of the threshold
 //               and exact coincidences are re-decided with the FP64 test, so the candidate lists are
 //               those of the FP64 path; a list whose FP32 sum is 0 is redone in FP64.
-#include <hipcub/hipcub.hpp>
+#include <algorithm>
 
 #include "rl_search_eval.h"
 
@@ -204,16 +204,9 @@ __device__ uint32_t list_pick(const SearchArgs& a, const Lane& L, uint32_t root,
     return t;
 }
 
+// The whole search of source src by one lane; stk is the lane's LDS stack column (stride apart).
 template <bool kMixed, bool kFallback>
-__global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
-    extern __shared__ uint32_t lds_stack[];
-    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (t >= a.nwork) {
-        return;
-    }
-    uint32_t* stk = lds_stack + threadIdx.x;
-    const uint32_t stride = blockDim.x;
-    const uint32_t src = a.order[t];
+__device__ __forceinline__ void search_source(const SearchArgs& a, uint32_t src, uint32_t* stk, uint32_t stride) {
     const uint32_t na = a.vac_axon[src];
     const uint64_t slot0 = a.slot_off[src];
     if (a.enabled[src] == 0 || na == 0) {
@@ -241,13 +234,13 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
         }
         a.found[src] = f;
         a.searched[src] = f == na ? na : 1;
-        atomicAdd(&a.counters[2], static_cast<unsigned long long>(f == na ? na : 1));
+        atomicAdd(&a.counters[kCtrSearches], static_cast<unsigned long long>(f == na ? na : 1));
         return;
     }
     // level 1: one list for all axons of the neuron
     const ListSum l1 = list_sum<kMixed, kFallback>(a, L, L.root, stk, stride);
     if (l1.redo) {
-        a.redo[atomicAdd(a.n_redo, 1U)] = src;
+        a.redo[atomicAdd(a.n_redo, 1ULL)] = src;
         return;
     }
     exec += l1.tests_exec;
@@ -255,9 +248,9 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
         a.slot_tests[slot0] = l1.tests_ref;
         a.found[src] = 0;
         a.searched[src] = 1;
-        atomicAdd(&a.counters[0], static_cast<unsigned long long>(exec));
-        atomicAdd(&a.counters[1], static_cast<unsigned long long>(l1.tests_ref));
-        atomicAdd(&a.counters[2], 1ULL);
+        atomicAdd(&a.counters[kCtrTestsExec], static_cast<unsigned long long>(exec));
+        atomicAdd(&a.counters[kCtrTestsRef], static_cast<unsigned long long>(l1.tests_ref));
+        atomicAdd(&a.counters[kCtrSearches], 1ULL);
         return;
     }
     const double next1 = nextafter(l1.sum, l1.sum + 0.00001);
@@ -267,7 +260,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
 #pragma unroll
         for (int k = 0; k < kDrawsPerPass; k++) {
             pv.pick[k] = kNone;
-            pv.r[k] = k < pv.nd ? draw_to_uniform(pick_draw(L.seed, L.local, a.step, j0 + k, 1U), next1) : 0.0;
+            pv.r[k] = k < pv.nd ? draw_to_uniform(pick_draw(L.seed, L.local, L.step, j0 + k, 1U), next1) : 0.0;
         }
         exec += list_pick<kMixed, kFallback>(a, L, L.root, l1, pv, stk, stride);
 #pragma unroll
@@ -297,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
             }
             const ListSum ls = list_sum<kMixed, kFallback>(a, L, node, stk, stride);
             if (ls.redo) {
-                a.redo[atomicAdd(a.n_redo, 1U)] = src;
+                a.redo[atomicAdd(a.n_redo, 1ULL)] = src;
                 return;
             }
             exec += ls.tests_exec;
@@ -313,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
                 pv.pick[k] = kNone;
                 pv.r[k] = 0.0;
             }
-            pv.r[0] = draw_to_uniform(pick_draw(L.seed, L.local, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
+            pv.r[0] = draw_to_uniform(pick_draw(L.seed, L.local, L.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
             exec += list_pick<kMixed, kFallback>(a, L, node, ls, pv, stk, stride);
             node = pv.pick[0];
         }
@@ -328,24 +321,27 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
     }
     a.found[src] = found;
     a.searched[src] = searched;
-    atomicAdd(&a.counters[0], static_cast<unsigned long long>(exec));
-    atomicAdd(&a.counters[1], static_cast<unsigned long long>(ref));
-    atomicAdd(&a.counters[2], static_cast<unsigned long long>(searched));
+    atomicAdd(&a.counters[kCtrTestsExec], static_cast<unsigned long long>(exec));
+    atomicAdd(&a.counters[kCtrTestsRef], static_cast<unsigned long long>(ref));
+    atomicAdd(&a.counters[kCtrSearches], static_cast<unsigned long long>(searched));
+}
+
+// Sources a.order[0 .. *a.nwork), grid-stride, one lane per source.
+template <bool kMixed, bool kFallback>
+__global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint64_t count = *a.nwork;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < count; t += stride) {
+        search_source<kMixed, kFallback>(a, a.order[t], lds_stack + threadIdx.x, blockDim.x);
+    }
 }
 
 // Descents that the cooperative kernels hand over (a level that needs expansion or the reference's
 // fallbacks): one lane continues the search of one slot from (node, level) with the general
 // traversal, as the descent loop of k_search does.
 template <bool kMixed>
-__global__ void __launch_bounds__(kBlock) k_descend_lane(SearchArgs a, const uint4* __restrict__ items, uint64_t count) {
-    extern __shared__ uint32_t lds_stack[];
-    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (t >= count) {
-        return;
-    }
-    uint32_t* stk = lds_stack + threadIdx.x;
-    const uint32_t stride = blockDim.x;
-    const uint4 it = items[t];
+__device__ __forceinline__ void descend_item(const SearchArgs& a, const uint4 it, uint32_t* stk, uint32_t stride) {
     const uint32_t slot = it.x, src = it.y;
     uint32_t node = it.z;
     Lane L;
@@ -378,21 +374,23 @@ __global__ void __launch_bounds__(kBlock) k_descend_lane(SearchArgs a, const uin
             pv.pick[k] = kNone;
             pv.r[k] = 0.0;
         }
-        pv.r[0] = draw_to_uniform(pick_draw(L.seed, L.local, a.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
+        pv.r[0] = draw_to_uniform(pick_draw(L.seed, L.local, L.step, j, level), nextafter(ls.sum, ls.sum + 0.00001));
         exec += list_pick<kMixed, true>(a, L, node, ls, pv, stk, stride);
         node = pv.pick[0];
     }
     a.slot_tests[slot] += tests;
     a.slot_target[slot] = ok ? node : kNone;
-    atomicAdd(&a.counters[0], static_cast<unsigned long long>(exec));
+    atomicAdd(&a.counters[kCtrTestsExec], static_cast<unsigned long long>(exec));
 }
 
-__global__ void k_widen(uint64_t n, const uint32_t* __restrict__ v, uint64_t* __restrict__ out) {
-    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i < n) {
-        out[i] = v[i];
-    } else if (i == n) {
-        out[i] = 0;
+template <bool kMixed>
+__global__ void __launch_bounds__(kBlock) k_descend_lane(SearchArgs a, const uint4* __restrict__ items,
+                                                         const unsigned long long* __restrict__ n_items) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint64_t count = *n_items;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < count; t += stride) {
+        descend_item<kMixed>(a, items[t], lds_stack + threadIdx.x, blockDim.x);
     }
 }
 
@@ -415,91 +413,148 @@ __global__ void k_emit_requests(uint64_t n, const uint32_t* __restrict__ found, 
     }
 }
 
+// Start of a search: results and counters cleared, slots of this update initialised.  When the
+// update has more vacant axons than the slot buffers hold, kCtrOverCap is raised and no source is
+// searched (the update then changes nothing).
+__global__ void k_search_reset(uint64_t n, const uint64_t* __restrict__ slot_total, uint64_t cap,
+                               uint32_t* __restrict__ found, uint32_t* __restrict__ searched,
+                               uint32_t* __restrict__ slot_target, uint32_t* __restrict__ slot_tests,
+                               unsigned long long* __restrict__ counters, unsigned long long* __restrict__ cursor) {
+    const uint64_t total = *slot_total;
+    const bool over = total > cap;
+    const uint64_t slots = over ? 0 : total;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    const uint64_t t0 = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    for (uint64_t i = t0; i < n; i += stride) {
+        found[i] = 0;
+        searched[i] = 0;
+    }
+    for (uint64_t i = t0; i < slots; i += stride) {
+        slot_target[i] = kNone;
+        slot_tests[i] = 0;
+    }
+    if (t0 < kCounters) {
+        if (t0 == kCtrSticky) {
+            if (over) {
+                counters[t0] = 1;
+            }
+        } else {
+            counters[t0] = (t0 == kCtrOverCap && over) ? 1ULL : 0ULL;
+        }
+    }
+    if (t0 < kCursorSlots) {
+        cursor[t0] = 0;
+    }
+}
+
 __global__ void k_active_flags(uint64_t n, const uint32_t* __restrict__ order, const uint8_t* __restrict__ enabled,
-                               const uint32_t* __restrict__ vac, uint32_t* __restrict__ flag) {
+                               const uint32_t* __restrict__ vac, const unsigned long long* __restrict__ counters,
+                               uint32_t* __restrict__ flag) {
     const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (t < n) {
         const uint32_t src = order[t];
-        flag[t] = (enabled[src] != 0 && vac[src] != 0) ? 1U : 0U;
+        flag[t] = (counters[kCtrOverCap] == 0 && enabled[src] != 0 && vac[src] != 0) ? 1U : 0U;
+    }
+}
+
+__global__ void k_scatter_active(uint64_t n, const uint32_t* __restrict__ flag, const uint64_t* __restrict__ off,
+                                 const uint32_t* __restrict__ order, uint32_t* __restrict__ work) {
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t < n && flag[t] != 0) {
+        work[off[t]] = order[t];
+    }
+}
+
+__global__ void k_collect(unsigned long long* __restrict__ counters, const uint64_t* __restrict__ slots,
+                          const uint64_t* __restrict__ requests, const uint64_t* __restrict__ active) {
+    if (threadIdx.x == 0) {
+        counters[kCtrSlots] = *slots;
+        counters[kCtrRequests] = *requests;
+        counters[kCtrActive] = *active;
     }
 }
 
 }  // namespace
 
-int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, uint64_t count) {
+int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, const unsigned long long* count) {
     hipStream_t s = c->stream;
     const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
     if (c->cfg.precision == RL_PREC_FP32_MIXED) {
-        k_descend_lane<true><<<grid_for(count), kBlock, lds, s>>>(sa, items, count);
+        k_descend_lane<true><<<kStreamBlocks / 2, kBlock, lds, s>>>(sa, items, count);
     } else {
-        k_descend_lane<false><<<grid_for(count), kBlock, lds, s>>>(sa, items, count);
+        k_descend_lane<false><<<kStreamBlocks / 2, kBlock, lds, s>>>(sa, items, count);
     }
     RL_HIP(c, hipGetLastError());
     return RL_OK;
 }
 
-// Active sources (enabled, vacant axons > 0) in Morton order into c->coop_work.
-int compact_active(rl_bh_ctx* c, uint64_t* nwork) {
-    const uint64_t n = c->n;
-    hipStream_t s = c->stream;
-    k_active_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->coop_flag);
-    RL_HIP(c, hipGetLastError());
-    size_t tb = 0;
-    RL_HIP(c, hipcub::DeviceSelect::Flagged(nullptr, tb, c->order, c->coop_flag, c->coop_work, c->coop_nwork,
-                                            static_cast<int>(n), s));
-    RL_TRY(ensure_tmp(c, tb));
-    RL_HIP(c, hipcub::DeviceSelect::Flagged(c->tmp, tb, c->order, c->coop_flag, c->coop_work, c->coop_nwork,
-                                            static_cast<int>(n), s));
-    int nw = 0;
-    RL_HIP(c, hipMemcpyAsync(&nw, c->coop_nwork, sizeof(int), hipMemcpyDeviceToHost, s));
-    RL_HIP(c, hipStreamSynchronize(s));
-    *nwork = static_cast<uint64_t>(nw);
-    return RL_OK;
-}
-
-int find_targets(rl_bh_ctx* c, uint32_t step) {
-    if (!c->prepared) {
-        return fail(c, RL_ESTATE, "find_target_neurons called before prepare_update_connectivity");
-    }
-    const uint64_t n = c->n;
-    hipStream_t s = c->stream;
-    c->nreq = 0;
-    c->searched_valid = false;
-    if (n == 0) {
-        c->searched_valid = true;
+// Slot and request buffers for `want` vacant axons.  Grows with headroom, so that updates whose
+// vacancies vary do not reallocate every time; a captured period refers to the old buffers and is
+// released.
+int ensure_slots(rl_bh_ctx* c, uint64_t want) {
+    if (want <= c->slots_cap && c->slot_target != nullptr) {
         return RL_OK;
     }
-    // slot offsets: exclusive prefix sum of the vacant axons
-    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->vac[0], c->wide);
-    RL_HIP(c, hipGetLastError());
-    size_t tb = 0;
-    RL_HIP(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, c->wide, c->slot_off, static_cast<int>(n) + 1, s));
-    RL_TRY(ensure_tmp(c, tb));
-    RL_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, c->wide, c->slot_off, static_cast<int>(n) + 1, s));
-    uint64_t slots = 0;
-    RL_HIP(c, hipMemcpyAsync(&slots, c->slot_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    RL_HIP(c, hipStreamSynchronize(s));
-    if (slots >= (1ULL << 32)) {
+    if (c->capturing) {
+        return fail(c, RL_ESTATE, "slot buffers cannot grow during graph capture");
+    }
+    constexpr uint64_t kMaxSlots = 0xFFFFFFFFULL;
+    if (want > kMaxSlots) {
         return fail(c, RL_ELIMIT, "more than 2^32 - 1 vacant axons in one update");
     }
-    if (slots > c->slots_cap) {
-        RL_TRY(dalloc(c, &c->slot_target, slots));
-        RL_TRY(dalloc(c, &c->slot_tests, slots));
-        RL_TRY(dalloc(c, &c->slow_items, slots));
-        c->slots_cap = slots;
+    const uint64_t cap = std::min<uint64_t>(kMaxSlots, want + want / 8 + 1024);
+    destroy_graph(c);
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    RL_TRY(dalloc(c, &c->slot_target, cap));
+    RL_TRY(dalloc(c, &c->slot_tests, cap));
+    RL_TRY(dalloc(c, &c->slow_items, cap));
+    RL_TRY(dalloc(c, &c->req_target, cap));
+    RL_TRY(dalloc(c, &c->req_source, cap));
+    RL_TRY(dalloc(c, &c->req_signal, cap));
+    RL_TRY(dalloc(c, &c->req_j, cap));
+    RL_TRY(dalloc(c, &c->req_ok, cap));
+    RL_TRY(dalloc(c, &c->c_prio, cap));
+    RL_TRY(dalloc(c, &c->c_order, cap));
+    RL_TRY(dalloc(c, &c->c_sprio, cap));
+    RL_TRY(dalloc(c, &c->c_sidx, cap));
+    c->slots_cap = cap;
+    return RL_OK;
+}
+
+// Sizes the slot buffers for the resident vacancies (one host read of their sum).
+int search_capacity(rl_bh_ctx* c) {
+    RL_TRY(scan_counts(c, c->vac[0], c->n, c->slot_off));
+    uint64_t total = 0;
+    RL_HIP(c, hipMemcpyAsync(&total, c->slot_off + c->n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    return ensure_slots(c, total);
+}
+
+// The search of one update on the context's stream, driven by counts in device memory: slot
+// offsets, the active sources in Morton order, the search kernels, the dense requests.  No host
+// synchronisation, allocation or memset, so it can be captured into a graph.  The update index is
+// read from d_steps[1].
+int enqueue_search(rl_bh_ctx* c) {
+    const uint64_t n = c->n;
+    hipStream_t s = c->stream;
+    if (n == 0) {
+        return RL_OK;
     }
-    c->slots = slots;
-    unsigned long long* counters = c->counters;
-    RL_HIP(c, hipMemsetAsync(counters, 0, 4 * sizeof(unsigned long long), s));
-    if (slots > 0) {
-        RL_HIP(c, hipMemsetAsync(c->slot_target, 0xFF, slots * sizeof(uint32_t), s));
-        RL_HIP(c, hipMemsetAsync(c->slot_tests, 0, slots * sizeof(uint32_t), s));
-    }
+    RL_TRY(scan_counts(c, c->vac[0], n, c->slot_off));
+    k_search_reset<<<kStreamBlocks, kBlock, 0, s>>>(n, c->slot_off + n, c->slots_cap, c->found, c->searched,
+                                                    c->slot_target, c->slot_tests, c->counters, c->coop_cursor);
+    RL_HIP(c, hipGetLastError());
+    k_active_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->counters, c->coop_flag);
+    RL_HIP(c, hipGetLastError());
+    RL_TRY(scan_counts(c, c->coop_flag, n, c->wide));
+    k_scatter_active<<<grid_for(n), kBlock, 0, s>>>(n, c->coop_flag, c->wide, c->order, c->coop_work);
+    RL_HIP(c, hipGetLastError());
 
     SearchArgs a{};
     a.n = n;
     a.m = c->tree.m;
-    a.order = c->order;
+    a.order = c->coop_work;
+    a.nwork = c->wide + n;
     a.pos = c->pos;
     a.qpos = c->qpos;
     a.signal = c->signal;
@@ -524,93 +579,95 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     a.net_invq = c->net_invq;
     a.guard = 2.0e-5f;
     a.seed = c->cfg.seed;
-    a.step = step;
+    a.step = c->d_steps + 1;
     a.slot_target = c->slot_target;
     a.slot_tests = c->slot_tests;
     a.found = c->found;
     a.searched = c->searched;
-    a.counters = counters;
+    a.counters = c->counters;
+    a.redo = c->redo;
+    a.n_redo = c->counters + kCtrRedo;
     const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
-    RL_HIP(c, hipMemsetAsync(c->found, 0, n * sizeof(uint32_t), s));
-    RL_HIP(c, hipMemsetAsync(c->searched, 0, n * sizeof(uint32_t), s));
-    RL_HIP(c, hipEventRecord(c->ev0, s));
     const bool exact = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0;
     const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
-    uint64_t nlane = 0;
-    a.redo = c->redo;
-    a.n_redo = reinterpret_cast<unsigned int*>(c->coop_cursor + 9);
-    RL_HIP(c, hipMemsetAsync(c->coop_cursor, 0, 10 * sizeof(unsigned long long), s));
+    const bool timed = !c->capturing;
     if (exact) {
-        // per-lane kernel over the active sources in Morton order, then the fallback launch
-        RL_TRY(compact_active(c, &nlane));
-        a.order = c->coop_work;
-        a.nwork = nlane;
-        c->last_kernel_ms = 0.0;
-        if (nlane > 0) {
-            RL_HIP(c, hipEventRecord(c->ev2, s));
-            if (mixed) {
-                k_search<true, false><<<grid_for(nlane), kBlock, lds, s>>>(a);
-            } else {
-                k_search<false, false><<<grid_for(nlane), kBlock, lds, s>>>(a);
-            }
-            RL_HIP(c, hipGetLastError());
-            RL_HIP(c, hipEventRecord(c->ev3, s));
-        }
-        unsigned int nredo = 0;
-        RL_HIP(c, hipMemcpyAsync(&nredo, a.n_redo, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-        RL_HIP(c, hipStreamSynchronize(s));
-        if (nlane > 0) {
-            float kms = 0.0f;
-            RL_HIP(c, hipEventElapsedTime(&kms, c->ev2, c->ev3));
-            c->last_kernel_ms = kms;
-        }
-        nlane = nredo;
-        a.order = c->redo;
-    } else {
-        RL_TRY(launch_search_coop(c, a, &nlane));
-        a.order = c->coop_overflow;
-    }
-    a.nwork = nlane;
-    if (nlane > 0) {
+        // one lane per active source in Morton order, then the fallback pass over the redo list
+        if (timed) RL_HIP(c, hipEventRecord(c->ev2, s));
         if (mixed) {
-            k_search<true, true><<<grid_for(nlane), kBlock, lds, s>>>(a);
+            k_search<true, false><<<grid_for(n), kBlock, lds, s>>>(a);
         } else {
-            k_search<false, true><<<grid_for(nlane), kBlock, lds, s>>>(a);
+            k_search<false, false><<<grid_for(n), kBlock, lds, s>>>(a);
         }
         RL_HIP(c, hipGetLastError());
+        if (timed) RL_HIP(c, hipEventRecord(c->ev3, s));
+        a.order = c->redo;
+        a.nwork = reinterpret_cast<const uint64_t*>(c->counters + kCtrRedo);
+    } else {
+        RL_TRY(launch_search_coop(c, a));
+        a.order = c->coop_overflow;
+        a.nwork = reinterpret_cast<const uint64_t*>(c->counters + kCtrOverflow);
     }
-    c->last_lane_sources = nlane;
-    RL_HIP(c, hipEventRecord(c->ev1, s));
-    // dense requests in (source, j) order
-    k_widen<<<grid_for(n + 1), kBlock, 0, s>>>(n, c->found, c->wide);
+    if (mixed) {
+        k_search<true, true><<<kStreamBlocks / 8, kBlock, lds, s>>>(a);
+    } else {
+        k_search<false, true><<<kStreamBlocks / 8, kBlock, lds, s>>>(a);
+    }
     RL_HIP(c, hipGetLastError());
-    RL_HIP(c, hipcub::DeviceScan::ExclusiveSum(c->tmp, tb, c->wide, c->req_off, static_cast<int>(n) + 1, s));
-    uint64_t nreq = 0;
-    unsigned long long hc[4] = { 0, 0, 0, 0 };
-    RL_HIP(c, hipMemcpyAsync(&nreq, c->req_off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    RL_HIP(c, hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
-    RL_HIP(c, hipStreamSynchronize(s));
-    float ms = 0.0f;
+    // dense requests in (source, j) order
+    RL_TRY(scan_counts(c, c->found, n, c->req_off));
+    k_emit_requests<<<grid_for(n), kBlock, 0, s>>>(n, c->found, c->req_off, c->slot_off, c->slot_target, c->signal,
+                                                    c->req_target, c->req_source, c->req_signal, c->req_j);
+    RL_HIP(c, hipGetLastError());
+    k_collect<<<1, 64, 0, s>>>(c->counters, c->slot_off + n, c->req_off + n, c->wide + n);
+    RL_HIP(c, hipGetLastError());
+    return RL_OK;
+}
+
+int read_counters(rl_bh_ctx* c, unsigned long long* h) {
+    RL_HIP(c, hipMemcpyAsync(h, c->counters, kCounters * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    if (h[kCtrSticky] != 0) {
+        RL_TRY(fill_u32(c, reinterpret_cast<uint32_t*>(c->counters + kCtrSticky), 0U, 2, nullptr, 0));
+    }
+    return RL_OK;
+}
+
+int find_targets(rl_bh_ctx* c, uint32_t step) {
+    if (!c->prepared) {
+        return fail(c, RL_ESTATE, "find_target_neurons called before prepare_update_connectivity");
+    }
+    const uint64_t n = c->n;
+    hipStream_t s = c->stream;
+    c->nreq = 0;
+    c->slots = 0;
+    c->searched_valid = false;
+    if (n == 0) {
+        c->searched_valid = true;
+        return RL_OK;
+    }
+    RL_TRY(search_capacity(c));
+    RL_TRY(set_steps(c, 0, step));
+    RL_HIP(c, hipEventRecord(c->ev0, s));
+    RL_TRY(enqueue_search(c));
+    RL_HIP(c, hipEventRecord(c->ev1, s));
+    unsigned long long h[kCounters];
+    RL_TRY(read_counters(c, h));
+    if (h[kCtrOverCap] != 0) {
+        return fail(c, RL_ELIMIT, "vacant axons exceed the slot buffers");
+    }
+    float ms = 0.0f, kms = 0.0f;
     RL_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    RL_HIP(c, hipEventElapsedTime(&kms, c->ev2, c->ev3));
     c->last_search_ms = ms;
-    if (nreq > c->req_cap) {
-        RL_TRY(dalloc(c, &c->req_target, nreq));
-        RL_TRY(dalloc(c, &c->req_source, nreq));
-        RL_TRY(dalloc(c, &c->req_signal, nreq));
-        RL_TRY(dalloc(c, &c->req_j, nreq));
-        RL_TRY(dalloc(c, &c->req_ok, nreq));
-        c->req_cap = nreq;
-    }
-    c->nreq = nreq;
-    if (nreq > 0) {
-        k_emit_requests<<<grid_for(n), kBlock, 0, s>>>(n, c->found, c->req_off, c->slot_off, c->slot_target, c->signal,
-                                                        c->req_target, c->req_source, c->req_signal, c->req_j);
-        RL_HIP(c, hipGetLastError());
-    }
-    RL_HIP(c, hipStreamSynchronize(s));
-    c->last_tests = hc[0];
-    c->last_tests_ref = hc[1];
-    c->last_searches = hc[2];
+    c->last_kernel_ms = h[kCtrActive] > 0 ? kms : 0.0;
+    c->slots = h[kCtrSlots];
+    c->nreq = h[kCtrRequests];
+    c->last_tests = h[kCtrTestsExec];
+    c->last_tests_ref = h[kCtrTestsRef];
+    c->last_searches = h[kCtrSearches];
+    c->last_slow_descents = h[kCtrSlow];
+    c->last_lane_sources = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0 ? h[kCtrRedo] : h[kCtrOverflow];
     c->searched_valid = true;
     return RL_OK;
 }

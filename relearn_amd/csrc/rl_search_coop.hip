@@ -29,7 +29,7 @@
 // contiguous segment per XCD so that an XCD's L2 caches one region of the tree; waves take sources
 // from their XCD's segment, then from the others.  A source whose slot table would overflow is handed
 // to the per-lane kernel.
-#include <hipcub/hipcub.hpp>
+#include <algorithm>
 
 #include "rl_search_eval.h"
 
@@ -40,21 +40,20 @@ constexpr int kWave = 64;
 constexpr uint32_t kQueueCap = 1024;   // LDS ring of queued / not yet flushed slots (power of two)
 constexpr uint32_t kStageCap = 256;    // LDS ring of slot records not yet flushed (power of two)
 constexpr uint32_t kFlushAt = 192;     // flush when this many processed slots are staged
-constexpr int kXcds = 8;
+constexpr int kXcds = kCursorSlots;
 constexpr uint32_t kSlotNone = 0x0FFFFFFFU;
 
 struct CoopArgs {
     SearchArgs s;
     const uint32_t* work;
-    uint64_t seg_begin[kXcds];
-    uint64_t seg_end[kXcds];
+    const uint64_t* nwork;        // device count of work; XCD segment x is [nwork x / 8, nwork (x + 1) / 8)
     unsigned long long* cursor;   // kXcds counters
     uint32_t* t_my;               // per wave: cap slots, packed child range of the slot's node
     uint32_t* t_fexp;             // first expanded child slot | expanded-child count << 28
     double* t_mass;
     uint32_t cap;
     uint32_t* overflow;
-    unsigned int* n_overflow;
+    unsigned long long* n_overflow;
     uint4* slow;                  // (slot, source, node, level) of descents for the per-lane kernel
     unsigned long long* n_slow;
 };
@@ -589,7 +588,7 @@ __device__ __forceinline__ int descend_fast(const SearchArgs& s, const Lane& L, 
             }
         }
         const bool pick = state == kDrawActive;
-        const double r = pick ? draw_to_uniform(pick_draw(L.seed, L.local, s.step, jd, level), nextafter(gsum, gsum + 0.00001))
+        const double r = pick ? draw_to_uniform(pick_draw(L.seed, L.local, L.step, jd, level), nextafter(gsum, gsum + 0.00001))
                               : 0.0;
         const double pa = st == kAccept ? p : 0.0;
         const double cum = group_suffix8(pa);
@@ -627,13 +626,15 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
     int seg = static_cast<int>(blockIdx.x % kXcds);
     int tried = 0;
     unsigned long long acc_exec = 0;
+    const uint64_t nw = *a.nwork;
+    const uint32_t step = *s.step;
     while (true) {
         uint64_t w = 0;
         if (ln == 0) {
-            w = a.seg_begin[seg] + atomicAdd(&a.cursor[seg], 1ULL);
+            w = nw * static_cast<uint64_t>(seg) / kXcds + atomicAdd(&a.cursor[seg], 1ULL);
         }
         w = __shfl(w, 0, kWave);
-        if (w >= a.seg_end[seg]) {
+        if (w >= nw * static_cast<uint64_t>(seg + 1) / kXcds) {
             if (++tried >= kXcds) {
                 break;
             }
@@ -651,6 +652,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
         const uint32_t net = __builtin_amdgcn_readfirstlane(s.net_of[src]);
         L.root = net;
         L.local = src - __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s.net_off[net]));
+        L.step = step;
         L.seed = (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s.net_seed[net] >> 32U)))) << 32U) |
                  static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(s.net_seed[net] & 0xFFFFFFFFULL)));
         L.k2q = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s.net_k2q[net])));
@@ -669,7 +671,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
         acc_exec += l1.tests;
         if (l1.overflow) {
             if (ln == 0) {
-                const unsigned int k = atomicAdd(a.n_overflow, 1U);
+                const unsigned long long k = atomicAdd(a.n_overflow, 1ULL);
                 a.overflow[k] = src;
                 s.found[src] = kOverflowMark;
             }
@@ -682,7 +684,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
             uint32_t nav_tests = 0;
             for (uint32_t j0 = 0; j0 < na; j0 += 8) {
                 const bool gact = j0 + q < na;
-                const double r = gact ? draw_to_uniform(pick_draw(L.seed, L.local, s.step, j0 + q, 1U), next1) : 0.0;
+                const double r = gact ? draw_to_uniform(pick_draw(L.seed, L.local, L.step, j0 + q, 1U), next1) : 0.0;
                 uint32_t node = navigate<kMixed, kMu0>(a, L, r, gact, base, &nav_tests);
                 uint32_t level = 2, tests = l1.tests;
                 int state = !gact ? kDrawIdle : (node == kNone ? kDrawFailed : kDrawActive);
@@ -709,7 +711,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(kMix
         __syncthreads();
     }
     if (ln == 0) {
-        atomicAdd(&s.counters[0], acc_exec);
+        atomicAdd(&s.counters[kCtrTestsExec], acc_exec);
     }
 }
 
@@ -723,10 +725,11 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // Per source, in axon order: a search that fails ends the source's searches (find_target_neurons,
 // BarnesHutBase.h:283-289).  Counts the searches, the successful ones and the reference's tests.
-__global__ void __launch_bounds__(256) k_finalize(SearchArgs s, const uint32_t* __restrict__ work, uint64_t nwork) {
+__global__ void __launch_bounds__(256) k_finalize(SearchArgs s, const uint32_t* __restrict__ work,
+                                                  const uint64_t* __restrict__ n_work) {
     const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     unsigned long long ref = 0, searched_n = 0;
-    if (t < nwork) {
+    if (t < *n_work) {
         const uint32_t src = work[t];
         if (s.found[src] != kOverflowMark) {
             const uint32_t na = s.vac_axon[src];
@@ -754,69 +757,49 @@ __global__ void __launch_bounds__(256) k_finalize(SearchArgs s, const uint32_t* 
     ref = wave_sum(ref);
     searched_n = wave_sum(searched_n);
     if ((threadIdx.x & (kWave - 1)) == 0) {
-        atomicAdd(&s.counters[1], ref);
-        atomicAdd(&s.counters[2], searched_n);
+        atomicAdd(&s.counters[kCtrTestsRef], ref);
+        atomicAdd(&s.counters[kCtrSearches], searched_n);
     }
 }
 
 }  // namespace
 
-// Cooperative search over the active sources: the search kernel, the per-lane general descents it
-// hands over, then the per-source finalisation.  Sources that overflow the slot table (or whose root
-// list sums to 0) are returned in c->coop_overflow (count in *n_overflow) for the per-lane kernel.
-int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa, uint64_t* n_overflow) {
+// Cooperative search over the active sources (c->coop_work, count in sa.nwork): the search kernel,
+// the per-lane general descents it hands over, then the per-source finalisation.  Sources that
+// overflow the slot table (or whose root list sums to 0) go to c->coop_overflow, counted in
+// counters[kCtrOverflow], for the per-lane kernel.  All counts stay on the device.
+int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa) {
     hipStream_t s = c->stream;
-    uint64_t nw = 0;
-    RL_TRY(compact_active(c, &nw));
-    RL_HIP(c, hipMemsetAsync(c->coop_cursor, 0, 9 * sizeof(unsigned long long), s));
-    RL_HIP(c, hipMemsetAsync(c->pipe, 0, 8 * sizeof(unsigned long long), s));
     CoopArgs a{};
     a.s = sa;
     a.work = c->coop_work;
-    for (int x = 0; x < kXcds; x++) {
-        a.seg_begin[x] = nw * x / kXcds;
-        a.seg_end[x] = nw * (x + 1) / kXcds;
-    }
+    a.nwork = sa.nwork;
     a.cursor = c->coop_cursor;
     a.t_my = c->coop_node;
     a.t_fexp = c->coop_fexp;
     a.t_mass = c->coop_mass;
     a.cap = c->coop_cap;
     a.overflow = c->coop_overflow;
-    a.n_overflow = reinterpret_cast<unsigned int*>(c->coop_cursor + kXcds);
+    a.n_overflow = c->counters + kCtrOverflow;
     a.slow = c->slow_items;
-    a.n_slow = c->pipe;
+    a.n_slow = c->counters + kCtrSlow;
     const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
     const bool mu0 = c->cfg.mu == 0.0;
-    c->last_kernel_ms = 0.0;
-    if (nw > 0) {
-        RL_HIP(c, hipEventRecord(c->ev2, s));
-        if (mixed && mu0) {
-            k_level1<true, true><<<c->coop_blocks, kWave, 0, s>>>(a);
-        } else if (mixed) {
-            k_level1<true, false><<<c->coop_blocks, kWave, 0, s>>>(a);
-        } else {
-            k_level1<false, false><<<c->coop_blocks, kWave, 0, s>>>(a);
-        }
-        RL_HIP(c, hipGetLastError());
-        RL_HIP(c, hipEventRecord(c->ev3, s));
-        unsigned long long nslow = 0;
-        RL_HIP(c, hipMemcpyAsync(&nslow, c->pipe, sizeof(nslow), hipMemcpyDeviceToHost, s));
-        RL_HIP(c, hipStreamSynchronize(s));
-        float kms = 0.0f;
-        RL_HIP(c, hipEventElapsedTime(&kms, c->ev2, c->ev3));
-        c->last_kernel_ms = kms;
-        c->last_slow_descents = nslow;
-        if (nslow > 0) {
-            RL_TRY(launch_descend_lane(c, sa, c->slow_items, nslow));
-        }
-        k_finalize<<<static_cast<unsigned>((nw + 255) / 256), 256, 0, s>>>(sa, c->coop_work, nw);
-        RL_HIP(c, hipGetLastError());
+    const bool timed = !c->capturing;
+    if (timed) RL_HIP(c, hipEventRecord(c->ev2, s));
+    if (mixed && mu0) {
+        k_level1<true, true><<<c->coop_blocks, kWave, 0, s>>>(a);
+    } else if (mixed) {
+        k_level1<true, false><<<c->coop_blocks, kWave, 0, s>>>(a);
+    } else {
+        k_level1<false, false><<<c->coop_blocks, kWave, 0, s>>>(a);
     }
-    unsigned int nov = 0;
-    RL_HIP(c, hipMemcpyAsync(&nov, a.n_overflow, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-    RL_HIP(c, hipStreamSynchronize(s));
-    *n_overflow = nov;
+    RL_HIP(c, hipGetLastError());
+    if (timed) RL_HIP(c, hipEventRecord(c->ev3, s));
+    RL_TRY(launch_descend_lane(c, sa, c->slow_items, c->counters + kCtrSlow));
+    const uint64_t n = std::max<uint64_t>(c->n, 1);
+    k_finalize<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(sa, c->coop_work, sa.nwork);
+    RL_HIP(c, hipGetLastError());
     return RL_OK;
 }
 

@@ -10,7 +10,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFU;
 
 struct SearchArgs {
     uint64_t n;
-    uint64_t nwork;      // sources in `order` processed by the per-lane kernel
+    const uint64_t* nwork;  // device count of the sources in `order` for the per-lane kernel
     uint64_t m;          // tree nodes
     const uint32_t* order;
     const double* pos;
@@ -37,14 +37,14 @@ struct SearchArgs {
     const float* net_k2q;
     const float* net_invq;
     uint64_t seed;
-    uint32_t step;
+    const uint32_t* step;  // device: plasticity update index (the draw-stream address)
     uint32_t* slot_target;
     uint32_t* slot_tests;
     uint32_t* found;
     uint32_t* searched;
     unsigned long long* counters;  // [0] executed tests, [1] reference tests, [2] searches
     uint32_t* redo;                // sources whose Gaussian sum underflowed (fallback launch)
-    unsigned int* n_redo;
+    unsigned long long* n_redo;
 };
 
 // Per-source state; the record pointers are resolved once for the source's signal type so that the
@@ -56,6 +56,7 @@ struct Lane {
     uint32_t local;      // neuron id inside its network (the draw-stream address)
     uint32_t root;       // root node of the network
     uint64_t seed;       // draw-stream seed of the network
+    uint32_t step;       // plasticity update index
     float k2q, inv_q;    // fixed-point constants of the network
     int sig;
     const uint2* meta;
@@ -78,6 +79,7 @@ __device__ __forceinline__ void lane_source(const SearchArgs& a, Lane& L, uint32
     L.root = g;
     L.local = src - static_cast<uint32_t>(a.net_off[g]);
     L.seed = a.net_seed[g];
+    L.step = *a.step;
     L.k2q = a.net_k2q[g];
     L.inv_q = a.net_invq[g];
     L.px = a.pos[src];

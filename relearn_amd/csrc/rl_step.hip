@@ -54,6 +54,7 @@ struct StepArgs {
     const uint64_t* net_off;     // first neuron of each network
     const uint64_t* net_seed;    // draw seed of each network
     const double* net_nu;        // growth rates per network (3 each) or null: p.nu for all
+    const uint32_t* step_base;   // device step base of a captured period (step = *step_base + step) or null
     uint32_t step;
     double* x;
     double* u;
@@ -73,8 +74,9 @@ __global__ void k_step(StepArgs a) {
     }
     const rl_step_params& p = a.p;
     const uint32_t g = a.net_of[i];
+    const uint32_t step = a.step_base != nullptr ? *a.step_base + a.step : a.step;
     const double input = p.input_mode == 0 ? a.input[i]
-                                           : normal_input(a.net_seed[g], static_cast<uint32_t>(i - a.net_off[g]), a.step,
+                                           : normal_input(a.net_seed[g], static_cast<uint32_t>(i - a.net_off[g]), step,
                                                           p.input_mean, p.input_stddev);
     const double scale = 1.0 / static_cast<double>(p.h);
     const double ab = p.a * p.b;
@@ -161,6 +163,7 @@ __global__ void k_elements_update(uint64_t n, double* __restrict__ grown, double
 }  // namespace
 
 void neurons_free(rl_bh_ctx* c) {
+    destroy_graph(c);
     Neurons& s = c->neurons;
     dfree(&s.x);
     dfree(&s.u);
@@ -180,6 +183,7 @@ int neurons_set_network_growth(rl_bh_ctx* c, const double* nu) {
     if (!s.ready) {
         return fail(c, RL_ESTATE, "rl_neurons_set_network_growth before rl_neurons_init");
     }
+    destroy_graph(c);  // a captured period holds the old growth-rate pointer
     if (nu == nullptr) {
         dfree(&s.net_nu);
         return RL_OK;
@@ -237,6 +241,32 @@ int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const d
     return RL_OK;
 }
 
+namespace {
+
+StepArgs step_args(rl_bh_ctx* c) {
+    Neurons& s = c->neurons;
+    StepArgs a{};
+    a.n = c->n;
+    a.p = s.p;
+    a.net_of = c->net_of;
+    a.net_off = c->net_off;
+    a.net_seed = c->net_seed;
+    a.net_nu = s.net_nu;
+    a.step_base = nullptr;
+    a.step = 0;
+    a.x = s.x;
+    a.u = s.u;
+    a.ca = s.ca;
+    a.input = s.input;
+    a.fired = s.fired;
+    a.target_ca = s.target_ca;
+    a.delta = s.delta;
+    a.enabled = c->enabled;
+    return a;
+}
+
+}  // namespace
+
 int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
     Neurons& s = c->neurons;
     if (!s.ready) {
@@ -252,28 +282,33 @@ int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
     if (s.p.input_mode == 0) {
         RL_HIP(c, hipMemcpyAsync(s.input, input, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
-    StepArgs a{};
-    a.n = n;
-    a.p = s.p;
-    a.net_of = c->net_of;
-    a.net_off = c->net_off;
-    a.net_seed = c->net_seed;
-    a.net_nu = s.net_nu;
+    StepArgs a = step_args(c);
     a.step = step;
-    a.x = s.x;
-    a.u = s.u;
-    a.ca = s.ca;
-    a.input = s.input;
-    a.fired = s.fired;
-    a.target_ca = s.target_ca;
-    a.delta = s.delta;
-    a.enabled = c->enabled;
     k_step<<<grid_for(n), kBlock, 0, c->stream>>>(a);
     RL_HIP(c, hipGetLastError());
     return RL_OK;
 }
 
-int neurons_commit(rl_bh_ctx* c, uint32_t* deleted) {
+// `steps` neuron steps d_steps[0] + k on the context's stream (counter-based input only).
+int enqueue_neuron_steps(rl_bh_ctx* c, uint32_t steps) {
+    Neurons& s = c->neurons;
+    if (!s.ready || s.p.input_mode == 0) {
+        return fail(c, RL_ESTATE, "captured neuron steps need rl_neurons_init with counter-based input");
+    }
+    if (c->n == 0) {
+        return RL_OK;
+    }
+    StepArgs a = step_args(c);
+    a.step_base = c->d_steps;
+    for (uint32_t k = 0; k < steps; k++) {
+        a.step = k;
+        k_step<<<grid_for(c->n), kBlock, 0, c->stream>>>(a);
+        RL_HIP(c, hipGetLastError());
+    }
+    return RL_OK;
+}
+
+int enqueue_elements_commit(rl_bh_ctx* c) {
     Neurons& s = c->neurons;
     if (!s.ready) {
         return fail(c, RL_ESTATE, "rl_neurons_commit_elements called before rl_neurons_init");
@@ -285,8 +320,14 @@ int neurons_commit(rl_bh_ctx* c, uint32_t* deleted) {
     k_elements_update<<<grid_for(3 * n), kBlock, 0, c->stream>>>(n, s.grown, s.delta, c->conn[0], c->conn[1], c->conn[2],
                                                                   c->vac[0], c->vac[1], c->vac[2], s.deleted);
     RL_HIP(c, hipGetLastError());
-    if (deleted != nullptr) {
-        RL_HIP(c, hipMemcpyAsync(deleted, s.deleted, 3 * n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    return RL_OK;
+}
+
+int neurons_commit(rl_bh_ctx* c, uint32_t* deleted) {
+    RL_TRY(enqueue_elements_commit(c));
+    if (deleted != nullptr && c->n > 0) {
+        RL_HIP(c, hipMemcpyAsync(deleted, c->neurons.deleted, 3 * c->n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
     }
     RL_HIP(c, hipStreamSynchronize(c->stream));
     return RL_OK;
