@@ -27,6 +27,11 @@ METRIC = "target searches/s @16M neurons, FP32-mixed vs FP64; ms per plasticity 
 HBM_PEAK_GBS = 8000.0
 B_NODE = {"fp64": 40, "fp32_mixed": 24}   # bytes read per acceptance test (record + count/link)
 B_IO = 32                                  # source position + output ids per search
+# Simulation workloads: without synaptic input, Normal(5, 2) input drives the Izhikevich neurons to
+# about 0.0117 spikes per step, so calcium settles near 0.117 and never reaches the reference's
+# axon growth threshold 0.4; these thresholds and target let elements grow and connect.
+SIM_ETA = (0.02, 0.01, 0.0)
+SIM_TARGET_CA = 0.2
 
 
 def log(*a):
@@ -70,10 +75,22 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     tests_ref = sum(r[0]["node_tests_ref"] for r in recs)
     kernel_ms = [r[0]["ms_search"] for r in recs]
     dominant_ms = [r[0]["ms_search_kernel"] for r in recs]
+    # the same update captured as one HIP graph and replayed (vacancies restored before each replay,
+    # outside the timed replay)
+    slots = int(net.vac_axon.astype(np.uint64).sum())
+    g.prepare_update_connectivity_device(*d_vac)
+    g.graph_create(0, slots)
+    for w in range(warmup):
+        g.prepare_update_connectivity_device(*d_vac)
+        g.graph_launch(0, 1000 + w)
+    graph_ms = []
+    for k in range(steps):
+        g.prepare_update_connectivity_device(*d_vac)
+        graph_ms.append(g.graph_launch(0, k)["ms_total"])
+    g.graph_destroy()
     # pick results of step 0 for the precision comparison
     g.prepare_update_connectivity_device(*d_vac)
     g.find_target_neurons(0)
-    slots = int(net.vac_axon.astype(np.uint64).sum())
     targets, _, counts = g.get_search_slots(slots)
     created0 = g.commit(0)
     g.close()
@@ -91,6 +108,7 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
         "node_tests_per_step": tests / steps,
         "node_tests_ref_per_step": tests_ref / steps,
         "phase_s": [float(np.mean([r[2][i] for r in recs])) for i in range(3)],
+        "graph_ms_per_update": float(np.mean(graph_ms)),
         "requests_per_step": float(np.mean([r[0]["requests"] for r in recs])),
         "created_per_step": float(np.mean([r[1] for r in recs])),
         "targets": targets,
@@ -142,10 +160,20 @@ def run_ensemble(args):
             g.commit(k)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        # the same update of all networks captured as one graph
+        g.prepare_update_connectivity_device(*d_vac)
+        g.graph_create(0, int(ens.vac_axon.astype(np.uint64).sum()))
+        graph_ms = []
+        for k in range(args.warmup + args.steps):
+            g.prepare_update_connectivity_device(*d_vac)
+            ms = g.graph_launch(0, k)["ms_total"]
+            if k >= args.warmup:
+                graph_ms.append(ms)
         g.close()
-        out[name] = {"build_s": build_s, "ms_per_update": 1e3 * wall / args.steps, "searches_per_s": searches / wall}
+        out[name] = {"build_s": build_s, "ms_per_update": 1e3 * wall / args.steps, "searches_per_s": searches / wall,
+                     "graph_ms_per_update": float(np.mean(graph_ms))}
         log(f"ensemble {name}: {1e3 * wall / args.steps:.1f} ms per update of all networks, "
-            f"{searches / wall:.4g} searches/s, forest build {build_s:.2f} s")
+            f"{searches / wall:.4g} searches/s, graph replay {np.mean(graph_ms):.1f} ms, forest build {build_s:.2f} s")
     # the same networks, one context each, updated one after another (unbatched)
     if not args.no_unbatched:
         ctxs = []
@@ -177,6 +205,99 @@ def run_ensemble(args):
     out["value"] = out["fp32_mixed"]["searches_per_s"]
     out["ms_per_step"] = out["fp32_mixed"]["ms_per_update"]
     out["dtype"] = "f32-mixed"
+    out["vs_baseline"] = None
+    out["scaling"] = "weak"
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_sim(args):
+    """Simulation periods replayed as HIP graphs: `--period` neuron steps (Izhikevich, calcium, growth
+    curves, counter-based Normal(5, 2) input), the element commit and the connectivity update, one
+    graph replay per period.  Runs FP32-mixed and FP64 with the same seed and reports the mean
+    calcium and synapses-per-neuron trajectories of both."""
+    import torch
+
+    from relearn_amd import RL_PREC_FP32_MIXED, RL_PREC_FP64, BarnesHutHIP, BHError, StepParams
+    from relearn_amd.workloads import parameter_sweep, uniform_cube
+
+    t0 = time.perf_counter()
+    if args.workload == "sim-ensemble":
+        ens = parameter_sweep(args.networks, seed=args.seed)
+        n, G = ens.n, ens.num_networks
+        lo, hi = ens.boxes[:, :3].min(0), ens.boxes[:, 3:].max(0)
+        desc = (f"{G} networks, sizes log-uniform [2e3, 2e5] ({n} neurons), growth rates log-uniform "
+                f"[{5e-5 * args.nu_scale:g}, {5e-4 * args.nu_scale:g}] per network")
+    else:
+        net = uniform_cube(args.sim_neurons, args.sim_side, seed=args.seed)
+        n, G = net.n, 1
+        lo, hi = net.box_min, net.box_max
+        desc = f"{n} neurons uniform in a {args.sim_side:g} um cube, growth rate {1e-4 * args.nu_scale:g}"
+    log(f"sim workload: {desc}, generated in {time.perf_counter() - t0:.1f} s")
+    periods, period = args.sim_periods, args.period
+    every = max(1, periods // 10)
+    out = {"metric": "simulated neuron steps/s (plasticity every period steps, one graph replay per period)",
+           "unit": "neuron-steps/s", "n_gpus": 1, "steps": periods, "warmup": 0, "higher_is_better": True,
+           "data": "synthetic",
+           "config": {"workload": f"{desc}; Izhikevich + calcium + Gaussian growth, input Normal(5, 2), "
+                                  f"growth thresholds eta {SIM_ETA}, target calcium {SIM_TARGET_CA}, theta 0.3, "
+                                  f"sigma 750 um, {periods} periods of {period} steps, elements start at 0, "
+                                  "synaptic input not modelled",
+                      "neurons": n, "networks": G, "periods": periods, "period_steps": period, "seed": args.seed}}
+    traj = {}
+    for name, prec in (("fp32_mixed", RL_PREC_FP32_MIXED), ("fp64", RL_PREC_FP64)):
+        g = BarnesHutHIP(lo, hi, theta=0.3, sigma=750.0, seed=args.seed + 1, precision=prec)
+        if args.workload == "sim-ensemble":
+            g.init_networks(ens.offsets, ens.boxes, ens.xyz, ens.signal)
+        else:
+            g.init(net.xyz, net.signal)
+        p = StepParams.izhikevich(input_mode=1, nu=1e-4 * args.nu_scale)
+        for t in range(3):
+            p.eta[t] = SIM_ETA[t]
+        g.neurons_init(p, np.full(n, SIM_TARGET_CA), x=np.full(n, -65.0), u=np.full(n, -13.0))
+        if args.workload == "sim-ensemble":
+            g.neurons_set_network_growth(np.repeat(ens.nu * args.nu_scale, 3))
+        cap = 4 * n
+        g.graph_create(period, cap)
+        rows, created, regrown = [], 0, 0
+        torch.cuda.synchronize()
+        t_run = time.perf_counter()
+        for per in range(periods):
+            try:
+                st = g.graph_launch(per * period, per)
+            except BHError as e:
+                if e.code != -7:
+                    raise
+                # more vacant axons than the captured capacity: the replay's update changed nothing,
+                # so the eager update of the same index completes the period; then grow the graph
+                st = g.update_connectivity(per)
+                cap *= 2
+                g.graph_create(period, cap)
+                regrown += 1
+            created += st["created"]
+            if (per + 1) % every == 0 or per + 1 == periods:
+                s = g.neurons_get()
+                rows.append({"step": (per + 1) * period, "mean_ca": float(s["ca"].mean()),
+                             "synapses_per_neuron": float(s["connected"][0].sum()) / n,
+                             "vacant_axons": int(g.get_vacancies()[0].sum())})
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_run
+        g.close()
+        out[name] = {"ms_per_period": 1e3 * wall / periods, "neuron_steps_per_s": n * period * periods / wall,
+                     "created": created, "graph_regrown": regrown, "trajectory": rows}
+        traj[name] = rows
+        log(f"sim {name}: {1e3 * wall / periods:.2f} ms per period of {period} steps, "
+            f"{n * period * periods / wall:.4g} neuron-steps/s, {created} synapses created")
+    a, b = traj["fp64"], traj["fp32_mixed"]
+    out["fp32_vs_fp64"] = {
+        "max_rel_diff_mean_ca": max(abs(x["mean_ca"] - y["mean_ca"]) / max(abs(x["mean_ca"]), 1e-300) for x, y in zip(a, b)),
+        "max_rel_diff_synapses_per_neuron": max(abs(x["synapses_per_neuron"] - y["synapses_per_neuron"])
+                                                / max(x["synapses_per_neuron"], 1e-300) for x, y in zip(a, b)),
+        "speedup": out["fp64"]["ms_per_period"] / out["fp32_mixed"]["ms_per_period"],
+    }
+    out["value"] = out["fp32_mixed"]["neuron_steps_per_s"]
+    out["ms_per_step"] = out["fp32_mixed"]["ms_per_period"] / period
+    out["dtype"] = "f64 step, f32-mixed search"
     out["vs_baseline"] = None
     out["scaling"] = "weak"
     print(json.dumps(out), flush=True)
@@ -248,8 +369,14 @@ def main(argv=None):
                     help="search kernel: wavefront-cooperative (default) or one lane per source in list order")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (reference kernel for rocprofv3 byte counters)")
-    ap.add_argument("--workload", choices=["single", "ensemble"], default="single",
-                    help="single: one 16M-neuron network (default); ensemble: batched parameter-sweep networks")
+    ap.add_argument("--workload", choices=["single", "ensemble", "sim-ensemble", "sim-single"], default="single",
+                    help="single: one 16M-neuron network (default); ensemble: batched parameter-sweep networks; "
+                         "sim-ensemble / sim-single: simulation periods replayed as graphs")
+    ap.add_argument("--sim-periods", type=int, default=100, help="simulation: plasticity periods")
+    ap.add_argument("--period", type=int, default=100, help="simulation: neuron steps per plasticity update")
+    ap.add_argument("--sim-neurons", type=int, default=4_000_000, help="sim-single: neurons")
+    ap.add_argument("--sim-side", type=float, default=16_000.0, help="sim-single: cube side in micrometres")
+    ap.add_argument("--nu-scale", type=float, default=10.0, help="simulation: growth rates x this factor")
     ap.add_argument("--networks", type=int, default=256, help="networks of the ensemble workload")
     ap.add_argument("--no-unbatched", action="store_true", help="ensemble: skip the one-context-per-network run")
     args = ap.parse_args(argv)
@@ -263,6 +390,8 @@ def main(argv=None):
 
     if args.workload == "ensemble":
         return run_ensemble(args)
+    if args.workload.startswith("sim-"):
+        return run_sim(args)
 
     from relearn_amd import RL_FLAG_EXACT_ORDER, RL_PREC_FP32_MIXED, RL_PREC_FP64
     from relearn_amd.workloads import uniform_cube
@@ -289,7 +418,8 @@ def main(argv=None):
         res[name] = run_precision(net, prec, args.steps, args.warmup, args.seed + 1, d_vac, flags=flags)
         r = res[name]
         log(f"{name}: {r['searches_per_s']:.4g} searches/s whole step, {r['ms_per_step']:.1f} ms/step, "
-            f"search {r['search_ms']:.1f} ms (kernel {r['dominant_kernel_ms']:.1f} ms), phases {['%.3f' % x for x in r['phase_s']]} s")
+            f"search {r['search_ms']:.1f} ms (kernel {r['dominant_kernel_ms']:.1f} ms), phases {['%.3f' % x for x in r['phase_s']]} s, "
+            f"graph replay {r['graph_ms_per_update']:.1f} ms")
 
     main_name = "fp32_mixed" if "fp32_mixed" in res else ("fp64" if "fp64" in res else "fp64_plain")
     main_r = res[main_name]
