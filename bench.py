@@ -47,6 +47,12 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     build_s = time.perf_counter() - t0
     m, depth = g.tree_info()
     log(f"precision {precision}: tree {m} nodes, depth {depth}, build {build_s:.3f} s")
+    tuning = None
+    if not (flags & 1):
+        # search launch tuning for this workload (results do not depend on it)
+        g.prepare_update_connectivity_device(*d_vac)
+        tuning = g.autotune(999)
+        log(f"precision {precision}: autotuned {tuning}")
 
     def one_step(step):
         t_a = time.perf_counter()
@@ -109,6 +115,7 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
         "node_tests_ref_per_step": tests_ref / steps,
         "phase_s": [float(np.mean([r[2][i] for r in recs])) for i in range(3)],
         "graph_ms_per_update": float(np.mean(graph_ms)),
+        "tuning": tuning,
         "requests_per_step": float(np.mean([r[0]["requests"] for r in recs])),
         "created_per_step": float(np.mean([r[1] for r in recs])),
         "targets": targets,

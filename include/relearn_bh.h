@@ -218,6 +218,23 @@ int rl_neurons_set_network_growth(rl_bh_ctx* ctx, const double* nu);
 int rl_neurons_get(rl_bh_ctx* ctx, double* x, double* u, double* ca, uint8_t* fired, double* delta,
                    double* grown, uint32_t* connected);
 
+/* ------------------------------------------------------------------ launch tuning */
+
+typedef struct {
+    uint32_t search_waves_per_cu;  /* persistent waves of the cooperative search per compute unit, 1..24 */
+    uint32_t reserved[7];
+} rl_bh_tuning;
+
+/* Launch tuning of the search (no counterpart in the reference, whose CPU loop has none).  Results
+ * do not depend on it; a captured period is released when it changes. */
+int rl_bh_set_tuning(rl_bh_ctx* ctx, const rl_bh_tuning* tuning);
+int rl_bh_get_tuning(rl_bh_ctx* ctx, rl_bh_tuning* tuning);
+
+/* Times the search of the prepared update `step` (after rl_bh_prepare_*) at candidate tunings and
+ * keeps the fastest.  Only the search runs; the vacancies do not change, and afterwards the search
+ * result of `step` is readable as after rl_bh_find_target_neurons.  best may be NULL. */
+int rl_bh_autotune(rl_bh_ctx* ctx, uint32_t step, rl_bh_tuning* best);
+
 /* ------------------------------------------------------------------ captured plasticity periods */
 
 /* Captures one plasticity period of the simulation loop (Simulation::simulate, Simulation.cpp:443-459)

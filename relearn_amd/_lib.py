@@ -32,7 +32,7 @@ EXPORTED = [
     "rl_bh_update_connectivity", "rl_bh_get_vacancies", "rl_bh_tree_info", "rl_bh_export_tree",
     "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
     "rl_neurons_get", "rl_neurons_set_network_growth", "rl_bh_graph_create", "rl_bh_graph_launch",
-    "rl_bh_graph_destroy",
+    "rl_bh_graph_destroy", "rl_bh_set_tuning", "rl_bh_get_tuning", "rl_bh_autotune",
 ]
 
 
@@ -67,6 +67,13 @@ class StepStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("search_waves_per_cu", C.c_uint32),
+        ("reserved", C.c_uint32 * 7),
+    ]
 
 
 class StepParams(C.Structure):
@@ -154,6 +161,9 @@ def load():
     L.rl_bh_graph_create.argtypes = [vp, C.c_uint32, C.c_uint64]
     L.rl_bh_graph_launch.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(StepStats)]
     L.rl_bh_graph_destroy.argtypes = [vp]
+    L.rl_bh_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.rl_bh_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.rl_bh_autotune.argtypes = [vp, C.c_uint32, C.POINTER(Tuning)]
     for name in EXPORTED:
         f = getattr(L, name)
         if name not in ("rl_bh_last_error", "rl_bh_network_seed"):
@@ -296,6 +306,22 @@ class BarnesHutHIP:
         st = StepStats()
         self._check(self.L.rl_bh_update_connectivity(self.h, step, C.byref(st)))
         return st.as_dict()
+
+    def set_tuning(self, search_waves_per_cu):
+        t = Tuning()
+        t.search_waves_per_cu = int(search_waves_per_cu)
+        self._check(self.L.rl_bh_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.L.rl_bh_get_tuning(self.h, C.byref(t)))
+        return {"search_waves_per_cu": t.search_waves_per_cu}
+
+    def autotune(self, step=0):
+        """Keeps the fastest search tuning for the prepared update; returns it."""
+        t = Tuning()
+        self._check(self.L.rl_bh_autotune(self.h, int(step), C.byref(t)))
+        return {"search_waves_per_cu": t.search_waves_per_cu}
 
     def graph_create(self, steps=0, max_vacant_axons=None):
         """Captures a plasticity period (steps neuron steps, element commit, connectivity update)."""

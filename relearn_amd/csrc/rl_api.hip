@@ -317,9 +317,10 @@ int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const doubl
     {
         int cus = 0;
         RL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
-        c->coop_blocks = static_cast<uint32_t>(std::max(cus, 1) * 16);
+        c->cus = static_cast<uint32_t>(std::max(cus, 1));
+        c->coop_blocks = c->cus * c->waves_per_cu;
         c->coop_cap = 2048;
-        const size_t slots = static_cast<size_t>(c->coop_blocks) * c->coop_cap;
+        const size_t slots = static_cast<size_t>(c->cus) * kMaxWavesPerCu * c->coop_cap;
         RL_TRY(dalloc(c, &c->coop_node, slots));
         RL_TRY(dalloc(c, &c->coop_fexp, slots));
         RL_TRY(dalloc(c, &c->coop_mass, slots));
@@ -685,6 +686,65 @@ int rl_neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fire
     }
     RL_TRY(set_device(c));
     return neurons_get(c, x, u, ca, fired, delta, grown, connected);
+}
+
+int rl_bh_set_tuning(rl_bh_ctx* c, const rl_bh_tuning* t) {
+    if (c == nullptr || t == nullptr) {
+        return RL_EINVAL;
+    }
+    if (t->search_waves_per_cu < 1 || t->search_waves_per_cu > kMaxWavesPerCu) {
+        return fail(c, RL_EINVAL, "rl_bh_set_tuning: search_waves_per_cu must be in [1, 24]");
+    }
+    if (t->search_waves_per_cu != c->waves_per_cu) {
+        destroy_graph(c);  // the captured launch has the old grid
+    }
+    c->waves_per_cu = t->search_waves_per_cu;
+    c->coop_blocks = c->cus * c->waves_per_cu;
+    return RL_OK;
+}
+
+int rl_bh_get_tuning(rl_bh_ctx* c, rl_bh_tuning* t) {
+    if (c == nullptr || t == nullptr) {
+        return RL_EINVAL;
+    }
+    *t = rl_bh_tuning{};
+    t->search_waves_per_cu = c->waves_per_cu;
+    return RL_OK;
+}
+
+int rl_bh_autotune(rl_bh_ctx* c, uint32_t step, rl_bh_tuning* best) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (!c->prepared) {
+        return fail(c, RL_ESTATE, "rl_bh_autotune: prepare the update first");
+    }
+    if (c->cus == 0) {
+        return fail(c, RL_ESTATE, "rl_bh_autotune: no neurons (rl_bh_init first)");
+    }
+    RL_TRY(set_device(c));
+    const uint32_t start = c->waves_per_cu;
+    RL_TRY(find_targets(c, step));  // warm-up at the current setting
+    uint32_t best_w = start;
+    double best_ms = c->last_search_ms;
+    for (uint32_t w : { 8U, 12U, 16U, 20U, 24U }) {
+        rl_bh_tuning t{};
+        t.search_waves_per_cu = w;
+        RL_TRY(rl_bh_set_tuning(c, &t));
+        RL_TRY(find_targets(c, step));
+        if (c->last_search_ms < best_ms) {
+            best_ms = c->last_search_ms;
+            best_w = w;
+        }
+    }
+    rl_bh_tuning t{};
+    t.search_waves_per_cu = best_w;
+    RL_TRY(rl_bh_set_tuning(c, &t));
+    RL_TRY(find_targets(c, step));  // leaves the search result of `step` readable, as find_target_neurons does
+    if (best != nullptr) {
+        *best = t;
+    }
+    return RL_OK;
 }
 
 int rl_bh_graph_create(rl_bh_ctx* c, uint32_t steps, uint64_t max_vacant_axons) {

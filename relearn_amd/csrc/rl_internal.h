@@ -18,6 +18,7 @@ constexpr uint32_t kNoTarget = 0xFFFFFFFFU;
 constexpr int kDrawsPerPass = 8;             // draws resolved per pick traversal
 constexpr unsigned kStreamBlocks = 2048;     // grid of the grid-stride kernels sized on the device
 constexpr int kCursorSlots = 8;              // work cursors of the cooperative search, one per XCD
+constexpr uint32_t kMaxWavesPerCu = 24;      // slot tables are allocated for this many search waves per CU
 
 // Device counters of one update (rl_bh_ctx::counters), zeroed at the start of every search and read
 // back with one copy.
@@ -164,7 +165,9 @@ struct rl_bh_ctx {
     uint32_t* coop_fexp = nullptr;
     double* coop_mass = nullptr;
     uint32_t coop_cap = 0;
-    uint32_t coop_blocks = 0;
+    uint32_t coop_blocks = 0;        // persistent search waves: cus * waves_per_cu
+    uint32_t cus = 0;
+    uint32_t waves_per_cu = 16;
     // (slot, source, node, level) of descents that need the general per-lane traversal (slots_cap)
     uint4* slow_items = nullptr;
     void* tmp = nullptr;

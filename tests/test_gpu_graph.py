@@ -172,3 +172,30 @@ def test_graph_needs_counter_input():
         g.graph_launch(0, 0)
     g.graph_create(0, 1000)  # the connectivity update alone needs no neuron input
     g.close()
+
+
+@pytest.mark.parametrize("precision", [RL_PREC_FP64, RL_PREC_FP32_MIXED])
+def test_search_tuning_does_not_change_results(precision):
+    _say("search tuning invariance, precision", precision)
+    net = uniform_cube(30_000, 3000.0, 35)
+    g = BarnesHutHIP(net.box_min, net.box_max, seed=6, precision=precision)
+    g.init(net.xyz, net.signal)
+    slots = int(net.vac_axon.sum())
+    ref = None
+    for w in (1, 5, 16, 24):
+        g.set_tuning(w)
+        assert g.get_tuning()["search_waves_per_cu"] == w
+        g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+        g.find_target_neurons(2)
+        cur = g.get_search_slots(slots)
+        if ref is None:
+            ref = cur
+        for a, b in zip(ref, cur):
+            assert np.array_equal(a, b), w
+    best = g.autotune(2)
+    assert 1 <= best["search_waves_per_cu"] <= 24
+    for a, b in zip(ref, g.get_search_slots(slots)):
+        assert np.array_equal(a, b)
+    with pytest.raises(BHError):
+        g.set_tuning(0)
+    g.close()
