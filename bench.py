@@ -27,9 +27,10 @@ METRIC = "target searches/s @16M neurons, FP32-mixed vs FP64; ms per plasticity 
 HBM_PEAK_GBS = 8000.0
 B_NODE = {"fp64": 40, "fp32_mixed": 24}   # bytes read per acceptance test (record + count/link)
 B_IO = 32                                  # source position + output ids per search
-# Simulation workloads: without synaptic input, Normal(5, 2) input drives the Izhikevich neurons to
-# about 0.0117 spikes per step, so calcium settles near 0.117 and never reaches the reference's
-# axon growth threshold 0.4; these thresholds and target let elements grow and connect.
+# Simulation workloads: Normal(5, 2) input alone drives the Izhikevich neurons to about 0.0117 spikes
+# per step, so calcium settles near 0.117 and never reaches the reference's axon growth threshold
+# 0.4, and with no axons no synapse forms to add input; these thresholds and target let elements
+# grow and connect.
 SIM_ETA = (0.02, 0.01, 0.0)
 SIM_TARGET_CA = 0.2
 
@@ -249,7 +250,8 @@ def run_sim(args):
            "config": {"workload": f"{desc}; Izhikevich + calcium + Gaussian growth, input Normal(5, 2), "
                                   f"growth thresholds eta {SIM_ETA}, target calcium {SIM_TARGET_CA}, theta 0.3, "
                                   f"sigma 750 um, {periods} periods of {period} steps, elements start at 0, "
-                                  "synaptic input not modelled",
+                                  f"equally weighted synaptic input (conductance {args.conductance:g}) over the "
+                                  "network the updates build",
                       "neurons": n, "networks": G, "periods": periods, "period_steps": period, "seed": args.seed}}
     traj = {}
     for name, prec in (("fp32_mixed", RL_PREC_FP32_MIXED), ("fp64", RL_PREC_FP64)):
@@ -261,6 +263,10 @@ def run_sim(args):
         p = StepParams.izhikevich(input_mode=1, nu=1e-4 * args.nu_scale)
         for t in range(3):
             p.eta[t] = SIM_ETA[t]
+        # synaptic input over the device network graph, which every update extends
+        p.synaptic_input = 1
+        p.synapse_conductance = args.conductance
+        g.net_init(24 * n)
         g.neurons_init(p, np.full(n, SIM_TARGET_CA), x=np.full(n, -65.0), u=np.full(n, -13.0))
         if args.workload == "sim-ensemble":
             g.neurons_set_network_growth(np.repeat(ens.nu * args.nu_scale, 3))
@@ -268,16 +274,19 @@ def run_sim(args):
         g.graph_create(period, cap)
         rows, created, regrown = [], 0, 0
         torch.cuda.synchronize()
-        t_run = time.perf_counter()
+        wall = 0.0   # replays only (each waits for its period); checkpoint copies are not timed
         for per in range(periods):
+            t_run = time.perf_counter()
             try:
                 st = g.graph_launch(per * period, per)
+                wall += time.perf_counter() - t_run
             except BHError as e:
                 if e.code != -7:
                     raise
                 # more vacant axons than the captured capacity: the replay's update changed nothing,
                 # so the eager update of the same index completes the period; then grow the graph
                 st = g.update_connectivity(per)
+                wall += time.perf_counter() - t_run
                 cap *= 2
                 g.graph_create(period, cap)
                 regrown += 1
@@ -287,11 +296,10 @@ def run_sim(args):
                 rows.append({"step": (per + 1) * period, "mean_ca": float(s["ca"].mean()),
                              "synapses_per_neuron": float(s["connected"][0].sum()) / n,
                              "vacant_axons": int(g.get_vacancies()[0].sum())})
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t_run
+        edges = int(g.net_get_edges()[0][-1])
         g.close()
         out[name] = {"ms_per_period": 1e3 * wall / periods, "neuron_steps_per_s": n * period * periods / wall,
-                     "created": created, "graph_regrown": regrown, "trajectory": rows}
+                     "created": created, "graph_regrown": regrown, "edges": edges, "trajectory": rows}
         traj[name] = rows
         log(f"sim {name}: {1e3 * wall / periods:.2f} ms per period of {period} steps, "
             f"{n * period * periods / wall:.4g} neuron-steps/s, {created} synapses created")
@@ -384,6 +392,8 @@ def main(argv=None):
     ap.add_argument("--sim-neurons", type=int, default=4_000_000, help="sim-single: neurons")
     ap.add_argument("--sim-side", type=float, default=16_000.0, help="sim-single: cube side in micrometres")
     ap.add_argument("--nu-scale", type=float, default=10.0, help="simulation: growth rates x this factor")
+    ap.add_argument("--conductance", type=float, default=0.03,
+                    help="simulation: synapse conductance (the reference's default 0.03)")
     ap.add_argument("--networks", type=int, default=256, help="networks of the ensemble workload")
     ap.add_argument("--no-unbatched", action="store_true", help="ensemble: skip the one-context-per-network run")
     args = ap.parse_args(argv)

@@ -189,6 +189,9 @@ typedef struct {
     double tau_ca, beta_ca;  /* calcium decay and per-spike increment */
     double eta[3];           /* minimum calcium to grow: axon, excitatory, inhibitory dendrite */
     double nu[3];            /* growth rates */
+    double synapse_conductance;  /* conductance per unit synapse weight (SynapticEquallyWeightedActivityInput) */
+    uint32_t synaptic_input; /* 1: add the synaptic input of the network graph (rl_net_init) */
+    uint32_t reserved;
 } rl_step_params;
 
 /* Allocates and uploads the per-neuron state of the step kernel (IzhikevichModel x/u, calcium,
@@ -217,6 +220,27 @@ int rl_neurons_set_network_growth(rl_bh_ctx* ctx, const double* nu);
 /* Copies the neuron state: any pointer may be NULL; delta/grown are 3*n (type-major). */
 int rl_neurons_get(rl_bh_ctx* ctx, double* x, double* u, double* ca, uint8_t* fired, double* delta,
                    double* grown, uint32_t* connected);
+
+/* ------------------------------------------------------------------ network graph and synaptic input */
+
+/* Network graph of the synaptic input (NetworkGraph's local in-edges, source/neurons/NetworkGraph.h):
+ * per target neuron its in-edges (source, integer weight), CSR by target: edges
+ * [in_offsets[t], in_offsets[t+1]) of the n + 1 offsets.  in_offsets NULL starts with no edge.  From
+ * then on every commit appends the synapses it creates, as Neurons::create_synapses adds them to the
+ * graph (Neurons.cpp:440-477): weight +1 for an excitatory source, -1 for an inhibitory one
+ * (Connector.cpp:33-121).  capacity: edges held without reallocation; the eager calls grow it, a
+ * captured period reports RL_ELIMIT instead.  rl_step_params.synaptic_input = 1 adds
+ * synapse_conductance times the summed weights of the in-edges whose source fired in the previous
+ * step to the input of each step (SynapticEquallyWeightedActivityInputCPU::update_local_input,
+ * SynapticEquallyWeightedActivityInputCPU.cpp:20-50, combined with the background input as
+ * CombinedActivityInputCPU.h does). */
+int rl_net_init(rl_bh_ctx* ctx, uint64_t capacity, const uint64_t* in_offsets, const uint32_t* sources,
+                const int32_t* weights);
+
+/* Copies the network graph (CSR by target; the order of edges inside a target is unspecified).
+ * With all three arrays NULL only num_edges is returned. */
+int rl_net_get_edges(rl_bh_ctx* ctx, uint64_t capacity, uint64_t* in_offsets, uint32_t* sources, int32_t* weights,
+                     uint64_t* num_edges);
 
 /* ------------------------------------------------------------------ launch tuning */
 

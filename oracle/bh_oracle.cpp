@@ -1300,6 +1300,35 @@ int orc_neuron_step(uint64_t n, const orc_step_params* p, double* x, double* u, 
     return ORC_OK;
 }
 
+// SynapticEquallyWeightedActivityInputCPU::update_local_input
+// (source/neurons/input/SynapticEquallyWeightedActivityInputCPU.cpp:21-50): per neuron, the weights of
+// the in-edges whose source fired, accumulated as doubles in edge order, times the conductance; then
+// CombinedActivityInputCPU::update_input_range (source/neurons/input/CombinedActivityInputCPU.h:46-62)
+// fills the input with 0 and adds the background input and the synaptic input in turn.
+int orc_synaptic_input(uint64_t n, const uint64_t* in_off, const uint32_t* in_src, const int32_t* in_w,
+                       const uint8_t* fired, double conductance, const double* background, double* out) {
+    if (n > 0 && (in_off == nullptr || fired == nullptr || out == nullptr)) {
+        return ORC_EINVAL;
+    }
+    for (uint64_t id = 0; id < n; id++) {
+        double inp = 0.0;
+        for (uint64_t e = in_off[id]; e < in_off[id + 1]; e++) {
+            if (fired[in_src[e]] == 0) {
+                continue;
+            }
+            inp += static_cast<double>(in_w[e]);
+        }
+        const double synaptic = inp * conductance;
+        double total = 0.0;
+        if (background != nullptr) {
+            total += background[id];
+        }
+        total += synaptic;
+        out[id] = total;
+    }
+    return ORC_OK;
+}
+
 // ElementBaseBase::update (source/neurons/synaptic_elements/ElementBaseBase.h:420-487)
 int orc_elements_update(uint64_t n, double* grown, double* delta, uint32_t* connected, uint32_t* vacant,
                         const double* retract_ratio, uint32_t* out_deleted) {

@@ -163,6 +163,12 @@ int orc_neuron_step(uint64_t n, const orc_step_params* p, double* x, double* u, 
                     double* delta, const uint8_t* enabled);
 
 /* a18: ElementBaseBase::update for one element type over n neurons; returns deletions per neuron. */
+/* Synaptic input of the equally weighted model plus the background input: for every neuron the
+ * weights (as doubles) of its in-edges whose source fired, summed in edge order, times conductance,
+ * added to background[id] (NULL: 0).  In-edges are CSR by target (in_off n + 1, in_src, in_w). */
+int orc_synaptic_input(uint64_t n, const uint64_t* in_off, const uint32_t* in_src, const int32_t* in_w,
+                       const uint8_t* fired, double conductance, const double* background, double* out);
+
 int orc_elements_update(uint64_t n, double* grown, double* delta, uint32_t* connected, uint32_t* vacant,
                         const double* retract_ratio, uint32_t* out_deleted);
 

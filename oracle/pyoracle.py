@@ -132,6 +132,9 @@ def lib():
         L.orc_neuron_step.argtypes = [C.c_uint64, C.POINTER(StepParams), _f64p, _f64p, _f64p, _f64p, _u8p, _f64p,
                                       _f64p, _u8p]
         L.orc_neuron_step.restype = C.c_int
+        L.orc_synaptic_input.argtypes = [C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32), _u8p, C.c_double, _f64p,
+                                         _f64p]
+        L.orc_synaptic_input.restype = C.c_int
         L.orc_elements_update.argtypes = [C.c_uint64, _f64p, _f64p, _u32p, _u32p, _f64p, _u32p]
         L.orc_elements_update.restype = C.c_int
         _lib = L
@@ -337,6 +340,21 @@ def neuron_step(params: StepParams, x, u, ca, inp, fired, target_ca, delta, enab
     _check(lib().orc_neuron_step(n, C.byref(params), _ptr(x, _f64p), _ptr(u, _f64p), _ptr(ca, _f64p),
                                  _ptr(inp, _f64p), _ptr(fired, _u8p), _ptr(target_ca, _f64p), _ptr(delta, _f64p),
                                  _ptr(en, _u8p)), "orc_neuron_step")
+
+
+def synaptic_input(in_off, in_src, in_w, fired, conductance, background=None):
+    """Background plus equally weighted synaptic input of every neuron (in-edges CSR by target)."""
+    n = len(in_off) - 1
+    off = np.ascontiguousarray(in_off, dtype=np.uint64)
+    src = np.ascontiguousarray(in_src, dtype=np.uint32)
+    w = np.ascontiguousarray(in_w, dtype=np.int32)
+    f = np.ascontiguousarray(fired, dtype=np.uint8)
+    bg = None if background is None else np.ascontiguousarray(background, dtype=np.float64)
+    out = np.zeros(max(n, 1))
+    _check(lib().orc_synaptic_input(n, _ptr(off, _u64p), _ptr(src, _u32p), _ptr(w, C.POINTER(C.c_int32)),
+                                    _ptr(f, _u8p), float(conductance), _ptr(bg, _f64p), _ptr(out, _f64p)),
+           "orc_synaptic_input")
+    return out[:n]
 
 
 def elements_update(grown, delta, connected, vacant, retract_ratio=None):

@@ -32,7 +32,8 @@ EXPORTED = [
     "rl_bh_update_connectivity", "rl_bh_get_vacancies", "rl_bh_tree_info", "rl_bh_export_tree",
     "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
     "rl_neurons_get", "rl_neurons_set_network_growth", "rl_bh_graph_create", "rl_bh_graph_launch",
-    "rl_bh_graph_destroy", "rl_bh_set_tuning", "rl_bh_get_tuning", "rl_bh_autotune",
+    "rl_bh_graph_destroy", "rl_bh_set_tuning", "rl_bh_get_tuning", "rl_bh_autotune", "rl_net_init",
+    "rl_net_get_edges",
 ]
 
 
@@ -94,6 +95,9 @@ class StepParams(C.Structure):
         ("beta_ca", C.c_double),
         ("eta", C.c_double * 3),
         ("nu", C.c_double * 3),
+        ("synapse_conductance", C.c_double),
+        ("synaptic_input", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     @classmethod
@@ -162,6 +166,8 @@ def load():
     L.rl_bh_graph_launch.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(StepStats)]
     L.rl_bh_graph_destroy.argtypes = [vp]
     L.rl_bh_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.rl_net_init.argtypes = [vp, C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32)]
+    L.rl_net_get_edges.argtypes = [vp, C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32), _u64p]
     L.rl_bh_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.rl_bh_autotune.argtypes = [vp, C.c_uint32, C.POINTER(Tuning)]
     for name in EXPORTED:
@@ -306,6 +312,25 @@ class BarnesHutHIP:
         st = StepStats()
         self._check(self.L.rl_bh_update_connectivity(self.h, step, C.byref(st)))
         return st.as_dict()
+
+    def net_init(self, capacity, in_offsets=None, sources=None, weights=None):
+        """Network graph of the synaptic input: CSR by target (n + 1 offsets), later commits append."""
+        off = None if in_offsets is None else np.ascontiguousarray(in_offsets, dtype=np.uint64)
+        src = None if sources is None else np.ascontiguousarray(sources, dtype=np.uint32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.int32)
+        self._check(self.L.rl_net_init(self.h, int(capacity), _ptr(off, _u64p), _ptr(src, _u32p),
+                                       _ptr(w, C.POINTER(C.c_int32))))
+
+    def net_get_edges(self):
+        ne = C.c_uint64()
+        self._check(self.L.rl_net_get_edges(self.h, 0, None, None, None, C.byref(ne)))
+        m = int(ne.value)
+        off = np.zeros(self.n + 1, np.uint64)
+        src = np.zeros(max(m, 1), np.uint32)
+        w = np.zeros(max(m, 1), np.int32)
+        self._check(self.L.rl_net_get_edges(self.h, m, _ptr(off, _u64p), _ptr(src, _u32p),
+                                            _ptr(w, C.POINTER(C.c_int32)), C.byref(ne)))
+        return off, src[:m], w[:m]
 
     def set_tuning(self, search_waves_per_cu):
         t = Tuning()

@@ -58,6 +58,7 @@ void free_all(rl_bh_ctx* c) {
     destroy_graph(c);
     free_tree(c);
     neurons_free(c);
+    net_free(c);
     dfree(&c->pos);
     dfree(&c->qpos);
     dfree(&c->signal);
@@ -511,6 +512,10 @@ int rl_bh_commit(rl_bh_ctx* c, uint32_t step, const uint64_t* order, uint64_t* c
     }
     RL_TRY(commit(c, step, order, created));
     c->prepared = false;  // vacancies changed: the tree must be prepared again before the next search
+    if (c->nreq > 0) {
+        RL_TRY(net_append(c));
+        RL_HIP(c, hipStreamSynchronize(c->stream));
+    }
     return RL_OK;
 }
 
@@ -558,6 +563,7 @@ int rl_bh_update_connectivity(rl_bh_ctx* c, uint32_t step, rl_bh_step_stats* sta
         }
         if ((status = commit(c, step, nullptr, &created)) != RL_OK) break;
         c->prepared = false;
+        if (c->nreq > 0 && (status = net_append(c)) != RL_OK) break;
         if (hipEventRecord(e[3], s) != hipSuccess || hipEventSynchronize(e[3]) != hipSuccess) {
             status = fail(c, RL_EHIP, "event record");
             break;
@@ -764,6 +770,7 @@ int rl_bh_graph_create(rl_bh_ctx* c, uint32_t steps, uint64_t max_vacant_axons) 
     RL_TRY(set_device(c));
     destroy_graph(c);
     RL_TRY(ensure_slots(c, std::max<uint64_t>(max_vacant_axons, 1)));
+    RL_TRY(normalize_fired(c));
     hipStream_t s = c->stream;
     RL_HIP(c, hipStreamSynchronize(s));
     RL_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -776,6 +783,7 @@ int rl_bh_graph_create(rl_bh_ctx* c, uint32_t steps, uint64_t max_vacant_axons) 
     if (status == RL_OK) status = prepare_tree(c);
     if (status == RL_OK) status = enqueue_search(c);
     if (status == RL_OK) status = enqueue_commit(c, nullptr);
+    if (status == RL_OK) status = enqueue_net_append(c);
     c->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
@@ -806,6 +814,7 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     }
     RL_TRY(set_device(c));
     hipStream_t s = c->stream;
+    RL_TRY(normalize_fired(c));
     RL_TRY(set_steps(c, first_step, update));
     if (stats != nullptr) {
         RL_HIP(c, hipEventRecord(c->ev0, s));
@@ -840,6 +849,24 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
         return fail(c, RL_ELIMIT, "rl_bh_graph_launch: an update had more vacant axons than max_vacant_axons");
     }
     return RL_OK;
+}
+
+int rl_net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const uint32_t* sources,
+                const int32_t* weights) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    return net_init(c, capacity, in_offsets, sources, weights);
+}
+
+int rl_net_get_edges(rl_bh_ctx* c, uint64_t capacity, uint64_t* in_offsets, uint32_t* sources, int32_t* weights,
+                     uint64_t* num_edges) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    return net_get(c, capacity, in_offsets, sources, weights, num_edges);
 }
 
 int rl_bh_graph_destroy(rl_bh_ctx* c) {

@@ -35,6 +35,8 @@ enum : int {
     kCtrRedo = 9,        // sources of the per-lane path that need the fallback pass
     kCtrActive = 10,     // sources with vacant axons
     kCtrSticky = 11,     // over-capacity seen since the host last read the counters
+    kCtrNetOver = 12,    // 1: the created synapses did not fit the network graph, none was appended
+    kCtrNetNew = 13,     // synapses appended to the network graph by the last commit
     kCounters = 16
 };
 
@@ -63,13 +65,32 @@ struct Tree {
     uint32_t* node_net = nullptr;       // network of each node (the roots are nodes 0..networks-1)
 };
 
+// In-edges of every neuron for the synaptic input (the local in-edges of NetworkGraph), CSR by
+// target: edges [off[t], off[t+1]) hold (source, weight).  Every commit appends the synapses it
+// creates.  The append writes the merged graph to the second buffer and copies it back, so the
+// arrays a captured period refers to never change.
+struct NetGraph {
+    uint64_t* off = nullptr;       // n + 1
+    uint32_t* src = nullptr;       // cap
+    int32_t* w = nullptr;          // cap
+    uint64_t* off2 = nullptr;      // n + 1: merge buffers
+    uint32_t* src2 = nullptr;
+    int32_t* w2 = nullptr;
+    uint32_t* ncnt = nullptr;      // n: synapses appended per target
+    uint64_t* npre = nullptr;      // n + 1: their prefix sum
+    uint64_t cap = 0;
+    bool ready = false;
+};
+
 struct Neurons {
     double* x = nullptr;
     double* u = nullptr;
     double* ca = nullptr;
     double* target_ca = nullptr;
     double* input = nullptr;
-    uint8_t* fired = nullptr;
+    uint8_t* fired_buf[2] = { nullptr, nullptr };  // fired status of the last two steps
+    uint8_t* fired = nullptr;     // fired_buf[parity]: the last step's
+    int parity = 0;
     double* grown = nullptr;      // 3n
     double* delta = nullptr;      // 3n
     uint32_t* connected = nullptr;  // 3n
@@ -177,6 +198,7 @@ struct rl_bh_ctx {
     uint32_t graph_steps = 0;
     bool capturing = false;
     rl::Neurons neurons;
+    rl::NetGraph net;
 };
 
 namespace rl {
@@ -247,11 +269,21 @@ int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const d
 int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input);
 int enqueue_neuron_steps(rl_bh_ctx* c, uint32_t steps);  // steps d_steps[0] + k, k < steps
 int enqueue_elements_commit(rl_bh_ctx* c);
+int normalize_fired(rl_bh_ctx* c);      // the last fired status into the buffer a captured period reads
 int neurons_commit(rl_bh_ctx* c, uint32_t* deleted);
 int neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
                 uint32_t* connected);
 int neurons_set_network_growth(rl_bh_ctx* c, const double* nu);
 void neurons_free(rl_bh_ctx* c);
+
+// network graph (rl_net.hip)
+int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const uint32_t* sources,
+             const int32_t* weights);
+int net_get(rl_bh_ctx* c, uint64_t capacity, uint64_t* in_offsets, uint32_t* sources, int32_t* weights,
+            uint64_t* num_edges);
+int enqueue_net_append(rl_bh_ctx* c);   // appends the accepted requests of the last commit
+int net_append(rl_bh_ctx* c);           // eager: grows the capacity if needed, then appends
+void net_free(rl_bh_ctx* c);
 
 }  // namespace rl
 

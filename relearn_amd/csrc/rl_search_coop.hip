@@ -727,9 +727,13 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // BarnesHutBase.h:283-289).  Counts the searches, the successful ones and the reference's tests.
 __global__ void __launch_bounds__(256) k_finalize(SearchArgs s, const uint32_t* __restrict__ work,
                                                   const uint64_t* __restrict__ n_work) {
+    const uint64_t nw = *n_work;
+    if (blockIdx.x * static_cast<uint64_t>(blockDim.x) >= nw) {
+        return;  // the grid covers every neuron; only the active sources have work
+    }
     const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     unsigned long long ref = 0, searched_n = 0;
-    if (t < *n_work) {
+    if (t < nw) {
         const uint32_t src = work[t];
         if (s.found[src] != kOverflowMark) {
             const uint32_t na = s.vac_axon[src];
@@ -756,7 +760,7 @@ __global__ void __launch_bounds__(256) k_finalize(SearchArgs s, const uint32_t* 
     }
     ref = wave_sum(ref);
     searched_n = wave_sum(searched_n);
-    if ((threadIdx.x & (kWave - 1)) == 0) {
+    if ((threadIdx.x & (kWave - 1)) == 0 && searched_n != 0) {
         atomicAdd(&s.counters[kCtrTestsRef], ref);
         atomicAdd(&s.counters[kCtrSearches], searched_n);
     }

@@ -60,24 +60,46 @@ struct StepArgs {
     double* u;
     double* ca;
     const double* input;
+    const uint8_t* fired_prev;   // fired status of the previous step (synaptic input, disabled carry)
     uint8_t* fired;
     const double* target_ca;
     double* delta;
     const uint8_t* enabled;
+    // synaptic input: in-edges CSR by target, or null
+    const uint64_t* in_off;
+    const uint32_t* in_src;
+    const int32_t* in_w;
+    double conductance;
 };
 
 __global__ void k_step(StepArgs a) {
 #pragma clang fp contract(off)
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i >= a.n || a.enabled[i] == 0) {
+    if (i >= a.n) {
+        return;
+    }
+    if (a.enabled[i] == 0) {
+        a.fired[i] = a.fired_prev[i];  // the step leaves a disabled neuron's status as it was
         return;
     }
     const rl_step_params& p = a.p;
     const uint32_t g = a.net_of[i];
     const uint32_t step = a.step_base != nullptr ? *a.step_base + a.step : a.step;
-    const double input = p.input_mode == 0 ? a.input[i]
-                                           : normal_input(a.net_seed[g], static_cast<uint32_t>(i - a.net_off[g]), step,
-                                                          p.input_mean, p.input_stddev);
+    double input = p.input_mode == 0 ? a.input[i]
+                                     : normal_input(a.net_seed[g], static_cast<uint32_t>(i - a.net_off[g]), step,
+                                                    p.input_mean, p.input_stddev);
+    if (a.in_off != nullptr) {
+        // SynapticEquallyWeightedActivityInputCPU::update_local_input: the weights of the in-edges
+        // whose source fired in the previous step, times the conductance; the integer sum is what the
+        // reference's FP64 accumulation of integer weights yields.  CombinedActivityInputCPU adds it
+        // to the background input.
+        int64_t acc = 0;
+        const uint64_t e1 = a.in_off[i + 1];
+        for (uint64_t e = a.in_off[i]; e < e1; e++) {
+            acc += a.fired_prev[a.in_src[e]] != 0 ? a.in_w[e] : 0;
+        }
+        input = input + static_cast<double>(acc) * a.conductance;
+    }
     const double scale = 1.0 / static_cast<double>(p.h);
     const double ab = p.a * p.b;
     double xv = a.x[i];
@@ -170,7 +192,10 @@ void neurons_free(rl_bh_ctx* c) {
     dfree(&s.ca);
     dfree(&s.target_ca);
     dfree(&s.input);
-    dfree(&s.fired);
+    dfree(&s.fired_buf[0]);
+    dfree(&s.fired_buf[1]);
+    s.fired = nullptr;
+    s.parity = 0;
     dfree(&s.grown);
     dfree(&s.delta);
     dfree(&s.deleted);
@@ -212,7 +237,10 @@ int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const d
     RL_TRY(dalloc(c, &s.ca, n));
     RL_TRY(dalloc(c, &s.target_ca, n));
     RL_TRY(dalloc(c, &s.input, n));
-    RL_TRY(dalloc(c, &s.fired, n));
+    RL_TRY(dalloc(c, &s.fired_buf[0], n));
+    RL_TRY(dalloc(c, &s.fired_buf[1], n));
+    s.parity = 0;
+    s.fired = s.fired_buf[0];
     RL_TRY(dalloc(c, &s.grown, 3 * n));
     RL_TRY(dalloc(c, &s.delta, 3 * n));
     RL_TRY(dalloc(c, &s.deleted, 3 * n));
@@ -230,7 +258,8 @@ int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const d
     RL_TRY(up(s.target_ca, target_ca, n));
     RL_TRY(up(s.grown, grown, 3 * n));
     RL_HIP(c, hipMemsetAsync(s.delta, 0, 3 * n * sizeof(double), st));
-    RL_HIP(c, hipMemsetAsync(s.fired, 0, n, st));
+    RL_HIP(c, hipMemsetAsync(s.fired_buf[0], 0, n, st));
+    RL_HIP(c, hipMemsetAsync(s.fired_buf[1], 0, n, st));
     for (int t = 0; t < 3; t++) {
         if (connected != nullptr) {
             RL_HIP(c, hipMemcpyAsync(c->conn[t], connected + t * n, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -258,11 +287,32 @@ StepArgs step_args(rl_bh_ctx* c) {
     a.u = s.u;
     a.ca = s.ca;
     a.input = s.input;
-    a.fired = s.fired;
+    a.fired_prev = s.fired_buf[s.parity];
+    a.fired = s.fired_buf[s.parity ^ 1];
     a.target_ca = s.target_ca;
     a.delta = s.delta;
     a.enabled = c->enabled;
+    if (s.p.synaptic_input != 0) {
+        a.in_off = c->net.off;
+        a.in_src = c->net.src;
+        a.in_w = c->net.w;
+        a.conductance = s.p.synapse_conductance;
+    }
     return a;
+}
+
+__global__ void k_copy_u8(uint64_t n, const uint8_t* __restrict__ from, uint8_t* __restrict__ to) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
+        to[i] = from[i];
+    }
+}
+
+int check_synaptic(rl_bh_ctx* c) {
+    if (c->neurons.p.synaptic_input != 0 && !c->net.ready) {
+        return fail(c, RL_ESTATE, "synaptic input needs the network graph (rl_net_init)");
+    }
+    return RL_OK;
 }
 
 }  // namespace
@@ -279,6 +329,7 @@ int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
     if (n == 0) {
         return RL_OK;
     }
+    RL_TRY(check_synaptic(c));
     if (s.p.input_mode == 0) {
         RL_HIP(c, hipMemcpyAsync(s.input, input, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
@@ -286,6 +337,21 @@ int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
     a.step = step;
     k_step<<<grid_for(n), kBlock, 0, c->stream>>>(a);
     RL_HIP(c, hipGetLastError());
+    s.parity ^= 1;
+    s.fired = s.fired_buf[s.parity];
+    return RL_OK;
+}
+
+// The last step's fired status into fired_buf[0], where a captured period expects it.
+int normalize_fired(rl_bh_ctx* c) {
+    Neurons& s = c->neurons;
+    if (!s.ready || s.parity == 0) {
+        return RL_OK;
+    }
+    k_copy_u8<<<kStreamBlocks, kBlock, 0, c->stream>>>(c->n, s.fired_buf[1], s.fired_buf[0]);
+    RL_HIP(c, hipGetLastError());
+    s.parity = 0;
+    s.fired = s.fired_buf[0];
     return RL_OK;
 }
 
@@ -298,11 +364,23 @@ int enqueue_neuron_steps(rl_bh_ctx* c, uint32_t steps) {
     if (c->n == 0) {
         return RL_OK;
     }
+    RL_TRY(check_synaptic(c));
+    if (s.parity != 0) {
+        return fail(c, RL_ESTATE, "captured neuron steps start from fired_buf[0]");
+    }
+    // step k reads fired_buf[k & 1] and writes the other; an odd count copies the last status back,
+    // so every replay starts and ends with it in fired_buf[0]
     StepArgs a = step_args(c);
     a.step_base = c->d_steps;
     for (uint32_t k = 0; k < steps; k++) {
         a.step = k;
+        a.fired_prev = s.fired_buf[k & 1U];
+        a.fired = s.fired_buf[(k + 1) & 1U];
         k_step<<<grid_for(c->n), kBlock, 0, c->stream>>>(a);
+        RL_HIP(c, hipGetLastError());
+    }
+    if ((steps & 1U) != 0) {
+        k_copy_u8<<<kStreamBlocks, kBlock, 0, c->stream>>>(c->n, s.fired_buf[1], s.fired_buf[0]);
         RL_HIP(c, hipGetLastError());
     }
     return RL_OK;

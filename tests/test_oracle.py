@@ -444,3 +444,28 @@ def test_elements_update_cases(oracle):
     assert conn.tolist() == [2, 2, 1, 0]
     assert deleted.tolist() == [0, 0, 1, 0]
     assert np.all(delta == 0)
+
+
+def test_synaptic_input_matches_dense_product(oracle):
+    # closed form: input = background + conductance * A f, where A[t, s] sums the weights of the
+    # edges s -> t (repeated pairs add up) and f is the fired vector; the source's status counts,
+    # never the target's
+    rng = np.random.default_rng(21)
+    n, m = 60, 700
+    tgt = rng.integers(0, n, m)
+    src = rng.integers(0, n, m)
+    w = rng.integers(-3, 4, m).astype(np.int32)
+    order = np.argsort(tgt, kind="stable")
+    off = np.concatenate([[0], np.cumsum(np.bincount(tgt, minlength=n))]).astype(np.uint64)
+    fired = (rng.random(n) < 0.4).astype(np.uint8)
+    bg = rng.normal(5.0, 2.0, n)
+    A = np.zeros((n, n))
+    np.add.at(A, (tgt, src), w)
+    got = oracle.synaptic_input(off, src[order], w[order], fired, 0.25, bg)
+    want = bg + 0.25 * (A @ fired.astype(np.float64))
+    assert np.array_equal(got, want)
+    # without background input the result is the synaptic term alone; no edges give 0
+    assert np.array_equal(oracle.synaptic_input(off, src[order], w[order], fired, 0.25), 0.25 * (A @ fired))
+    empty = np.zeros(n + 1, np.uint64)
+    assert np.array_equal(oracle.synaptic_input(empty, np.zeros(0, np.uint32), np.zeros(0, np.int32), fired, 2.0),
+                          np.zeros(n))
