@@ -845,8 +845,15 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     c->last_tests_ref = h[kCtrTestsRef];
     c->last_slow_descents = h[kCtrSlow];
     c->searched_valid = true;
-    if (h[kCtrOverCap] != 0 || h[kCtrSticky] != 0) {
-        return fail(c, RL_ELIMIT, "rl_bh_graph_launch: an update had more vacant axons than max_vacant_axons");
+    if (h[kCtrOverCap] != 0) {
+        return fail(c, RL_ELIMIT, "rl_bh_graph_launch: more vacant axons than max_vacant_axons; the update changed nothing");
+    }
+    if (h[kCtrNetOver] != 0) {
+        return fail(c, RL_ELIMIT, "rl_bh_graph_launch: the created synapses exceed the network graph's capacity and were "
+                                  "not added to it");
+    }
+    if (h[kCtrSticky] != 0) {
+        return fail(c, RL_ELIMIT, "rl_bh_graph_launch: an earlier replay without stats exceeded a capacity");
     }
     return RL_OK;
 }

@@ -171,6 +171,14 @@ public:
                                   vacant_inhibitory_dendrites.data()));
     }
 
+    // Chooses the search launch tuning on the prepared update (after prepare_update_connectivity) by
+    // timing it; the search results do not depend on it.  Returns the persistent waves per CU kept.
+    std::uint32_t autotune() {
+        rl_bh_tuning t{};
+        check(rl_bh_autotune(ctx_, step_, &t));
+        return t.search_waves_per_cu;
+    }
+
     // The algorithm is selected under the CPU path's name (relearn.cpp:313).
     static constexpr const char* get_algorithm_type() { return "barnes-hut"; }
 
