@@ -1,0 +1,121 @@
+"""End-to-end parity of a small simulation (1000 neurons in a 1000 um cube, 80/20 E/I) with the oracle.
+
+Each period runs 100 neuron steps with background input (the same caller arrays on both sides) and
+synaptic input over the network the updates build, the element commit of the three element types,
+and one connectivity update (leaf fill, upward pass, search, commit).  The oracle restates each of
+these on the CPU (neuron step, synaptic input, ElementBaseBase::update, pointer octree search under
+the counter-based draws, serial commit) and keeps its own network graph.  After every period the
+neuron state, the element counts, the vacancies, the created synapses and the network graph must be
+identical.  FP64 only: the FP32-mixed picks are allowed to differ from FP64 (measured by bench.py).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from relearn_amd import RL_PREC_FP64, BarnesHutHIP, StepParams
+from relearn_amd.workloads import uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[gpu-case]", *a, file=sys.stderr, flush=True)
+
+
+def _csr(tgt, src, w, n):
+    order = np.argsort(tgt, kind="stable")
+    off = np.concatenate([[0], np.cumsum(np.bincount(tgt, minlength=n))]).astype(np.uint64)
+    return off, np.asarray(src)[order].astype(np.uint32), np.asarray(w)[order].astype(np.int32)
+
+
+def _multiset(off, src, w):
+    t = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off.astype(np.int64)))
+    keys = np.stack([t, src.astype(np.int64), w.astype(np.int64)])
+    return keys[:, np.lexsort(keys[::-1])]
+
+
+def test_small_simulation_matches_oracle(oracle):
+    n, periods, steps, seed, cond = 1000, 6, 100, 5, 1.5
+    _say("simulation parity", n, "neurons,", periods, "periods of", steps, "steps")
+    net = uniform_cube(n, 1000.0, 0)
+    rng = np.random.default_rng(17)
+    eta, target = (0.02, 0.01, 0.0), 0.2
+    pg = StepParams.izhikevich(input_mode=0, nu=0.02)
+    pg.synaptic_input, pg.synapse_conductance = 1, cond
+    po = oracle.StepParams()
+    for f in ("a", "b", "c", "d", "v_spike", "k1", "k2", "k3", "h", "tau_ca", "beta_ca"):
+        setattr(po, f, getattr(pg, f))
+    for t in range(3):
+        pg.eta[t] = po.eta[t] = eta[t]
+        po.nu[t] = pg.nu[t]
+    x = np.full(n, -65.0)
+    u = np.full(n, -13.0)
+    ca = rng.uniform(0.05, 0.15, n)
+    tca = np.full(n, target)
+    grown = rng.poisson(1.0, (3, n)).astype(np.float64)
+
+    g = BarnesHutHIP(net.box_min, net.box_max, seed=seed, precision=RL_PREC_FP64)
+    g.init(net.xyz, net.signal)
+    g.net_init(50 * n)
+    g.neurons_init(pg, tca, x=x, u=u, ca=ca, grown=grown.ravel())
+
+    tree = oracle.Tree(net.xyz, net.box_min, net.box_max)
+    xo, uo, co = x.copy(), u.copy(), ca.copy()
+    fo = np.zeros(n, np.uint8)
+    do = np.zeros(3 * n)
+    go = grown.copy()
+    conn = np.zeros((3, n), np.uint32)
+    vac = np.zeros((3, n), np.uint32)
+    et, es, ew = np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.int64)
+    created_total = 0
+    for per in range(periods):
+        off, src, w = _csr(et, es, ew, n)
+        for k in range(steps):
+            bg = rng.normal(5.0, 2.0, n)
+            inp = oracle.synaptic_input(off, src, w, fo, cond, bg)
+            oracle.neuron_step(po, xo, uo, co, inp, fo, tca, do, None)
+            g.neurons_step(per * steps + k, bg)
+        dv = do.reshape(3, n)
+        for t in range(3):
+            gt, dt, ct, vt = go[t].copy(), dv[t].copy(), conn[t].copy(), vac[t].copy()
+            oracle.elements_update(gt, dt, ct, vt)
+            go[t], dv[t], conn[t], vac[t] = gt, dt, ct, vt
+        do = dv.ravel().copy()
+        g.neurons_commit_elements()
+        # connectivity update in the oracle: search under the counter-based draws, serial commit
+        tree.update(vac[1], vac[2])
+        r = tree.find_targets(net.signal, vac[0], seed=seed, step=per)
+        assert np.all(r["margin"] > 1e-12), "a draw within 1e-12 of a boundary: picks may legitimately differ"
+        cnt = r["count"].astype(np.int64)
+        rs = np.repeat(np.arange(n), cnt)
+        rj = np.concatenate([np.arange(c) for c in cnt]) if cnt.sum() else np.zeros(0, np.int64)
+        rt = r["target"][r["offsets"].astype(np.int64)[rs] + rj]
+        rg = net.signal[rs]
+        ve, vi, va = vac[1].copy(), vac[2].copy(), vac[0].copy()
+        ce, ci, cax = conn[1].copy(), conn[2].copy(), conn[0].copy()
+        ok, created = oracle.commit(rt, rs, rg, rj, ve, vi, va, seed=seed, step=per, conn_exc=ce, conn_inh=ci,
+                                    conn_axon=cax)
+        vac[1], vac[2], vac[0], conn[1], conn[2], conn[0] = ve, vi, va, ce, ci, cax
+        acc = ok.astype(bool)
+        et = np.concatenate([et, rt[acc].astype(np.int64)])
+        es = np.concatenate([es, rs[acc].astype(np.int64)])
+        ew = np.concatenate([ew, np.where(rg[acc] != 0, -1, 1)])
+        st = g.update_connectivity(per)
+        created_total += created
+        _say("period", per, "requests", len(rt), "created", created, "edges", len(et))
+        # compare the whole state
+        assert st["requests"] == len(rt) and st["created"] == created, per
+        sg = g.neurons_get()
+        for key, ref in (("x", xo), ("u", uo), ("ca", co), ("fired", fo)):
+            assert np.array_equal(sg[key], ref), (per, key)
+        assert np.allclose(sg["delta"].ravel(), do, rtol=1e-9, atol=1e-300), per
+        assert np.allclose(sg["grown"], go, rtol=1e-9, atol=1e-12), per
+        assert np.array_equal(sg["connected"], conn), per
+        gv = g.get_vacancies()
+        for t in range(3):
+            assert np.array_equal(gv[t], vac[t]), (per, t)
+        assert np.array_equal(_multiset(*g.net_get_edges()), _multiset(*_csr(et, es, ew, n))), per
+    assert created_total > 0
+    tree.close()
+    g.close()
