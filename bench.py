@@ -100,6 +100,13 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     g.find_target_neurons(0)
     targets, _, counts = g.get_search_slots(slots)
     created0 = g.commit(0)
+    # in-degree of every neuron after that update, and the level-1 candidate lists of a fixed
+    # sample of sources (probabilities of this precision) for the precision comparison
+    _, ve1, vi1 = g.get_vacancies()
+    indeg = (net.vac_exc.astype(np.int64) - ve1) + (net.vac_inh.astype(np.int64) - vi1)
+    g.prepare_update_connectivity_device(*d_vac)
+    sample = np.random.default_rng(seed + 7).choice(net.n, min(1000, net.n), replace=False)
+    cand = g.candidate_lists(sample, capacity=16384)
     g.close()
     return {
         "build_s": build_s,
@@ -121,6 +128,8 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
         "created_per_step": float(np.mean([r[1] for r in recs])),
         "targets": targets,
         "counts": counts,
+        "indeg": indeg,
+        "cand": cand,
         "created0": created0,
     }
 
@@ -334,6 +343,33 @@ def copy_calibration(mib):
     return 0
 
 
+def precision_comparison(a, b):
+    """Probability L-infinity on level-1 candidate lists (probabilities normalised by the FP64 list
+    total) and the total-variation distance of the in-degree histograms, FP64 run a vs FP32-mixed b."""
+    linf, rel, same, longest = 0.0, 0.0, True, 0
+    for (n64, p64, c64), (n32, p32, c32) in zip(a["cand"], b["cand"]):
+        longest = max(longest, c64)
+        if c64 != c32 or not np.array_equal(n64, n32):
+            same = False
+            continue
+        if len(p64) == 0:
+            continue
+        tot = float(p64.sum())
+        if tot > 0:
+            linf = max(linf, float(np.abs(p32 - p64).max()) / tot)
+        # candidates that can matter to a pick (FP32 exp2 flushes probabilities below ~1e-38 to 0)
+        pos = p64 > 1e-12 * tot
+        if pos.any():
+            rel = max(rel, float((np.abs(p32 - p64)[pos] / p64[pos]).max()))
+    h64, h32 = np.bincount(a["indeg"]), np.bincount(b["indeg"])
+    m = max(len(h64), len(h32))
+    h64 = np.pad(h64, (0, m - len(h64)))
+    h32 = np.pad(h32, (0, m - len(h32)))
+    return {"probability_linf": linf, "probability_max_rel_err_above_1e-12": rel, "candidate_lists_identical": same,
+            "candidate_sample": len(a["cand"]), "longest_candidate_list": longest,
+            "in_degree_tv_distance": 0.5 * float(np.abs(h32 - h64).sum()) / max(1, int(h64.sum()))}
+
+
 def read_traffic(kernel, precision, neurons):
     """HBM bytes per search launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
     from the committed summary profiles/<kernel>_<precision>_<neurons>_traffic.txt, or None."""
@@ -486,7 +522,7 @@ def main(argv=None):
         },
     }
     for name, r in res.items():
-        out[name] = {k: v for k, v in r.items() if k not in ("targets", "counts")}
+        out[name] = {k: v for k, v in r.items() if k not in ("targets", "counts", "indeg", "cand")}
     if "fp64" in res and "fp32_mixed" in res:
         a, b = res["fp64"], res["fp32_mixed"]
         valid = a["targets"] != 0xFFFFFFFF
@@ -501,6 +537,7 @@ def main(argv=None):
             "mean_synapses_per_neuron_fp32": b["created0"] / args.neurons,
             "synapses_rel_diff": abs(b["created0"] - a["created0"]) / max(1, a["created0"]),
         }
+        out["fp32_vs_fp64"].update(precision_comparison(a, b))
     if "fp64_plain" in res and "fp32_mixed" in res:
         a, b = res["fp64_plain"], res["fp32_mixed"]
         valid = a["targets"] != 0xFFFFFFFF

@@ -173,6 +173,15 @@ int rl_bh_export_tree(rl_bh_ctx* ctx, uint8_t* level, uint8_t* nchild, uint32_t*
 /* Milliseconds of the last search kernel(s), measured with HIP events on the context's stream. */
 int rl_bh_last_search_ms(rl_bh_ctx* ctx, double* ms);
 
+/* Diagnostic: the level-1 candidate list of each given source after rl_bh_prepare_* (the accepted
+ * nodes of BarnesHutBase::get_nodes_to_consider at the root, BarnesHutBase.h:129-206, in list
+ * order) with each candidate's probability as the context's precision computes it
+ * (Kernel::calculate_attractiveness_to_connect, Kernel.h:61-75).  Source i's list is written to
+ * nodes/probabilities[i * capacity ...]; counts[i] is its full length (entries beyond capacity are
+ * not written). */
+int rl_bh_candidate_lists(rl_bh_ctx* ctx, uint64_t num_sources, const uint32_t* sources, uint64_t capacity,
+                          uint32_t* nodes, double* probabilities, uint32_t* counts);
+
 /* Counters of the last search: searches, requests, node_tests, node_tests_ref, ms_search and
  * ms_search_kernel are filled (created and the other times are 0). */
 int rl_bh_last_search_stats(rl_bh_ctx* ctx, rl_bh_step_stats* stats);

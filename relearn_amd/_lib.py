@@ -33,7 +33,7 @@ EXPORTED = [
     "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
     "rl_neurons_get", "rl_neurons_set_network_growth", "rl_bh_graph_create", "rl_bh_graph_launch",
     "rl_bh_graph_destroy", "rl_bh_set_tuning", "rl_bh_get_tuning", "rl_bh_autotune", "rl_net_init",
-    "rl_net_get_edges",
+    "rl_net_get_edges", "rl_bh_candidate_lists",
 ]
 
 
@@ -166,6 +166,7 @@ def load():
     L.rl_bh_graph_launch.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(StepStats)]
     L.rl_bh_graph_destroy.argtypes = [vp]
     L.rl_bh_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.rl_bh_candidate_lists.argtypes = [vp, C.c_uint64, _u32p, C.c_uint64, _u32p, _f64p, _u32p]
     L.rl_net_init.argtypes = [vp, C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32)]
     L.rl_net_get_edges.argtypes = [vp, C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32), _u64p]
     L.rl_bh_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -331,6 +332,22 @@ class BarnesHutHIP:
         self._check(self.L.rl_net_get_edges(self.h, m, _ptr(off, _u64p), _ptr(src, _u32p),
                                             _ptr(w, C.POINTER(C.c_int32)), C.byref(ne)))
         return off, src[:m], w[:m]
+
+    def candidate_lists(self, sources, capacity=4096):
+        """Level-1 candidate lists (nodes, probabilities) of the given sources after a prepare."""
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        k = len(src)
+        nodes = np.zeros(max(k * capacity, 1), np.uint32)
+        probs = np.zeros(max(k * capacity, 1))
+        counts = np.zeros(max(k, 1), np.uint32)
+        self._check(self.L.rl_bh_candidate_lists(self.h, k, _ptr(src, _u32p), capacity, _ptr(nodes, _u32p),
+                                                 _ptr(probs, _f64p), _ptr(counts, _u32p)))
+        out = []
+        for i in range(k):
+            c = min(int(counts[i]), capacity)
+            out.append((nodes[i * capacity:i * capacity + c].copy(), probs[i * capacity:i * capacity + c].copy(),
+                        int(counts[i])))
+        return out
 
     def set_tuning(self, search_waves_per_cu):
         t = Tuning()

@@ -694,6 +694,19 @@ int rl_neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fire
     return neurons_get(c, x, u, ca, fired, delta, grown, connected);
 }
 
+int rl_bh_candidate_lists(rl_bh_ctx* c, uint64_t num_sources, const uint32_t* sources, uint64_t capacity,
+                          uint32_t* nodes, double* probabilities, uint32_t* counts) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (num_sources > 0 && (sources == nullptr || counts == nullptr ||
+                            (capacity > 0 && (nodes == nullptr || probabilities == nullptr)))) {
+        return fail(c, RL_EINVAL, "rl_bh_candidate_lists: null array");
+    }
+    RL_TRY(set_device(c));
+    return candidate_lists(c, num_sources, sources, capacity, nodes, probabilities, counts);
+}
+
 int rl_bh_set_tuning(rl_bh_ctx* c, const rl_bh_tuning* t) {
     if (c == nullptr || t == nullptr) {
         return RL_EINVAL;

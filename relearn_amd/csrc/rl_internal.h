@@ -255,6 +255,8 @@ int search_capacity(rl_bh_ctx* c);
 int enqueue_search(rl_bh_ctx* c);
 int find_targets(rl_bh_ctx* c, uint32_t step);
 int read_counters(rl_bh_ctx* c, unsigned long long* h);
+int candidate_lists(rl_bh_ctx* c, uint64_t num_sources, const uint32_t* sources, uint64_t cap, uint32_t* nodes,
+                    double* probs, uint32_t* counts);
 struct SearchArgs;
 int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa);
 int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, const unsigned long long* count);

@@ -465,6 +465,42 @@ __global__ void k_scatter_active(uint64_t n, const uint32_t* __restrict__ flag, 
     }
 }
 
+// Writes the accepted nodes of a traversal with their probabilities, up to a capacity; counts all.
+struct RecordVisitor {
+    uint32_t* nodes;
+    double* probs;
+    uint64_t cap;
+    uint64_t count = 0;
+    __device__ __forceinline__ void operator()(uint32_t node, double p) {
+        if (count < cap) {
+            nodes[count] = node;
+            probs[count] = p;
+        }
+        count++;
+    }
+};
+
+// Level-1 candidate list of each listed source (one lane per source): the list the search sums and
+// picks from, with the probabilities of the context's precision.
+template <bool kMixed>
+__global__ void __launch_bounds__(kBlock) k_candidates(SearchArgs a, const uint32_t* __restrict__ sources,
+                                                       uint64_t count, uint64_t cap, uint32_t* __restrict__ nodes,
+                                                       double* __restrict__ probs, uint32_t* __restrict__ counts) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (t >= count) {
+        return;
+    }
+    Lane L;
+    lane_source(a, L, sources[t], kMixed);
+    RecordVisitor v;
+    v.nodes = nodes + t * cap;
+    v.probs = probs + t * cap;
+    v.cap = cap;
+    (void)traverse<kMixed>(a, L, L.root, kGauss, lds_stack + threadIdx.x, blockDim.x, v);
+    counts[t] = static_cast<uint32_t>(v.count);
+}
+
 __global__ void k_collect(unsigned long long* __restrict__ counters, const uint64_t* __restrict__ slots,
                           const uint64_t* __restrict__ requests, const uint64_t* __restrict__ active) {
     if (threadIdx.x == 0) {
@@ -521,40 +557,11 @@ int ensure_slots(rl_bh_ctx* c, uint64_t want) {
     return RL_OK;
 }
 
-// Sizes the slot buffers for the resident vacancies (one host read of their sum).
-int search_capacity(rl_bh_ctx* c) {
-    RL_TRY(scan_counts(c, c->vac[0], c->n, c->slot_off));
-    uint64_t total = 0;
-    RL_HIP(c, hipMemcpyAsync(&total, c->slot_off + c->n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
-    RL_HIP(c, hipStreamSynchronize(c->stream));
-    return ensure_slots(c, total);
-}
-
-// The search of one update on the context's stream, driven by counts in device memory: slot
-// offsets, the active sources in Morton order, the search kernels, the dense requests.  No host
-// synchronisation, allocation or memset, so it can be captured into a graph.  The update index is
-// read from d_steps[1].
-int enqueue_search(rl_bh_ctx* c) {
-    const uint64_t n = c->n;
-    hipStream_t s = c->stream;
-    if (n == 0) {
-        return RL_OK;
-    }
-    RL_TRY(scan_counts(c, c->vac[0], n, c->slot_off));
-    k_search_reset<<<kStreamBlocks, kBlock, 0, s>>>(n, c->slot_off + n, c->slots_cap, c->found, c->searched,
-                                                    c->slot_target, c->slot_tests, c->counters, c->coop_cursor);
-    RL_HIP(c, hipGetLastError());
-    k_active_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->counters, c->coop_flag);
-    RL_HIP(c, hipGetLastError());
-    RL_TRY(scan_counts(c, c->coop_flag, n, c->wide));
-    k_scatter_active<<<grid_for(n), kBlock, 0, s>>>(n, c->coop_flag, c->wide, c->order, c->coop_work);
-    RL_HIP(c, hipGetLastError());
-
+// The search arguments of the context's current tree, buffers and parameters.
+SearchArgs search_args(rl_bh_ctx* c) {
     SearchArgs a{};
-    a.n = n;
+    a.n = c->n;
     a.m = c->tree.m;
-    a.order = c->coop_work;
-    a.nwork = c->wide + n;
     a.pos = c->pos;
     a.qpos = c->qpos;
     a.signal = c->signal;
@@ -587,6 +594,41 @@ int enqueue_search(rl_bh_ctx* c) {
     a.counters = c->counters;
     a.redo = c->redo;
     a.n_redo = c->counters + kCtrRedo;
+    return a;
+}
+
+// Sizes the slot buffers for the resident vacancies (one host read of their sum).
+int search_capacity(rl_bh_ctx* c) {
+    RL_TRY(scan_counts(c, c->vac[0], c->n, c->slot_off));
+    uint64_t total = 0;
+    RL_HIP(c, hipMemcpyAsync(&total, c->slot_off + c->n, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    return ensure_slots(c, total);
+}
+
+// The search of one update on the context's stream, driven by counts in device memory: slot
+// offsets, the active sources in Morton order, the search kernels, the dense requests.  No host
+// synchronisation, allocation or memset, so it can be captured into a graph.  The update index is
+// read from d_steps[1].
+int enqueue_search(rl_bh_ctx* c) {
+    const uint64_t n = c->n;
+    hipStream_t s = c->stream;
+    if (n == 0) {
+        return RL_OK;
+    }
+    RL_TRY(scan_counts(c, c->vac[0], n, c->slot_off));
+    k_search_reset<<<kStreamBlocks, kBlock, 0, s>>>(n, c->slot_off + n, c->slots_cap, c->found, c->searched,
+                                                    c->slot_target, c->slot_tests, c->counters, c->coop_cursor);
+    RL_HIP(c, hipGetLastError());
+    k_active_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->counters, c->coop_flag);
+    RL_HIP(c, hipGetLastError());
+    RL_TRY(scan_counts(c, c->coop_flag, n, c->wide));
+    k_scatter_active<<<grid_for(n), kBlock, 0, s>>>(n, c->coop_flag, c->wide, c->order, c->coop_work);
+    RL_HIP(c, hipGetLastError());
+
+    SearchArgs a = search_args(c);
+    a.order = c->coop_work;
+    a.nwork = c->wide + n;
     const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
     const bool exact = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0;
     const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
@@ -670,6 +712,61 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     c->last_lane_sources = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0 ? h[kCtrRedo] : h[kCtrOverflow];
     c->searched_valid = true;
     return RL_OK;
+}
+
+int candidate_lists(rl_bh_ctx* c, uint64_t num_sources, const uint32_t* sources, uint64_t cap, uint32_t* nodes,
+                    double* probs, uint32_t* counts) {
+    if (!c->prepared) {
+        return fail(c, RL_ESTATE, "rl_bh_candidate_lists: prepare the update first");
+    }
+    for (uint64_t i = 0; i < num_sources; i++) {
+        if (sources[i] >= c->n) {
+            return fail(c, RL_EINVAL, "rl_bh_candidate_lists: source out of range");
+        }
+    }
+    if (num_sources == 0) {
+        return RL_OK;
+    }
+    hipStream_t s = c->stream;
+    uint32_t* d_src = nullptr;
+    uint32_t* d_nodes = nullptr;
+    double* d_probs = nullptr;
+    uint32_t* d_counts = nullptr;
+    const uint64_t total = num_sources * std::max<uint64_t>(cap, 1);
+    int status = RL_OK;
+    do {
+        if ((status = dalloc(c, &d_src, num_sources)) != RL_OK) break;
+        if ((status = dalloc(c, &d_nodes, total)) != RL_OK) break;
+        if ((status = dalloc(c, &d_probs, total)) != RL_OK) break;
+        if ((status = dalloc(c, &d_counts, num_sources)) != RL_OK) break;
+        hipError_t e = hipMemcpyAsync(d_src, sources, num_sources * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            status = hip_fail(c, e, "hipMemcpyAsync");
+            break;
+        }
+        const SearchArgs a = search_args(c);
+        const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
+        if (c->cfg.precision == RL_PREC_FP32_MIXED) {
+            k_candidates<true><<<grid_for(num_sources), kBlock, lds, s>>>(a, d_src, num_sources, cap, d_nodes, d_probs,
+                                                                           d_counts);
+        } else {
+            k_candidates<false><<<grid_for(num_sources), kBlock, lds, s>>>(a, d_src, num_sources, cap, d_nodes, d_probs,
+                                                                            d_counts);
+        }
+        if ((e = hipGetLastError()) != hipSuccess ||
+            (e = hipMemcpyAsync(counts, d_counts, num_sources * sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (cap > 0 && (e = hipMemcpyAsync(nodes, d_nodes, num_sources * cap * sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) ||
+            (cap > 0 && (e = hipMemcpyAsync(probs, d_probs, num_sources * cap * sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) ||
+            (e = hipStreamSynchronize(s)) != hipSuccess) {
+            status = hip_fail(c, e, "candidate lists");
+        }
+    } while (false);
+    (void)hipStreamSynchronize(s);
+    dfree(&d_src);
+    dfree(&d_nodes);
+    dfree(&d_probs);
+    dfree(&d_counts);
+    return status;
 }
 
 }  // namespace rl
