@@ -72,6 +72,7 @@ typedef struct {
     double ms_commit;           /* conflict resolution + vacancy commit */
     double ms_search_kernel;    /* the dominant search kernel alone (cooperative or per-lane) */
     double ms_total;            /* the whole update, or the whole replayed period (rl_bh_graph_launch) */
+    uint64_t deleted;           /* synapses deleted by the period's element commit (rl_bh_graph_launch) */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a
@@ -218,8 +219,16 @@ int rl_neurons_step(rl_bh_ctx* ctx, uint32_t step, const double* input);
 
 /* ElementBaseBase::update (ElementBaseBase.h:420-487) for the three element types: applies the
  * deltas to grown/vacant/connected counts at a plasticity step.  deleted (3*n, may be NULL) gets the
- * number of bound elements each neuron must delete per type. */
+ * number of bound elements each neuron lost per type.  With a network graph (rl_net_init) the
+ * synapses those elements held are deleted as SynapseDeletionFinderCPU::delete_synapses does
+ * (SynapseDeletionFinderCPU.cpp): the types are committed in the order axons, excitatory, inhibitory
+ * dendrites, each followed by drawing the lost number of synapse units uniformly without
+ * replacement (RandomSynapseDeletionFinder, SynapseDeletionFinder.cpp:53-69; counter-based draw
+ * addressed by plasticity update `update`), lowering the pairs' weights and disconnecting one element
+ * of each partner (ElementBaseBase::disconnect_elements, ElementBaseBase.h:311-319).
+ * rl_neurons_commit_elements uses update index 0. */
 int rl_neurons_commit_elements(rl_bh_ctx* ctx, uint32_t* deleted);
+int rl_neurons_commit_elements_at(rl_bh_ctx* ctx, uint32_t update, uint32_t* deleted);
 
 /* Growth rates per network of an ensemble (nu[3g + t] for element type t of network g: axons,
  * excitatory, inhibitory dendrites), replacing rl_step_params.nu for those networks; NULL restores

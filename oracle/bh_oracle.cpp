@@ -32,6 +32,7 @@
 #include <cstring>
 #include <deque>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <random>
 #include <vector>
@@ -522,6 +523,16 @@ uint32_t pick_draw(uint64_t seed, uint32_t neuron, uint32_t step, uint32_t j, ui
 
 uint64_t commit_priority(uint64_t seed, uint32_t source, uint32_t step, uint32_t j) {
     const uint32_t ctr[4] = { source, step, j, 2U };
+    const uint32_t key[2] = { static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32U) };
+    uint32_t out[4];
+    philox4x32_10(ctr, key, out);
+    return (static_cast<uint64_t>(out[0]) << 32U) | out[1];
+}
+
+// Deletion key of synapse unit `copy` between `neuron` and `partner` for element `type` (the same
+// counter-based draw as the device; the reference draws indices with RandomHolder, parity unpinned).
+uint64_t deletion_key(uint64_t seed, uint32_t neuron, uint32_t step, uint32_t partner, uint32_t type, uint32_t copy) {
+    const uint32_t ctr[4] = { neuron, step, partner, (copy << 4U) | (type << 2U) };
     const uint32_t key[2] = { static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32U) };
     uint32_t out[4];
     philox4x32_10(ctr, key, out);
@@ -1115,6 +1126,10 @@ uint32_t orc_pick_draw(uint64_t seed, uint32_t neuron, uint32_t step, uint32_t j
     return pick_draw(seed, neuron, step, j, level);
 }
 
+uint64_t orc_deletion_key(uint64_t seed, uint32_t neuron, uint32_t step, uint32_t partner, uint32_t type, uint32_t copy) {
+    return deletion_key(seed, neuron, step, partner, type, copy);
+}
+
 uint64_t orc_commit_priority(uint64_t seed, uint32_t source, uint32_t step, uint32_t j) {
     return commit_priority(seed, source, step, j);
 }
@@ -1325,6 +1340,164 @@ int orc_synaptic_input(uint64_t n, const uint64_t* in_off, const uint32_t* in_sr
         }
         total += synaptic;
         out[id] = total;
+    }
+    return ORC_OK;
+}
+
+// Local plastic part of NetworkGraph (source/neurons/NetworkGraph.h): per neuron its in-edges
+// (source -> weight) and out-edges (target -> weight); add_synapse adds the weight to the pair and
+// erases a pair whose weight becomes 0.
+struct orc_graph {
+    uint64_t n = 0;
+    std::vector<std::map<uint32_t, int32_t>> in, out;
+
+    void add(uint32_t target, uint32_t source, int32_t weight) {
+        auto add_one = [](std::map<uint32_t, int32_t>& m, uint32_t key, int32_t w) {
+            const int32_t now = (m[key] += w);
+            if (now == 0) {
+                m.erase(key);
+            }
+        };
+        add_one(in[target], source, weight);
+        add_one(out[source], target, weight);
+    }
+};
+
+int orc_graph_create(uint64_t n, const uint64_t* in_off, const uint32_t* src, const int32_t* w, orc_graph** out) {
+    if (out == nullptr) {
+        return ORC_EINVAL;
+    }
+    auto* g = new orc_graph();
+    g->n = n;
+    g->in.resize(n);
+    g->out.resize(n);
+    if (in_off != nullptr) {
+        for (uint64_t t = 0; t < n; t++) {
+            for (uint64_t e = in_off[t]; e < in_off[t + 1]; e++) {
+                if (src[e] >= n) {
+                    delete g;
+                    return ORC_EINVAL;
+                }
+                if (w[e] != 0) {
+                    g->add(static_cast<uint32_t>(t), src[e], w[e]);
+                }
+            }
+        }
+    }
+    *out = g;
+    return ORC_OK;
+}
+
+void orc_graph_free(orc_graph* g) {
+    delete g;
+}
+
+int orc_graph_add(orc_graph* g, uint64_t m, const uint32_t* target, const uint32_t* source, const int32_t* weight) {
+    for (uint64_t i = 0; i < m; i++) {
+        if (target[i] >= g->n || source[i] >= g->n) {
+            return ORC_EINVAL;
+        }
+        g->add(target[i], source[i], weight[i]);
+    }
+    return ORC_OK;
+}
+
+uint64_t orc_graph_num_edges(const orc_graph* g) {
+    uint64_t e = 0;
+    for (const auto& m : g->in) {
+        e += m.size();
+    }
+    return e;
+}
+
+int orc_graph_export(const orc_graph* g, uint64_t cap, uint64_t* off, uint32_t* src, int32_t* w) {
+    uint64_t e = 0;
+    off[0] = 0;
+    for (uint64_t t = 0; t < g->n; t++) {
+        for (const auto& [s, wt] : g->in[t]) {
+            if (e >= cap) {
+                return ORC_EINVAL;
+            }
+            src[e] = s;
+            w[e] = wt;
+            e++;
+        }
+        off[t + 1] = e;
+    }
+    return ORC_OK;
+}
+
+// SynapseDeletionFinderCPU::find_synapses_to_delete + commit_deletions
+// (source/neurons/helper/SynapseDeletionFinderCPU.cpp) for one element type, single rank: every
+// neuron with to_delete[i] > 0 lists its synapse units (SynapseDeletionFinderBase::register_synapses,
+// SynapseDeletionFinderBase.cpp:33-151: axons its out-edges, dendrites its in-edges of the dendrite's
+// sign, |weight| units per pair), draws to_delete[i] of them without replacement
+// (RandomSynapseDeletionFinder::find_synapses_on_neuron, SynapseDeletionFinder.cpp:53-69; here the
+// units with the smallest deletion_key), and all draws are made before any is committed.  Each
+// committed unit lowers its pair's weight by one and disconnects one element of the partner
+// (ElementBaseBase::disconnect_elements, ElementBaseBase.h:311-319).  Neuron ids inside the draw are
+// local to the neuron's network.
+int orc_delete_synapses(orc_graph* g, int type, const uint32_t* to_delete, const uint32_t* net_of, const uint64_t* net_off,
+                        const uint64_t* net_seed, uint32_t step, uint32_t* conn0, uint32_t* conn1, uint32_t* conn2,
+                        uint32_t* vac0, uint32_t* vac1, uint32_t* vac2, const uint8_t* signal, uint64_t* deleted_total) {
+    if (g == nullptr || type < 0 || type > 2) {
+        return ORC_EINVAL;
+    }
+    uint32_t* conn[3] = { conn0, conn1, conn2 };
+    uint32_t* vac[3] = { vac0, vac1, vac2 };
+    struct Request {
+        uint32_t initiator, partner;
+    };
+    std::vector<Request> requests;
+    auto local = [&](uint32_t i) { return i - static_cast<uint32_t>(net_off[net_of[i]]); };
+    for (uint64_t i = 0; i < g->n; i++) {
+        const uint32_t want = to_delete[i];
+        if (want == 0) {
+            continue;
+        }
+        struct U {
+            uint64_t key;
+            uint32_t partner_local, copy, partner;
+        };
+        std::vector<U> units;
+        const auto& edges = type == 0 ? g->out[i] : g->in[i];
+        for (const auto& [p, w] : edges) {
+            if (type == 1 && w < 0) continue;   // searching excitatory synapses, found an inhibitory one
+            if (type == 2 && w > 0) continue;
+            const uint32_t aw = static_cast<uint32_t>(w < 0 ? -w : w);
+            for (uint32_t c = 0; c < aw; c++) {
+                units.push_back({ deletion_key(net_seed[net_of[i]], local(static_cast<uint32_t>(i)), step, local(p),
+                                               static_cast<uint32_t>(type), c), local(p), c, p });
+            }
+        }
+        if (want > units.size()) {
+            return ORC_EINTERNAL;  // the reference throws: more deletions than synapses
+        }
+        std::sort(units.begin(), units.end(), [](const U& a, const U& b) {
+            if (a.key != b.key) return a.key < b.key;
+            if (a.partner_local != b.partner_local) return a.partner_local < b.partner_local;
+            return a.copy < b.copy;
+        });
+        for (uint32_t k = 0; k < want; k++) {
+            requests.push_back({ static_cast<uint32_t>(i), units[k].partner });
+        }
+    }
+    for (const auto& r : requests) {
+        if (type == 0) {
+            const int32_t weight = signal[r.initiator] == 0 ? -1 : 1;   // excitatory pair loses +1
+            g->add(r.partner, r.initiator, weight);
+            const int dt = signal[r.initiator] == 0 ? 1 : 2;
+            conn[dt][r.partner] -= 1;
+            vac[dt][r.partner] += 1;
+        } else {
+            const int32_t weight = type == 1 ? -1 : 1;
+            g->add(r.initiator, r.partner, weight);
+            conn[0][r.partner] -= 1;
+            vac[0][r.partner] += 1;
+        }
+    }
+    if (deleted_total != nullptr) {
+        *deleted_total = requests.size();
     }
     return ORC_OK;
 }

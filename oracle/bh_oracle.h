@@ -169,6 +169,19 @@ int orc_neuron_step(uint64_t n, const orc_step_params* p, double* x, double* u, 
 int orc_synaptic_input(uint64_t n, const uint64_t* in_off, const uint32_t* in_src, const int32_t* in_w,
                        const uint8_t* fired, double conductance, const double* background, double* out);
 
+/* Network graph (NetworkGraph's local plastic edges, pairs merged, zero weights erased) and synapse
+ * deletion of one element type (SynapseDeletionFinderCPU, counter-based draw). */
+typedef struct orc_graph orc_graph;
+int orc_graph_create(uint64_t n, const uint64_t* in_off, const uint32_t* src, const int32_t* w, orc_graph** out);
+void orc_graph_free(orc_graph* g);
+int orc_graph_add(orc_graph* g, uint64_t m, const uint32_t* target, const uint32_t* source, const int32_t* weight);
+uint64_t orc_graph_num_edges(const orc_graph* g);
+int orc_graph_export(const orc_graph* g, uint64_t cap, uint64_t* off, uint32_t* src, int32_t* w);
+int orc_delete_synapses(orc_graph* g, int type, const uint32_t* to_delete, const uint32_t* net_of, const uint64_t* net_off,
+                        const uint64_t* net_seed, uint32_t step, uint32_t* conn0, uint32_t* conn1, uint32_t* conn2,
+                        uint32_t* vac0, uint32_t* vac1, uint32_t* vac2, const uint8_t* signal, uint64_t* deleted_total);
+uint64_t orc_deletion_key(uint64_t seed, uint32_t neuron, uint32_t step, uint32_t partner, uint32_t type, uint32_t copy);
+
 int orc_elements_update(uint64_t n, double* grown, double* delta, uint32_t* connected, uint32_t* vacant,
                         const double* retract_ratio, uint32_t* out_deleted);
 

@@ -132,6 +132,21 @@ def lib():
         L.orc_neuron_step.argtypes = [C.c_uint64, C.POINTER(StepParams), _f64p, _f64p, _f64p, _f64p, _u8p, _f64p,
                                       _f64p, _u8p]
         L.orc_neuron_step.restype = C.c_int
+        L.orc_graph_create.argtypes = [C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32), C.POINTER(C.c_void_p)]
+        L.orc_graph_create.restype = C.c_int
+        L.orc_graph_free.argtypes = [C.c_void_p]
+        L.orc_graph_free.restype = None
+        L.orc_graph_add.argtypes = [C.c_void_p, C.c_uint64, _u32p, _u32p, C.POINTER(C.c_int32)]
+        L.orc_graph_add.restype = C.c_int
+        L.orc_graph_num_edges.argtypes = [C.c_void_p]
+        L.orc_graph_num_edges.restype = C.c_uint64
+        L.orc_graph_export.argtypes = [C.c_void_p, C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32)]
+        L.orc_graph_export.restype = C.c_int
+        L.orc_delete_synapses.argtypes = [C.c_void_p, C.c_int, _u32p, _u32p, _u64p, _u64p, C.c_uint32,
+                                          _u32p, _u32p, _u32p, _u32p, _u32p, _u32p, _u8p, _u64p]
+        L.orc_delete_synapses.restype = C.c_int
+        L.orc_deletion_key.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_deletion_key.restype = C.c_uint64
         L.orc_synaptic_input.argtypes = [C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32), _u8p, C.c_double, _f64p,
                                          _f64p]
         L.orc_synaptic_input.restype = C.c_int
@@ -340,6 +355,65 @@ def neuron_step(params: StepParams, x, u, ca, inp, fired, target_ca, delta, enab
     _check(lib().orc_neuron_step(n, C.byref(params), _ptr(x, _f64p), _ptr(u, _f64p), _ptr(ca, _f64p),
                                  _ptr(inp, _f64p), _ptr(fired, _u8p), _ptr(target_ca, _f64p), _ptr(delta, _f64p),
                                  _ptr(en, _u8p)), "orc_neuron_step")
+
+
+class Graph:
+    """Network graph restatement: pairs merged, zero weights erased, in- and out-edges per neuron."""
+
+    def __init__(self, n, in_off=None, src=None, w=None):
+        self.n = n
+        off = None if in_off is None else np.ascontiguousarray(in_off, dtype=np.uint64)
+        s = None if src is None else np.ascontiguousarray(src, dtype=np.uint32)
+        ww = None if w is None else np.ascontiguousarray(w, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().orc_graph_create(n, _ptr(off, _u64p), _ptr(s, _u32p), _ptr(ww, C.POINTER(C.c_int32)),
+                                      C.byref(h)), "orc_graph_create")
+        self.h = h
+
+    def add(self, target, source, weight):
+        t = np.ascontiguousarray(target, dtype=np.uint32)
+        s = np.ascontiguousarray(source, dtype=np.uint32)
+        w = np.ascontiguousarray(weight, dtype=np.int32)
+        _check(lib().orc_graph_add(self.h, len(t), _ptr(t, _u32p), _ptr(s, _u32p), _ptr(w, C.POINTER(C.c_int32))),
+               "orc_graph_add")
+
+    def export(self):
+        m = int(lib().orc_graph_num_edges(self.h))
+        off = np.zeros(self.n + 1, np.uint64)
+        src = np.zeros(max(m, 1), np.uint32)
+        w = np.zeros(max(m, 1), np.int32)
+        _check(lib().orc_graph_export(self.h, m, _ptr(off, _u64p), _ptr(src, _u32p), _ptr(w, C.POINTER(C.c_int32))),
+               "orc_graph_export")
+        return off, src[:m], w[:m]
+
+    def delete(self, etype, to_delete, signal, conn, vac, *, step, seed=0, net_of=None, net_off=None, net_seed=None):
+        """Deletes the synapses of element type etype (0 axon, 1 exc, 2 inh dendrite); conn and vac are
+        (3, n) uint32 arrays updated in place.  Returns the number of deleted units."""
+        n = self.n
+        td = np.ascontiguousarray(to_delete, dtype=np.uint32)
+        no = np.zeros(n, np.uint32) if net_of is None else np.ascontiguousarray(net_of, dtype=np.uint32)
+        nf = np.zeros(1, np.uint64) if net_off is None else np.ascontiguousarray(net_off, dtype=np.uint64)
+        ns = np.array([seed], np.uint64) if net_seed is None else np.ascontiguousarray(net_seed, dtype=np.uint64)
+        sig = np.ascontiguousarray(signal, dtype=np.uint8)
+        assert conn.dtype == np.uint32 and vac.dtype == np.uint32 and conn.flags.c_contiguous and vac.flags.c_contiguous
+        tot = C.c_uint64()
+        _check(lib().orc_delete_synapses(self.h, int(etype), _ptr(td, _u32p), _ptr(no, _u32p), _ptr(nf, _u64p),
+                                         _ptr(ns, _u64p), int(step), _ptr(conn[0], _u32p), _ptr(conn[1], _u32p),
+                                         _ptr(conn[2], _u32p), _ptr(vac[0], _u32p), _ptr(vac[1], _u32p),
+                                         _ptr(vac[2], _u32p), _ptr(sig, _u8p), C.byref(tot)), "orc_delete_synapses")
+        return int(tot.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orc_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def deletion_key(seed, neuron, step, partner, etype, copy):
+    return int(lib().orc_deletion_key(seed, neuron, step, partner, etype, copy))
 
 
 def synaptic_input(in_off, in_src, in_w, fired, conductance, background=None):

@@ -33,7 +33,7 @@ EXPORTED = [
     "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
     "rl_neurons_get", "rl_neurons_set_network_growth", "rl_bh_graph_create", "rl_bh_graph_launch",
     "rl_bh_graph_destroy", "rl_bh_set_tuning", "rl_bh_get_tuning", "rl_bh_autotune", "rl_net_init",
-    "rl_net_get_edges", "rl_bh_candidate_lists",
+    "rl_net_get_edges", "rl_bh_candidate_lists", "rl_neurons_commit_elements_at",
 ]
 
 
@@ -64,6 +64,7 @@ class StepStats(C.Structure):
         ("ms_commit", C.c_double),
         ("ms_search_kernel", C.c_double),
         ("ms_total", C.c_double),
+        ("deleted", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -161,6 +162,7 @@ def load():
     L.rl_neurons_init.argtypes = [vp, C.POINTER(StepParams), _f64p, _f64p, _f64p, _f64p, _f64p, _u32p]
     L.rl_neurons_step.argtypes = [vp, C.c_uint32, _f64p]
     L.rl_neurons_commit_elements.argtypes = [vp, _u32p]
+    L.rl_neurons_commit_elements_at.argtypes = [vp, C.c_uint32, _u32p]
     L.rl_neurons_get.argtypes = [vp, _f64p, _f64p, _f64p, _u8p, _f64p, _f64p, _u32p]
     L.rl_bh_graph_create.argtypes = [vp, C.c_uint32, C.c_uint64]
     L.rl_bh_graph_launch.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(StepStats)]
@@ -437,9 +439,11 @@ class BarnesHutHIP:
         i = None if inp is None else np.ascontiguousarray(inp, dtype=np.float64)
         self._check(self.L.rl_neurons_step(self.h, step, _ptr(i, _f64p)))
 
-    def neurons_commit_elements(self):
+    def neurons_commit_elements(self, update=0):
+        """Element commit of the three types (with a network graph, also the synapse deletions drawn
+        for plasticity update `update`); returns the unbound elements per type and neuron."""
         d = np.zeros(max(3 * self.n, 1), np.uint32)
-        self._check(self.L.rl_neurons_commit_elements(self.h, _ptr(d, _u32p)))
+        self._check(self.L.rl_neurons_commit_elements_at(self.h, int(update), _ptr(d, _u32p)))
         return d[:3 * self.n].reshape(3, self.n)
 
     def neurons_get(self):

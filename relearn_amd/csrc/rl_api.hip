@@ -677,12 +677,16 @@ int rl_neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
 }
 
 int rl_neurons_commit_elements(rl_bh_ctx* c, uint32_t* deleted) {
+    return rl_neurons_commit_elements_at(c, 0, deleted);
+}
+
+int rl_neurons_commit_elements_at(rl_bh_ctx* c, uint32_t update, uint32_t* deleted) {
     if (c == nullptr) {
         return RL_EINVAL;
     }
     RL_TRY(set_device(c));
     c->prepared = false;
-    return neurons_commit(c, deleted);
+    return neurons_commit(c, update, deleted);
 }
 
 int rl_neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
@@ -850,6 +854,7 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     stats->created = h[kCtrCreated];
     stats->node_tests = h[kCtrTestsExec];
     stats->node_tests_ref = h[kCtrTestsRef];
+    stats->deleted = h[kCtrDeleted];
     stats->ms_total = ms;
     c->slots = h[kCtrSlots];
     c->nreq = h[kCtrRequests];
@@ -867,6 +872,10 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     }
     if (h[kCtrSticky] != 0) {
         return fail(c, RL_ELIMIT, "rl_bh_graph_launch: an earlier replay without stats exceeded a capacity");
+    }
+    if (h[kCtrDelShort] != 0) {
+        return fail(c, RL_EINVAL, "rl_bh_graph_launch: the network graph held fewer synapses than the element "
+                                  "commit unbound (connected counts and graph disagree)");
     }
     return RL_OK;
 }

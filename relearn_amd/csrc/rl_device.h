@@ -59,6 +59,18 @@ __device__ __forceinline__ uint64_t commit_priority(uint64_t seed, uint32_t sour
     return (static_cast<uint64_t>(o.x) << 32U) | o.y;
 }
 
+// 64-bit key of synapse unit `copy` between `neuron` and `partner` (ids inside their network) when
+// `neuron` deletes `type` elements (0 axon, 1 excitatory, 2 inhibitory dendrite) in plasticity update
+// `step`: the deleted units are the ones with the smallest keys, a uniform random subset whatever
+// the order the synapses are stored in.  The low two bits of the last counter word are 0, so the
+// stream never meets the tags 1-3 of the other draws.
+__device__ __forceinline__ uint64_t deletion_key(uint64_t seed, uint32_t neuron, uint32_t step, uint32_t partner,
+                                                 uint32_t type, uint32_t copy) {
+    const uint4 o = philox4x32_10(make_uint4(neuron, step, partner, (copy << 4U) | (type << 2U)),
+                                  make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32U)));
+    return (static_cast<uint64_t>(o.x) << 32U) | o.y;
+}
+
 // uniform_real_distribution(0, next) of a 32-bit engine: (k / 2^32) * (next - 0) + 0
 // (boost::random::detail::generate_uniform_real; ProbabilityPicker.h:108-109).
 __device__ __forceinline__ double draw_to_uniform(uint32_t k, double next) {

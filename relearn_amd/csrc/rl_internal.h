@@ -37,6 +37,8 @@ enum : int {
     kCtrSticky = 11,     // over-capacity seen since the host last read the counters
     kCtrNetOver = 12,    // 1: the created synapses did not fit the network graph, none was appended
     kCtrNetNew = 13,     // synapses appended to the network graph by the last commit
+    kCtrDeleted = 14,    // synapse units deleted by the last element commit
+    kCtrDelShort = 15,   // deletions that found fewer synapses than the element commit asked for
     kCounters = 16
 };
 
@@ -76,8 +78,11 @@ struct NetGraph {
     uint64_t* off2 = nullptr;      // n + 1: merge buffers
     uint32_t* src2 = nullptr;
     int32_t* w2 = nullptr;
-    uint32_t* ncnt = nullptr;      // n: synapses appended per target
-    uint64_t* npre = nullptr;      // n + 1: their prefix sum
+    uint32_t* ncnt = nullptr;      // n: synapses appended per target, then merged entries per target
+    uint64_t* npre = nullptr;      // n + 1: prefix sums of ncnt
+    uint32_t* ocnt = nullptr;      // n: out-entries per source of a deletion phase
+    uint64_t* ooff = nullptr;      // n + 1
+    uint2* olist = nullptr;        // cap: (entry, target) of each out-entry, grouped by source
     uint64_t cap = 0;
     bool ready = false;
 };
@@ -272,7 +277,7 @@ int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input);
 int enqueue_neuron_steps(rl_bh_ctx* c, uint32_t steps);  // steps d_steps[0] + k, k < steps
 int enqueue_elements_commit(rl_bh_ctx* c);
 int normalize_fired(rl_bh_ctx* c);      // the last fired status into the buffer a captured period reads
-int neurons_commit(rl_bh_ctx* c, uint32_t* deleted);
+int neurons_commit(rl_bh_ctx* c, uint32_t update, uint32_t* deleted);
 int neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
                 uint32_t* connected);
 int neurons_set_network_growth(rl_bh_ctx* c, const double* nu);
@@ -284,6 +289,7 @@ int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const 
 int net_get(rl_bh_ctx* c, uint64_t capacity, uint64_t* in_offsets, uint32_t* sources, int32_t* weights,
             uint64_t* num_edges);
 int enqueue_net_append(rl_bh_ctx* c);   // appends the accepted requests of the last commit
+int enqueue_net_delete(rl_bh_ctx* c, int type);  // deletes the synapses of the type's shrunken elements
 int net_append(rl_bh_ctx* c);           // eager: grows the capacity if needed, then appends
 void net_free(rl_bh_ctx* c);
 

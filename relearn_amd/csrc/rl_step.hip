@@ -136,13 +136,14 @@ __global__ void k_step(StepArgs a) {
     }
 }
 
-__global__ void k_elements_update(uint64_t n, double* __restrict__ grown, double* __restrict__ delta,
-                                  uint32_t* __restrict__ c0, uint32_t* __restrict__ c1, uint32_t* __restrict__ c2,
-                                  uint32_t* __restrict__ v0, uint32_t* __restrict__ v1, uint32_t* __restrict__ v2,
-                                  uint32_t* __restrict__ deleted) {
+// Element types [k0 / n, k1 / n) of every neuron: index k = type * n + neuron.
+__global__ void k_elements_update(uint64_t n, uint64_t k0, uint64_t k1, double* __restrict__ grown,
+                                  double* __restrict__ delta, uint32_t* __restrict__ c0, uint32_t* __restrict__ c1,
+                                  uint32_t* __restrict__ c2, uint32_t* __restrict__ v0, uint32_t* __restrict__ v1,
+                                  uint32_t* __restrict__ v2, uint32_t* __restrict__ deleted) {
 #pragma clang fp contract(off)
-    const uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (k >= 3 * n) {
+    const uint64_t k = k0 + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (k >= k1) {
         return;
     }
     const int t = static_cast<int>(k / n);
@@ -386,6 +387,11 @@ int enqueue_neuron_steps(rl_bh_ctx* c, uint32_t steps) {
     return RL_OK;
 }
 
+// ElementBaseBase::update for the three element types.  With a network graph the types are committed
+// one after the other, each followed by the deletion of the synapses its shrunken elements held
+// (SynapseDeletionFinderCPU::delete_synapses: axons, then excitatory, then inhibitory dendrites, each
+// seeing the disconnections of the previous ones); the deletion draws use the update index in
+// d_steps[1].  Without a graph the deletions are only counted (neurons.deleted).
 int enqueue_elements_commit(rl_bh_ctx* c) {
     Neurons& s = c->neurons;
     if (!s.ready) {
@@ -395,13 +401,27 @@ int enqueue_elements_commit(rl_bh_ctx* c) {
     if (n == 0) {
         return RL_OK;
     }
-    k_elements_update<<<grid_for(3 * n), kBlock, 0, c->stream>>>(n, s.grown, s.delta, c->conn[0], c->conn[1], c->conn[2],
-                                                                  c->vac[0], c->vac[1], c->vac[2], s.deleted);
-    RL_HIP(c, hipGetLastError());
+    if (!c->net.ready) {
+        k_elements_update<<<grid_for(3 * n), kBlock, 0, c->stream>>>(n, 0, 3 * n, s.grown, s.delta, c->conn[0], c->conn[1],
+                                                                      c->conn[2], c->vac[0], c->vac[1], c->vac[2], s.deleted);
+        RL_HIP(c, hipGetLastError());
+        return RL_OK;
+    }
+    RL_TRY(fill_u32(c, reinterpret_cast<uint32_t*>(c->counters + kCtrDeleted), 0U, 4, nullptr, 0));
+    for (int t = 0; t < 3; t++) {
+        k_elements_update<<<grid_for(n), kBlock, 0, c->stream>>>(n, t * n, (t + 1) * n, s.grown, s.delta, c->conn[0],
+                                                                  c->conn[1], c->conn[2], c->vac[0], c->vac[1], c->vac[2],
+                                                                  s.deleted);
+        RL_HIP(c, hipGetLastError());
+        RL_TRY(enqueue_net_delete(c, t));
+    }
     return RL_OK;
 }
 
-int neurons_commit(rl_bh_ctx* c, uint32_t* deleted) {
+int neurons_commit(rl_bh_ctx* c, uint32_t update, uint32_t* deleted) {
+    if (c->neurons.ready && c->n > 0) {
+        RL_TRY(set_steps(c, 0, update));
+    }
     RL_TRY(enqueue_elements_commit(c));
     if (deleted != nullptr && c->n > 0) {
         RL_HIP(c, hipMemcpyAsync(deleted, c->neurons.deleted, 3 * c->n * sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -1,0 +1,97 @@
+"""Synapse deletion on the device (element commit with a network graph) against the oracle.
+
+The state is a random pair-merged network whose connected counts match it.  A few neuron steps with
+calcium far above the target give every element type negative deltas, so the element commit unbinds
+elements of all three types.  The device commits the types in the reference's order (axons,
+excitatory, inhibitory dendrites), each followed by the deletion of the unbound synapses; the oracle
+does the same with ElementBaseBase::update and its SynapseDeletionFinderCPU restatement, under the
+same counter-based draw.  Bar: identical unbound counts, connected and vacant counts, grown counts
+(to 1e-9 relative, from FP64 deltas) and network graph (pairs and weights).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from relearn_amd import RL_PREC_FP64, BarnesHutHIP, StepParams
+from relearn_amd.workloads import uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def _pairs(off, src, w):
+    t = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off.astype(np.int64)))
+    out = {}
+    for a, b, c in zip(t.tolist(), src.tolist(), w.tolist()):
+        out[(a, b)] = out.get((a, b), 0) + c
+    return {k: v for k, v in out.items() if v != 0}
+
+
+def _network(n, m, signal, seed):
+    rng = np.random.default_rng(seed)
+    tgt = rng.integers(0, n, m)
+    src = rng.integers(0, n, m)
+    w = np.where(signal[src] != 0, -1, 1).astype(np.int32)
+    order = np.argsort(tgt, kind="stable")
+    off = np.concatenate([[0], np.cumsum(np.bincount(tgt, minlength=n))]).astype(np.uint64)
+    return off, src[order].astype(np.uint32), w[order]
+
+
+@pytest.mark.parametrize("update", [0, 7])
+def test_deletion_matches_oracle(oracle, update):
+    n, seed = 4000, 23
+    print("[gpu-case] deletion vs oracle, update", update, file=sys.stderr, flush=True)
+    net = uniform_cube(n, 1500.0, 61)
+    off, src, w = _network(n, 12 * n, net.signal, 62)
+    og = oracle.Graph(n, off, src, w)
+    eo, es, ew = og.export()
+    t_of = np.repeat(np.arange(n), np.diff(eo.astype(np.int64)))
+    conn = np.zeros((3, n), np.uint32)
+    np.add.at(conn[0], es, np.abs(ew))
+    np.add.at(conn[1], t_of[ew > 0], ew[ew > 0])
+    np.add.at(conn[2], t_of[ew < 0], -ew[ew < 0])
+    rng = np.random.default_rng(63)
+    grown = conn.astype(np.float64) + rng.uniform(0.0, 1.5, (3, n))
+    # calcium far above the target: every growth curve is near -1, deltas of about -3 * nu
+    p = StepParams.izhikevich(input_mode=0, nu=0.4)
+    for t in range(3):
+        p.eta[t] = 0.0
+    ca = np.full(n, 0.9)
+    tca = np.full(n, 0.1)
+    g = BarnesHutHIP(net.box_min, net.box_max, seed=seed, precision=RL_PREC_FP64)
+    g.init(net.xyz, net.signal)
+    g.net_init(len(es) + 10, off, src, w)
+    g.neurons_init(p, tca, ca=ca, grown=grown.ravel(), connected=conn.ravel())
+    po = oracle.StepParams()
+    for f in ("a", "b", "c", "d", "v_spike", "k1", "k2", "k3", "h", "tau_ca", "beta_ca"):
+        setattr(po, f, getattr(p, f))
+    for t in range(3):
+        po.eta[t], po.nu[t] = p.eta[t], p.nu[t]
+    xo, uo, co = np.zeros(n), np.zeros(n), ca.copy()
+    fo, do = np.zeros(n, np.uint8), np.zeros(3 * n)
+    for s in range(3):
+        bg = rng.normal(5.0, 2.0, n)
+        oracle.neuron_step(po, xo, uo, co, bg, fo, tca, do, None)
+        g.neurons_step(s, bg)
+    deleted = g.neurons_commit_elements(update)
+    # oracle: element commit of one type, then its deletions, type by type
+    go, dv, vac = grown.copy(), do.reshape(3, n).copy(), np.zeros((3, n), np.uint32)
+    cref, vref = conn.copy(), vac.copy()
+    dref = np.zeros((3, n), np.uint32)
+    total = 0
+    for t in range(3):
+        ct, vt = cref[t].copy(), vref[t].copy()
+        dref[t] = oracle.elements_update(go[t], dv[t], ct, vt)
+        cref[t], vref[t] = ct, vt
+        total += og.delete(t, dref[t], net.signal, cref, vref, step=update, seed=seed)
+    assert total > 0 and int(dref[1].sum()) > 0 and int(dref[2].sum()) > 0
+    assert np.array_equal(deleted, dref)
+    st = g.neurons_get()
+    assert np.array_equal(st["connected"], cref)
+    assert np.allclose(st["grown"], go, rtol=1e-9, atol=1e-12)
+    gv = g.get_vacancies()
+    for t in range(3):
+        assert np.array_equal(gv[t], vref[t]), t
+    assert _pairs(*g.net_get_edges()) == _pairs(*og.export())
+    og.close()
+    g.close()

@@ -1,12 +1,14 @@
 """End-to-end parity of a small simulation (1000 neurons in a 1000 um cube, 80/20 E/I) with the oracle.
 
 Each period runs 100 neuron steps with background input (the same caller arrays on both sides) and
-synaptic input over the network the updates build, the element commit of the three element types,
-and one connectivity update (leaf fill, upward pass, search, commit).  The oracle restates each of
-these on the CPU (neuron step, synaptic input, ElementBaseBase::update, pointer octree search under
-the counter-based draws, serial commit) and keeps its own network graph.  After every period the
-neuron state, the element counts, the vacancies, the created synapses and the network graph must be
-identical.  FP64 only: the FP32-mixed picks are allowed to differ from FP64 (measured by bench.py).
+synaptic input over the network the updates build, the element commit of the three element types
+with the deletion of the synapses of shrunken elements, and one connectivity update (leaf fill,
+upward pass, search, commit).  The oracle restates each of these on the CPU (neuron step, synaptic
+input, ElementBaseBase::update, SynapseDeletionFinderCPU, pointer octree search under the
+counter-based draws, serial commit) and keeps its own network graph.  The calcium target lies inside
+the initial calcium range, so some neurons grow elements while others shrink and delete synapses.
+After every period the neuron state, the element counts, the vacancies, the created and deleted
+synapses and the network graph must be identical.  FP64 only: the FP32-mixed picks are allowed to differ from FP64 (measured by bench.py).
 """
 import sys
 
@@ -23,16 +25,12 @@ def _say(*a):
     print("[gpu-case]", *a, file=sys.stderr, flush=True)
 
 
-def _csr(tgt, src, w, n):
-    order = np.argsort(tgt, kind="stable")
-    off = np.concatenate([[0], np.cumsum(np.bincount(tgt, minlength=n))]).astype(np.uint64)
-    return off, np.asarray(src)[order].astype(np.uint32), np.asarray(w)[order].astype(np.int32)
-
-
-def _multiset(off, src, w):
+def _pairs(off, src, w):
     t = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off.astype(np.int64)))
-    keys = np.stack([t, src.astype(np.int64), w.astype(np.int64)])
-    return keys[:, np.lexsort(keys[::-1])]
+    out = {}
+    for a, b, c in zip(t.tolist(), src.tolist(), w.tolist()):
+        out[(a, b)] = out.get((a, b), 0) + c
+    return {k: v for k, v in out.items() if v != 0}
 
 
 def test_small_simulation_matches_oracle(oracle):
@@ -40,7 +38,7 @@ def test_small_simulation_matches_oracle(oracle):
     _say("simulation parity", n, "neurons,", periods, "periods of", steps, "steps")
     net = uniform_cube(n, 1000.0, 0)
     rng = np.random.default_rng(17)
-    eta, target = (0.02, 0.01, 0.0), 0.2
+    eta, target = (0.02, 0.01, 0.0), 0.1
     pg = StepParams.izhikevich(input_mode=0, nu=0.02)
     pg.synaptic_input, pg.synapse_conductance = 1, cond
     po = oracle.StepParams()
@@ -61,28 +59,30 @@ def test_small_simulation_matches_oracle(oracle):
     g.neurons_init(pg, tca, x=x, u=u, ca=ca, grown=grown.ravel())
 
     tree = oracle.Tree(net.xyz, net.box_min, net.box_max)
+    og = oracle.Graph(n)
     xo, uo, co = x.copy(), u.copy(), ca.copy()
     fo = np.zeros(n, np.uint8)
     do = np.zeros(3 * n)
     go = grown.copy()
     conn = np.zeros((3, n), np.uint32)
     vac = np.zeros((3, n), np.uint32)
-    et, es, ew = np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0, np.int64)
-    created_total = 0
+    created_total = deleted_total = 0
     for per in range(periods):
-        off, src, w = _csr(et, es, ew, n)
+        off, src, w = og.export()
         for k in range(steps):
             bg = rng.normal(5.0, 2.0, n)
             inp = oracle.synaptic_input(off, src, w, fo, cond, bg)
             oracle.neuron_step(po, xo, uo, co, inp, fo, tca, do, None)
             g.neurons_step(per * steps + k, bg)
         dv = do.reshape(3, n)
+        dref = np.zeros((3, n), np.uint32)
         for t in range(3):
             gt, dt, ct, vt = go[t].copy(), dv[t].copy(), conn[t].copy(), vac[t].copy()
-            oracle.elements_update(gt, dt, ct, vt)
+            dref[t] = oracle.elements_update(gt, dt, ct, vt)
             go[t], dv[t], conn[t], vac[t] = gt, dt, ct, vt
+            deleted_total += og.delete(t, dref[t], net.signal, conn, vac, step=per, seed=seed)
         do = dv.ravel().copy()
-        g.neurons_commit_elements()
+        assert np.array_equal(g.neurons_commit_elements(per), dref), per
         # connectivity update in the oracle: search under the counter-based draws, serial commit
         tree.update(vac[1], vac[2])
         r = tree.find_targets(net.signal, vac[0], seed=seed, step=per)
@@ -98,12 +98,10 @@ def test_small_simulation_matches_oracle(oracle):
                                     conn_axon=cax)
         vac[1], vac[2], vac[0], conn[1], conn[2], conn[0] = ve, vi, va, ce, ci, cax
         acc = ok.astype(bool)
-        et = np.concatenate([et, rt[acc].astype(np.int64)])
-        es = np.concatenate([es, rs[acc].astype(np.int64)])
-        ew = np.concatenate([ew, np.where(rg[acc] != 0, -1, 1)])
+        og.add(rt[acc], rs[acc], np.where(rg[acc] != 0, -1, 1))
         st = g.update_connectivity(per)
         created_total += created
-        _say("period", per, "requests", len(rt), "created", created, "edges", len(et))
+        _say("period", per, "requests", len(rt), "created", created, "deleted so far", deleted_total)
         # compare the whole state
         assert st["requests"] == len(rt) and st["created"] == created, per
         sg = g.neurons_get()
@@ -115,7 +113,8 @@ def test_small_simulation_matches_oracle(oracle):
         gv = g.get_vacancies()
         for t in range(3):
             assert np.array_equal(gv[t], vac[t]), (per, t)
-        assert np.array_equal(_multiset(*g.net_get_edges()), _multiset(*_csr(et, es, ew, n))), per
-    assert created_total > 0
+        assert _pairs(*g.net_get_edges()) == _pairs(*og.export()), per
+    assert created_total > 0 and deleted_total > 0
     tree.close()
+    og.close()
     g.close()

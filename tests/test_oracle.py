@@ -469,3 +469,119 @@ def test_synaptic_input_matches_dense_product(oracle):
     empty = np.zeros(n + 1, np.uint64)
     assert np.array_equal(oracle.synaptic_input(empty, np.zeros(0, np.uint32), np.zeros(0, np.int32), fired, 2.0),
                           np.zeros(n))
+
+
+def _consistent_network(oracle, n, m, seed):
+    """A random pair-merged graph with connected counts that match it (axons: out units; excitatory /
+    inhibitory dendrites: in units of positive / negative weight), signal = sign of the source."""
+    rng = np.random.default_rng(seed)
+    signal = (rng.random(n) < 0.25).astype(np.uint8)
+    tgt = rng.integers(0, n, m)
+    src = rng.integers(0, n, m)
+    w = np.where(signal[src] != 0, -1, 1).astype(np.int32)
+    order = np.argsort(tgt, kind="stable")
+    off = np.concatenate([[0], np.cumsum(np.bincount(tgt, minlength=n))]).astype(np.uint64)
+    g = oracle.Graph(n, off, src[order].astype(np.uint32), w[order])
+    conn = np.zeros((3, n), np.uint32)
+    eo, es, ew = g.export()
+    t_of = np.repeat(np.arange(n), np.diff(eo.astype(np.int64)))
+    np.add.at(conn[0], es, np.abs(ew))
+    np.add.at(conn[1], t_of[ew > 0], ew[ew > 0])
+    np.add.at(conn[2], t_of[ew < 0], -ew[ew < 0])
+    return g, signal, conn, (off, src[order].astype(np.uint32), w[order])
+
+
+def _pairs(graph):
+    off, src, w = graph.export()
+    t = np.repeat(np.arange(len(off) - 1), np.diff(off.astype(np.int64)))
+    return {(int(a), int(b)): int(c) for a, b, c in zip(t, src, w)}
+
+
+@pytest.mark.parametrize("etype", [0, 1, 2])
+def test_deletion_conserves_counts(oracle, etype):
+    # every deleted unit lowers one pair of the deleting neuron by one and disconnects exactly one
+    # element of the partner; neurons delete exactly what they were asked to; others are untouched
+    n = 300
+    g, signal, conn, _ = _consistent_network(oracle, n, 3000, 1)
+    before = _pairs(g)
+    rng = np.random.default_rng(2)
+    units = conn[etype].astype(np.int64)
+    to_del = np.minimum(units, rng.integers(0, 4, n)).astype(np.uint32)
+    c, v = conn.copy(), np.zeros((3, n), np.uint32)
+    tot = g.delete(etype, to_del, signal, c, v, step=3, seed=9)
+    assert tot == int(to_del.sum())
+    after = _pairs(g)
+    lost = {k: before[k] - after.get(k, 0) for k in before}
+    per_neuron = np.zeros(n, np.int64)
+    per_partner = np.zeros(n, np.int64)
+    for (t, s), d in lost.items():
+        if d == 0:
+            continue
+        assert abs(before[(t, s)]) >= abs(after.get((t, s), 0))
+        mine, partner = (s, t) if etype == 0 else (t, s)
+        if etype == 1:
+            assert before[(t, s)] > 0
+        if etype == 2:
+            assert before[(t, s)] < 0
+        per_neuron[mine] += abs(d)
+        per_partner[partner] += abs(d)
+    assert np.array_equal(per_neuron, to_del.astype(np.int64))
+    # partner side: axons disconnect the target's dendrite of the source's sign, dendrites the source's axon
+    if etype == 0:
+        drop = np.zeros((3, n), np.int64)
+        for (t, s), d in lost.items():
+            drop[1 if signal[s] == 0 else 2][t] += abs(d)
+        assert np.array_equal(conn.astype(np.int64) - c, drop) and np.array_equal(v.astype(np.int64), drop)
+    else:
+        assert np.array_equal(conn[0].astype(np.int64) - c[0], per_partner)
+        assert np.array_equal(v[0].astype(np.int64), per_partner)
+    g.close()
+
+
+def test_deletion_all_or_nothing(oracle):
+    n = 120
+    g, signal, conn, _ = _consistent_network(oracle, n, 900, 3)
+    before = _pairs(g)
+    c, v = conn.copy(), np.zeros((3, n), np.uint32)
+    assert g.delete(0, np.zeros(n, np.uint32), signal, c, v, step=0) == 0
+    assert _pairs(g) == before
+    # deleting every axon unit removes every pair
+    assert g.delete(0, conn[0], signal, c, v, step=0) == int(conn[0].sum())
+    assert _pairs(g) == {}
+    assert np.all(c[1] == 0) and np.all(c[2] == 0)
+    g.close()
+
+
+def test_deletion_is_order_independent_and_uniform(oracle):
+    # the drawn subset depends on the pairs, not on the order they were stored in; over many update
+    # indices every unit of a neuron is drawn with frequency k/m
+    n = 40
+    _, signal, conn, (off, src, w) = _consistent_network(oracle, n, 400, 4)
+    perm_src, perm_w = src.copy(), w.copy()
+    for t in range(n):
+        a, b = int(off[t]), int(off[t + 1])
+        p = np.random.default_rng(t).permutation(b - a)
+        perm_src[a:b], perm_w[a:b] = src[a:b][p], w[a:b][p]
+    i = int(np.argmax(conn[0]))
+    k, m = 2, int(conn[0][i])
+    to_del = np.zeros(n, np.uint32)
+    to_del[i] = k
+    hits = {}
+    trials = 3000
+    for step in range(trials):
+        res = []
+        for s_, w_ in ((src, w), (perm_src, perm_w)):
+            g = oracle.Graph(n, off, s_, w_)
+            before = _pairs(g)
+            g.delete(0, to_del, signal, conn.copy(), np.zeros((3, n), np.uint32), step=step, seed=11)
+            after = _pairs(g)
+            res.append({key: before[key] - after.get(key, 0) for key in before if before[key] != after.get(key, 0)})
+            g.close()
+        assert res[0] == res[1]
+        for key, d in res[0].items():
+            hits[key] = hits.get(key, 0) + abs(d)
+    units = {key: abs(val) for key, val in _pairs(oracle.Graph(n, off, src, w)).items() if key[1] == i}
+    assert sum(units.values()) == m
+    for key, u in units.items():
+        expect = trials * k * u / m
+        assert abs(hits.get(key, 0) - expect) < 5 * math.sqrt(expect) + 5, (key, hits.get(key, 0), expect)
