@@ -37,14 +37,18 @@ __device__ __forceinline__ double growth_curve(double current, double eta, doubl
     return (2.0 * exp(-product)) - 1.0;
 }
 
-// Normal(mean, stddev) from two 32-bit Philox words by the Box-Muller transform (tag 3).
+// Normal(mean, stddev) from two Philox words (tag 3) by the Box-Muller transform in FP32 on 24-bit
+// uniforms: background noise needs no more (|z| <= 5.77, the mass beyond is 8e-9), and the FP32
+// transcendentals take a third of what the FP64 ones cost in this kernel.  This input mode has no
+// counterpart in the reference's stream (its normal input comes from Boost); the caller-array mode
+// is the one to compare with the CPU path.
 __device__ __forceinline__ double normal_input(uint64_t seed, uint32_t neuron, uint32_t step, double mean, double sd) {
     const uint4 o = philox4x32_10(make_uint4(neuron, step, 0U, 3U),
                                   make_uint2(static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32U)));
-    const double u1 = (static_cast<double>(o.x) + 1.0) * 2.3283064365386963e-10;  // (0, 1]
-    const double u2 = static_cast<double>(o.y) * 2.3283064365386963e-10;          // [0, 1)
-    const double z = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
-    return mean + sd * z;
+    const float u1 = static_cast<float>((o.x >> 8U) + 1U) * 5.9604644775390625e-8f;  // (0, 1]
+    const float u2 = static_cast<float>(o.y >> 8U) * 5.9604644775390625e-8f;         // [0, 1)
+    const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+    return mean + sd * static_cast<double>(z);
 }
 
 struct StepArgs {
@@ -72,38 +76,62 @@ struct StepArgs {
     double conductance;
 };
 
-__global__ void k_step(StepArgs a) {
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 #pragma clang fp contract(off)
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= a.n) {
         return;
     }
-    if (a.enabled[i] == 0) {
-        a.fired[i] = a.fired_prev[i];  // the step leaves a disabled neuron's status as it was
+    const uint64_t n = a.n;
+    const rl_step_params& p = a.p;
+    // every per-neuron load first, so that they are in flight together instead of one round trip
+    // after another
+    const uint8_t en = a.enabled[i];
+    const uint8_t fprev = a.fired_prev[i];
+    const uint32_t g = a.net_of[i];
+    double xv = a.x[i];
+    double uv = a.u[i];
+    double c = a.ca[i];
+    const double tgt = a.target_ca[i];
+    const double dl[3] = { a.delta[i], a.delta[n + i], a.delta[2 * n + i] };
+    const double caller_input = p.input_mode == 0 ? a.input[i] : 0.0;
+    uint64_t eb = 0, ee = 0;
+    if (a.in_off != nullptr) {
+        eb = a.in_off[i];
+        ee = a.in_off[i + 1];
+    }
+    if (en == 0) {
+        a.fired[i] = fprev;  // the step leaves a disabled neuron's status as it was
         return;
     }
-    const rl_step_params& p = a.p;
-    const uint32_t g = a.net_of[i];
     const uint32_t step = a.step_base != nullptr ? *a.step_base + a.step : a.step;
-    double input = p.input_mode == 0 ? a.input[i]
+    double input = p.input_mode == 0 ? caller_input
                                      : normal_input(a.net_seed[g], static_cast<uint32_t>(i - a.net_off[g]), step,
                                                     p.input_mean, p.input_stddev);
     if (a.in_off != nullptr) {
         // SynapticEquallyWeightedActivityInputCPU::update_local_input: the weights of the in-edges
         // whose source fired in the previous step, times the conductance; the integer sum is what the
         // reference's FP64 accumulation of integer weights yields.  CombinedActivityInputCPU adds it
-        // to the background input.
+        // to the background input.  Four edges at a time, so that their gathers overlap.
         int64_t acc = 0;
-        const uint64_t e1 = a.in_off[i + 1];
-        for (uint64_t e = a.in_off[i]; e < e1; e++) {
-            acc += a.fired_prev[a.in_src[e]] != 0 ? a.in_w[e] : 0;
+        for (uint64_t e = eb; e < ee; e += 4) {
+            uint32_t src[4];
+            int32_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool in = e + k < ee;
+                src[k] = in ? a.in_src[e + k] : 0U;
+                w[k] = in ? a.in_w[e + k] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                acc += a.fired_prev[src[k]] != 0 ? w[k] : 0;
+            }
         }
         input = input + static_cast<double>(acc) * a.conductance;
     }
     const double scale = 1.0 / static_cast<double>(p.h);
     const double ab = p.a * p.b;
-    double xv = a.x[i];
-    double uv = a.u[i];
     uint8_t spiked = 0;
     for (uint32_t s = 0; s < p.h; s++) {
         const double x_increase = (p.k1 * xv * xv) + (p.k2 * xv) + p.k3 - uv + input;
@@ -122,17 +150,15 @@ __global__ void k_step(StepArgs a) {
     a.u[i] = uv;
     const double tau_inv = -1.0 / p.tau_ca;
     const double local_beta = p.beta_ca * static_cast<double>(spiked == 1);
-    double c = a.ca[i];
     for (uint32_t s = 0; s < p.h; s++) {
         c += scale * (c * tau_inv + local_beta);
     }
     a.ca[i] = c;
-    const double tgt = a.target_ca[i];
     for (int t = 0; t < 3; t++) {
         const double clamped_target = tgt < p.eta[t] ? p.eta[t] : tgt;  // std::max(target, eta)
         const double dz = growth_curve(c, p.eta[t], clamped_target);
         const double nu = a.net_nu != nullptr ? a.net_nu[3 * static_cast<uint64_t>(g) + t] : p.nu[t];
-        a.delta[t * a.n + i] += dz * nu;
+        a.delta[t * n + i] = dl[t] + dz * nu;
     }
 }
 

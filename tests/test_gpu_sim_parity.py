@@ -1,4 +1,5 @@
-"""End-to-end parity of a small simulation (1000 neurons in a 1000 um cube, 80/20 E/I) with the oracle.
+"""End-to-end parity of simulations with the oracle: 1000 neurons in a 1000 um cube (80/20 E/I), and
+the reference's real 51,842-neuron positions with its 425,706-synapse input network.
 
 Each period runs 100 neuron steps with background input (the same caller arrays on both sides) and
 synaptic input over the network the updates build, the element commit of the three element types
@@ -10,6 +11,7 @@ the initial calcium range, so some neurons grow elements while others shrink and
 After every period the neuron state, the element counts, the vacancies, the created and deleted
 synapses and the network graph must be identical.  FP64 only: the FP32-mixed picks are allowed to differ from FP64 (measured by bench.py).
 """
+import os
 import sys
 
 import numpy as np
@@ -19,6 +21,8 @@ from relearn_amd import RL_PREC_FP64, BarnesHutHIP, StepParams
 from relearn_amd.workloads import uniform_cube
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def _say(*a):
@@ -33,11 +37,9 @@ def _pairs(off, src, w):
     return {k: v for k, v in out.items() if v != 0}
 
 
-def test_small_simulation_matches_oracle(oracle):
-    n, periods, steps, seed, cond = 1000, 6, 100, 5, 1.5
-    _say("simulation parity", n, "neurons,", periods, "periods of", steps, "steps")
-    net = uniform_cube(n, 1000.0, 0)
-    rng = np.random.default_rng(17)
+def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed, rng, cond=1.5):
+    """Runs `periods` periods on the device and in the oracle side by side and compares them."""
+    n = len(xyz)
     eta, target = (0.02, 0.01, 0.0), 0.1
     pg = StepParams.izhikevich(input_mode=0, nu=0.02)
     pg.synaptic_input, pg.synapse_conductance = 1, cond
@@ -47,24 +49,31 @@ def test_small_simulation_matches_oracle(oracle):
     for t in range(3):
         pg.eta[t] = po.eta[t] = eta[t]
         po.nu[t] = pg.nu[t]
+    og = oracle.Graph(n) if graph is None else oracle.Graph(n, *graph)
+    # connected counts that match the initial graph, a fraction of an element vacant on top
+    conn = np.zeros((3, n), np.uint32)
+    eo, es, ew = og.export()
+    t_of = np.repeat(np.arange(n), np.diff(eo.astype(np.int64)))
+    np.add.at(conn[0], es, np.abs(ew))
+    np.add.at(conn[1], t_of[ew > 0], ew[ew > 0])
+    np.add.at(conn[2], t_of[ew < 0], -ew[ew < 0])
+    grown = conn + (rng.poisson(1.0, (3, n)) if graph is None else rng.uniform(0.0, 1.5, (3, n)))
+    grown = grown.astype(np.float64)
     x = np.full(n, -65.0)
     u = np.full(n, -13.0)
     ca = rng.uniform(0.05, 0.15, n)
     tca = np.full(n, target)
-    grown = rng.poisson(1.0, (3, n)).astype(np.float64)
 
-    g = BarnesHutHIP(net.box_min, net.box_max, seed=seed, precision=RL_PREC_FP64)
-    g.init(net.xyz, net.signal)
-    g.net_init(50 * n)
-    g.neurons_init(pg, tca, x=x, u=u, ca=ca, grown=grown.ravel())
+    g = BarnesHutHIP(box_min, box_max, seed=seed, precision=RL_PREC_FP64)
+    g.init(xyz, signal)
+    g.net_init(len(es) + 50 * n, *(graph if graph is not None else (None, None, None)))
+    g.neurons_init(pg, tca, x=x, u=u, ca=ca, grown=grown.ravel(), connected=conn.ravel())
 
-    tree = oracle.Tree(net.xyz, net.box_min, net.box_max)
-    og = oracle.Graph(n)
+    tree = oracle.Tree(xyz, box_min, box_max)
     xo, uo, co = x.copy(), u.copy(), ca.copy()
     fo = np.zeros(n, np.uint8)
     do = np.zeros(3 * n)
     go = grown.copy()
-    conn = np.zeros((3, n), np.uint32)
     vac = np.zeros((3, n), np.uint32)
     created_total = deleted_total = 0
     for per in range(periods):
@@ -80,18 +89,18 @@ def test_small_simulation_matches_oracle(oracle):
             gt, dt, ct, vt = go[t].copy(), dv[t].copy(), conn[t].copy(), vac[t].copy()
             dref[t] = oracle.elements_update(gt, dt, ct, vt)
             go[t], dv[t], conn[t], vac[t] = gt, dt, ct, vt
-            deleted_total += og.delete(t, dref[t], net.signal, conn, vac, step=per, seed=seed)
+            deleted_total += og.delete(t, dref[t], signal, conn, vac, step=per, seed=seed)
         do = dv.ravel().copy()
         assert np.array_equal(g.neurons_commit_elements(per), dref), per
         # connectivity update in the oracle: search under the counter-based draws, serial commit
         tree.update(vac[1], vac[2])
-        r = tree.find_targets(net.signal, vac[0], seed=seed, step=per)
+        r = tree.find_targets(signal, vac[0], seed=seed, step=per)
         assert np.all(r["margin"] > 1e-12), "a draw within 1e-12 of a boundary: picks may legitimately differ"
         cnt = r["count"].astype(np.int64)
         rs = np.repeat(np.arange(n), cnt)
         rj = np.concatenate([np.arange(c) for c in cnt]) if cnt.sum() else np.zeros(0, np.int64)
         rt = r["target"][r["offsets"].astype(np.int64)[rs] + rj]
-        rg = net.signal[rs]
+        rg = signal[rs]
         ve, vi, va = vac[1].copy(), vac[2].copy(), vac[0].copy()
         ce, ci, cax = conn[1].copy(), conn[2].copy(), conn[0].copy()
         ok, created = oracle.commit(rt, rs, rg, rj, ve, vi, va, seed=seed, step=per, conn_exc=ce, conn_inh=ci,
@@ -114,7 +123,31 @@ def test_small_simulation_matches_oracle(oracle):
         for t in range(3):
             assert np.array_equal(gv[t], vac[t]), (per, t)
         assert _pairs(*g.net_get_edges()) == _pairs(*og.export()), per
-    assert created_total > 0 and deleted_total > 0
     tree.close()
     og.close()
     g.close()
+    return created_total, deleted_total
+
+
+def test_small_simulation_matches_oracle(oracle):
+    n, periods, steps, seed = 1000, 6, 100, 5
+    _say("simulation parity", n, "neurons,", periods, "periods of", steps, "steps")
+    net = uniform_cube(n, 1000.0, 0)
+    created, deleted = _simulate(oracle, net.xyz, net.signal, net.box_min, net.box_max, None, periods, steps, seed,
+                                 np.random.default_rng(17))
+    assert created > 0 and deleted > 0
+
+
+def test_real_network_simulation_matches_oracle(oracle):
+    # the reference's input/positions.txt (51,842 neurons) with its input/rank_0_in_network.txt
+    # (425,706 synapses), read from the committed fixtures
+    d = np.load(os.path.join(GOLDEN, "positions_51842.npz"))
+    e = np.load(os.path.join(GOLDEN, "network_51842.npz"))
+    n = len(d["xyz"])
+    order = np.argsort(e["target"], kind="stable")
+    off = np.concatenate([[0], np.cumsum(np.bincount(e["target"], minlength=n))]).astype(np.uint64)
+    graph = (off, e["source"][order].astype(np.uint32), np.ones(len(order), np.int32))
+    _say("real network simulation parity", n, "neurons,", len(order), "synapses")
+    created, deleted = _simulate(oracle, d["xyz"], d["signal"], d["box_min"], d["box_max"], graph, 2, 20, 8,
+                                 np.random.default_rng(18), cond=0.03)
+    assert deleted > 0
