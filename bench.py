@@ -163,6 +163,8 @@ def run_ensemble(args):
         t0 = time.perf_counter()
         g.init_networks(ens.offsets, ens.boxes, ens.xyz, ens.signal)
         build_s = time.perf_counter() - t0
+        g.prepare_update_connectivity_device(*d_vac)
+        tuning = g.autotune(999)
         for w in range(args.warmup):
             g.prepare_update_connectivity_device(*d_vac)
             g.find_target_neurons(1000 + w)
@@ -188,7 +190,7 @@ def run_ensemble(args):
                 graph_ms.append(ms)
         g.close()
         out[name] = {"build_s": build_s, "ms_per_update": 1e3 * wall / args.steps, "searches_per_s": searches / wall,
-                     "graph_ms_per_update": float(np.mean(graph_ms))}
+                     "graph_ms_per_update": float(np.mean(graph_ms)), "tuning": tuning}
         log(f"ensemble {name}: {1e3 * wall / args.steps:.1f} ms per update of all networks, "
             f"{searches / wall:.4g} searches/s, graph replay {np.mean(graph_ms):.1f} ms, forest build {build_s:.2f} s")
     # the same networks, one context each, updated one after another (unbatched)
@@ -269,6 +271,14 @@ def run_sim(args):
             g.init_networks(ens.offsets, ens.boxes, ens.xyz, ens.signal)
         else:
             g.init(net.xyz, net.signal)
+        # search launch tuning on a representative update (Poisson vacancies), then the run starts
+        # from zero elements as configured
+        vr = np.random.default_rng(args.seed + 3)
+        g.prepare_update_connectivity(vr.poisson(2.0, n).astype(np.uint32), vr.poisson(1.5, n).astype(np.uint32),
+                                      vr.poisson(1.5, n).astype(np.uint32))
+        tuning = g.autotune(999)
+        zero = np.zeros(n, np.uint32)
+        g.prepare_update_connectivity(zero, zero, zero)
         p = StepParams.izhikevich(input_mode=1, nu=1e-4 * args.nu_scale)
         for t in range(3):
             p.eta[t] = SIM_ETA[t]
@@ -281,7 +291,7 @@ def run_sim(args):
             g.neurons_set_network_growth(np.repeat(ens.nu * args.nu_scale, 3))
         cap = 4 * n
         g.graph_create(period, cap)
-        rows, created, regrown = [], 0, 0
+        rows, created, deleted, regrown, net_regrown = [], 0, 0, 0, 0
         torch.cuda.synchronize()
         wall = 0.0   # replays only (each waits for its period); checkpoint copies are not timed
         for per in range(periods):
@@ -290,16 +300,24 @@ def run_sim(args):
                 st = g.graph_launch(per * period, per)
                 wall += time.perf_counter() - t_run
             except BHError as e:
-                if e.code != -7:
+                if e.code != -7 or "max_vacant_axons" not in str(e):
                     raise
                 # more vacant axons than the captured capacity: the replay's update changed nothing,
                 # so the eager update of the same index completes the period; then grow the graph
+                wall += time.perf_counter() - t_run
+                need = int(g.get_vacancies()[0].sum())
+                t_run = time.perf_counter()
                 st = g.update_connectivity(per)
                 wall += time.perf_counter() - t_run
-                cap *= 2
+                cap = max(2 * cap, need + need // 4)
+                log(f"sim {name}: period {per}: {need} vacant axons over max_vacant_axons, captured again for {cap}")
                 g.graph_create(period, cap)
                 regrown += 1
+            if st.get("net_regrown"):
+                net_regrown += 1
+                log(f"sim {name}: period {per}: network graph grown to {st['net_capacity']} entries")
             created += st["created"]
+            deleted += st.get("deleted", 0)
             if (per + 1) % every == 0 or per + 1 == periods:
                 s = g.neurons_get()
                 rows.append({"step": (per + 1) * period, "mean_ca": float(s["ca"].mean()),
@@ -308,7 +326,9 @@ def run_sim(args):
         edges = int(g.net_get_edges()[0][-1])
         g.close()
         out[name] = {"ms_per_period": 1e3 * wall / periods, "neuron_steps_per_s": n * period * periods / wall,
-                     "created": created, "graph_regrown": regrown, "edges": edges, "trajectory": rows}
+                     "created": created, "deleted": deleted, "graph_regrown": regrown,
+                     "net_regrown": net_regrown, "edges": edges,
+                     "tuning": tuning, "trajectory": rows}
         traj[name] = rows
         log(f"sim {name}: {1e3 * wall / periods:.2f} ms per period of {period} steps, "
             f"{n * period * periods / wall:.4g} neuron-steps/s, {created} synapses created")

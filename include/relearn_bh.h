@@ -73,6 +73,9 @@ typedef struct {
     double ms_search_kernel;    /* the dominant search kernel alone (cooperative or per-lane) */
     double ms_total;            /* the whole update, or the whole replayed period (rl_bh_graph_launch) */
     uint64_t deleted;           /* synapses deleted by the period's element commit (rl_bh_graph_launch) */
+    uint64_t net_capacity;      /* network graph capacity after the replay (rl_bh_graph_launch) */
+    uint64_t net_regrown;       /* 1: the replay's synapses did not fit the network graph, which grew and
+                                   took them; the period was captured again (rl_bh_graph_launch) */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a
@@ -293,10 +296,15 @@ int rl_bh_graph_create(rl_bh_ctx* ctx, uint32_t steps, uint64_t max_vacant_axons
 
 /* Replays the captured period: neuron steps first_step .. first_step + steps - 1 and connectivity
  * update index `update` (the step argument of rl_bh_update_connectivity).  With stats, waits for the
- * period, fills searches, requests, created, node_tests, node_tests_ref and ms_total (the other
- * times are 0), and makes the requests and accept flags readable (rl_bh_get_requests,
- * rl_bh_get_commit_result); returns RL_ELIMIT when this or an earlier replay without stats exceeded
- * max_vacant_axons.  Without stats (NULL) the replay is asynchronous on the context's stream. */
+ * period, fills searches, requests, created, deleted, node_tests, node_tests_ref, ms_total (the other
+ * times are 0) and the network graph fields, and makes the requests and accept flags readable
+ * (rl_bh_get_requests, rl_bh_get_commit_result).  When the created synapses do not fit the network
+ * graph (rl_net_init's capacity), the replay appends none of them; with stats the call then grows the
+ * graph (at least doubling it), appends them, captures the period again with the same arguments and
+ * sets net_regrown, so the period completes as an eager one would.  Returns RL_ELIMIT when this
+ * replay exceeded max_vacant_axons (the update changed nothing: rl_bh_update_connectivity with the
+ * same index completes the period) or an earlier replay without stats exceeded a capacity.  Without
+ * stats (NULL) the replay is asynchronous on the context's stream. */
 int rl_bh_graph_launch(rl_bh_ctx* ctx, uint32_t first_step, uint32_t update, rl_bh_step_stats* stats);
 
 /* Releases the captured period (rl_bh_destroy does too). */

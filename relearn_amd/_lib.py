@@ -65,6 +65,8 @@ class StepStats(C.Structure):
         ("ms_search_kernel", C.c_double),
         ("ms_total", C.c_double),
         ("deleted", C.c_uint64),
+        ("net_capacity", C.c_uint64),
+        ("net_regrown", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -817,6 +817,7 @@ int rl_bh_graph_create(rl_bh_ctx* c, uint32_t steps, uint64_t max_vacant_axons) 
         return hip_fail(c, ei, "hipGraphInstantiate");
     }
     c->graph_steps = steps;
+    c->graph_max_vacant = max_vacant_axons;
     c->prepared = false;
     c->searched_valid = false;
     return RL_OK;
@@ -867,10 +868,19 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
         return fail(c, RL_ELIMIT, "rl_bh_graph_launch: more vacant axons than max_vacant_axons; the update changed nothing");
     }
     if (h[kCtrNetOver] != 0) {
-        return fail(c, RL_ELIMIT, "rl_bh_graph_launch: the created synapses exceed the network graph's capacity and were "
-                                  "not added to it");
+        // the period is complete except for the append of its synapses, whose counts are kept: grow
+        // the network graph, append them, and capture the period again over the new arrays
+        const uint32_t steps = c->graph_steps;
+        const uint64_t max_vacant = c->graph_max_vacant;
+        RL_TRY(net_append_retry(c));
+        RL_TRY(rl_bh_graph_create(c, steps, max_vacant));
+        stats->net_regrown = 1;
+        c->slots = h[kCtrSlots];
+        c->nreq = h[kCtrRequests];
+        c->searched_valid = true;
     }
-    if (h[kCtrSticky] != 0) {
+    stats->net_capacity = c->net.ready ? c->net.cap : 0;
+    if (h[kCtrSticky] > (h[kCtrNetOver] != 0 ? 1ULL : 0ULL)) {
         return fail(c, RL_ELIMIT, "rl_bh_graph_launch: an earlier replay without stats exceeded a capacity");
     }
     if (h[kCtrDelShort] != 0) {

@@ -34,7 +34,7 @@ enum : int {
     kCtrOverflow = 8,    // sources whose root list overflowed the cooperative slot table
     kCtrRedo = 9,        // sources of the per-lane path that need the fallback pass
     kCtrActive = 10,     // sources with vacant axons
-    kCtrSticky = 11,     // over-capacity seen since the host last read the counters
+    kCtrSticky = 11,     // over-capacity events (slots, network graph) since the host last read the counters
     kCtrNetOver = 12,    // 1: the created synapses did not fit the network graph, none was appended
     kCtrNetNew = 13,     // synapses appended to the network graph by the last commit
     kCtrDeleted = 14,    // synapse units deleted by the last element commit
@@ -201,6 +201,7 @@ struct rl_bh_ctx {
     // captured plasticity period (rl_bh_graph_create)
     hipGraphExec_t graph_exec = nullptr;
     uint32_t graph_steps = 0;
+    uint64_t graph_max_vacant = 0;
     bool capturing = false;
     rl::Neurons neurons;
     rl::NetGraph net;
@@ -291,6 +292,7 @@ int net_get(rl_bh_ctx* c, uint64_t capacity, uint64_t* in_offsets, uint32_t* sou
 int enqueue_net_append(rl_bh_ctx* c);   // appends the accepted requests of the last commit
 int enqueue_net_delete(rl_bh_ctx* c, int type);  // deletes the synapses of the type's shrunken elements
 int net_append(rl_bh_ctx* c);           // eager: grows the capacity if needed, then appends
+int net_append_retry(rl_bh_ctx* c);     // after a captured append over capacity: grow, then append
 void net_free(rl_bh_ctx* c);
 
 }  // namespace rl

@@ -69,7 +69,7 @@ __global__ void k_net_check(uint64_t n, const uint64_t* __restrict__ off, const 
         counters[kCtrNetOver] = over ? 1ULL : 0ULL;
         counters[kCtrNetNew] = over ? 0ULL : add;
         if (over) {
-            counters[kCtrSticky] = 1ULL;
+            counters[kCtrSticky] += 1ULL;
         }
     }
 }
@@ -561,42 +561,74 @@ int enqueue_net_append(rl_bh_ctx* c) {
     return enqueue_net_merge(c);
 }
 
-int net_append(rl_bh_ctx* c) {
+// Grows the edge arrays to hold `need` entries (at least doubling), keeping the current pairs.  A
+// captured period refers to the old arrays and is released.
+static int net_grow(rl_bh_ctx* c, uint64_t have, uint64_t need) {
     NetGraph& g = c->net;
-    if (!g.ready || c->n == 0) {
-        return RL_OK;
+    hipStream_t s = c->stream;
+    destroy_graph(c);
+    const uint64_t cap = std::max<uint64_t>(2 * g.cap, need);
+    uint32_t* src = nullptr;
+    int32_t* w = nullptr;
+    RL_TRY(dalloc(c, &src, cap));
+    RL_TRY(dalloc(c, &w, cap));
+    if (have > 0) {
+        RL_HIP(c, hipMemcpyAsync(src, g.src, have * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        RL_HIP(c, hipMemcpyAsync(w, g.w, have * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
     }
+    RL_HIP(c, hipStreamSynchronize(s));
+    dfree(&g.src);
+    dfree(&g.w);
+    dfree(&g.src2);
+    dfree(&g.w2);
+    dfree(&g.olist);
+    g.src = src;
+    g.w = w;
+    RL_TRY(dalloc(c, &g.src2, cap));
+    RL_TRY(dalloc(c, &g.w2, cap));
+    RL_TRY(dalloc(c, &g.olist, cap));
+    g.cap = cap;
+    return RL_OK;
+}
+
+// Merges the counted synapses (ncnt, npre of the last commit), growing the arrays first when they
+// would not fit.
+static int net_merge_grow(rl_bh_ctx* c) {
+    NetGraph& g = c->net;
     const uint64_t n = c->n;
     hipStream_t s = c->stream;
-    RL_TRY(enqueue_net_count(c));
     uint64_t have = 0, add = 0;
     RL_HIP(c, hipMemcpyAsync(&have, g.off + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     RL_HIP(c, hipMemcpyAsync(&add, g.npre + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     RL_HIP(c, hipStreamSynchronize(s));
     if (have + add > g.cap) {
-        // grow: keep the current pairs, double the capacity; a captured period refers to the old
-        // arrays and is released
-        destroy_graph(c);
-        const uint64_t cap = std::max<uint64_t>(2 * g.cap, have + add);
-        uint32_t* src = nullptr;
-        int32_t* w = nullptr;
-        RL_TRY(dalloc(c, &src, cap));
-        RL_TRY(dalloc(c, &w, cap));
-        if (have > 0) {
-            RL_HIP(c, hipMemcpyAsync(src, g.src, have * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-            RL_HIP(c, hipMemcpyAsync(w, g.w, have * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
-        }
-        RL_HIP(c, hipStreamSynchronize(s));
-        dfree(&g.src);
-        dfree(&g.w);
-        g.src = src;
-        g.w = w;
-        RL_TRY(dalloc(c, &g.src2, cap));
-        RL_TRY(dalloc(c, &g.w2, cap));
-        RL_TRY(dalloc(c, &g.olist, cap));
-        g.cap = cap;
+        RL_TRY(net_grow(c, have, have + add));
     }
     return enqueue_net_merge(c);
+}
+
+int net_append(rl_bh_ctx* c) {
+    if (!c->net.ready || c->n == 0) {
+        return RL_OK;
+    }
+    RL_TRY(enqueue_net_count(c));
+    return net_merge_grow(c);
+}
+
+// A captured append that did not fit left the counts of its synapses (ncnt, npre) and the graph
+// untouched: grow and merge them now.
+int net_append_retry(rl_bh_ctx* c) {
+    if (!c->net.ready || c->n == 0) {
+        return RL_OK;
+    }
+    RL_TRY(net_merge_grow(c));
+    unsigned long long over = 0;
+    RL_HIP(c, hipMemcpyAsync(&over, c->counters + kCtrNetOver, sizeof(over), hipMemcpyDeviceToHost, c->stream));
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    if (over != 0) {
+        return fail(c, RL_EHIP, "network graph append: still over capacity after growing");
+    }
+    return RL_OK;
 }
 
 // Deletes the synapses of the elements of `type` that the element commit unbound (neurons.deleted).

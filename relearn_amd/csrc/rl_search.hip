@@ -436,7 +436,7 @@ __global__ void k_search_reset(uint64_t n, const uint64_t* __restrict__ slot_tot
     if (t0 < kCounters) {
         if (t0 == kCtrSticky) {
             if (over) {
-                counters[t0] = 1;
+                counters[t0] += 1;
             }
         } else if (t0 != kCtrDeleted && t0 != kCtrDelShort) {  // those belong to the element commit before
             counters[t0] = (t0 == kCtrOverCap && over) ? 1ULL : 0ULL;

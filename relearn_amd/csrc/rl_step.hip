@@ -453,7 +453,18 @@ int neurons_commit(rl_bh_ctx* c, uint32_t update, uint32_t* deleted) {
         RL_HIP(c, hipMemcpyAsync(deleted, c->neurons.deleted, 3 * c->n * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                  c->stream));
     }
+    unsigned long long short_of = 0;
+    if (c->net.ready && c->n > 0) {
+        RL_HIP(c, hipMemcpyAsync(&short_of, c->counters + kCtrDelShort, sizeof(short_of), hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
     RL_HIP(c, hipStreamSynchronize(c->stream));
+    if (short_of != 0) {
+        // the reference throws here (RandomSynapseDeletionFinder::find_synapses_on_neuron); the
+        // deletions that the graph allowed have been made
+        return fail(c, RL_EINVAL, "rl_neurons_commit_elements: the network graph holds fewer synapses than the "
+                                  "connected counts of the elements to delete");
+    }
     return RL_OK;
 }
 

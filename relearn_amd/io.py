@@ -3,7 +3,8 @@
 Files written here are read by the reference's NeuronIO and files it writes are read here, so a
 network grown on the device can be continued or analysed by the CPU code and vice versa.
 
-  neurons (NeuronIO::read_neurons / write_neuron_positions_and_signals, source/io/NeuronIO.cpp:429-560):
+  neurons (NeuronIO::read_neuron_positions_and_signals / write_neuron_positions_and_signals,
+      source/io/NeuronIO.cpp:227-303 and 439-545):
       '#' header lines, then one line per neuron: <id> <x> <y> <z> <area> <type>, ids from 1, type
       'ex' or 'in'.
   in / out synapses (NeuronIO::read_in_synapses / write_in_synapses, NeuronIO.cpp:793-1100):

@@ -7,6 +7,8 @@ network_51842.npz: the in-network of the same neurons, input/rank_0_in_network.t
 synapses of weight 1 on rank 0), as target and source ids (from 0) of every line.
 network_head_{in,out}.txt: the first 40 data lines of input/rank_0_in_network.txt and
 input/rank_0_out_network.txt with their header lines, for the text-format readers.
+positions_head.txt: the header and first 200 data lines of input/positions.txt, for the neuron-file
+reader.
 """
 from __future__ import annotations
 
@@ -79,6 +81,10 @@ def main():
         with open(out, "w") as fh:
             fh.write(head(f"/root/reference/relearn/input/rank_0_{kind}_network.txt", 40))
         print(out, os.path.getsize(out))
+    out = os.path.join(HERE, "positions_head.txt")
+    with open(out, "w") as fh:
+        fh.write(head(SRC, 200))
+    print(out, os.path.getsize(out))
 
 
 if __name__ == "__main__":

@@ -95,3 +95,23 @@ def test_deletion_matches_oracle(oracle, update):
     assert _pairs(*g.net_get_edges()) == _pairs(*og.export())
     og.close()
     g.close()
+
+
+def test_deletion_reports_inconsistent_graph():
+    # connected counts that claim more synapses than the graph holds: the reference throws; the
+    # device makes the deletions the graph allows and reports the shortfall
+    n = 200
+    net = uniform_cube(n, 300.0, 64)
+    g = BarnesHutHIP(net.box_min, net.box_max, seed=3)
+    g.init(net.xyz, net.signal)
+    g.net_init(100)                      # empty graph
+    conn = np.zeros((3, n), np.uint32)
+    conn[0, :] = 2                       # every neuron claims two bound axons
+    grown = conn.astype(np.float64)
+    p = StepParams.izhikevich(input_mode=0, nu=0.4)
+    p.eta[0] = p.eta[1] = p.eta[2] = 0.0
+    g.neurons_init(p, np.full(n, 0.1), ca=np.full(n, 0.9), grown=grown.ravel(), connected=conn.ravel())
+    g.neurons_step(0, np.full(n, 5.0))
+    with pytest.raises(Exception):
+        g.neurons_commit_elements(1)
+    g.close()

@@ -133,16 +133,20 @@ def test_net_append_matches_commit(precision):
     g.close()
 
 
-def test_graph_period_with_synaptic_input_matches_eager():
+@pytest.mark.parametrize("room", [20, 0])
+def test_graph_period_with_synaptic_input_matches_eager(room):
+    """room: free network graph entries per neuron at the start; with none, the first replays' synapses
+    do not fit, and the replay grows the graph, appends them and captures the period again."""
     n, steps, periods = 10_000, 21, 3   # an odd step count exercises the fired-status copy-back
-    _say("graph period with synaptic input vs eager,", n, "neurons,", steps, "steps x", periods)
+    _say("graph period with synaptic input vs eager,", n, "neurons,", steps, "steps x", periods, "room", room)
     net = uniform_cube(n, 2200.0, 42)
     off0, src0, w0 = _random_graph(n, 8 * n, 7, wmax=1)
+    pairs0 = len(_pairs(off0, src0, w0))   # the entries the graph starts with
     ctxs = []
     for _ in range(2):
         c = BarnesHutHIP(net.box_min, net.box_max, seed=13, precision=RL_PREC_FP32_MIXED)
         c.init(net.xyz, net.signal)
-        c.net_init(len(src0) + 20 * n, off0, src0, w0)
+        c.net_init(pairs0 + room * n, off0, src0, w0)
         p = StepParams.izhikevich(input_mode=1, nu=0.02)
         p.synaptic_input = 1
         p.synapse_conductance = 2.0
@@ -156,6 +160,7 @@ def test_graph_period_with_synaptic_input_matches_eager():
     graph.neurons_step(0)       # leaves the last status in the second buffer before capture
     eager.neurons_step(0)
     graph.graph_create(steps, 8 * n)
+    regrown = 0
     for per in range(periods):
         first = 1 + per * steps
         for k in range(steps):
@@ -163,13 +168,16 @@ def test_graph_period_with_synaptic_input_matches_eager():
         eager.neurons_commit_elements()
         se_stats = eager.update_connectivity(per)
         sg_stats = graph.graph_launch(first, per)
-        _say("period", per, "created", sg_stats["created"])
+        _say("period", per, "created", sg_stats["created"], "net regrown", sg_stats["net_regrown"])
+        regrown += sg_stats["net_regrown"]
+        assert sg_stats["net_capacity"] >= len(graph.net_get_edges()[1])
         assert se_stats["created"] == sg_stats["created"]
         se, sg = eager.neurons_get(), graph.neurons_get()
         for key in ("x", "u", "ca", "fired", "delta", "grown", "connected"):
             assert np.array_equal(se[key], sg[key]), (per, key)
         assert _merged(*eager.net_get_edges()) == _merged(*graph.net_get_edges())
     assert len(_pairs(*eager.net_get_edges())) > len(_pairs(off0, src0, w0))
+    assert (regrown > 0) == (room == 0)
     eager.close()
     graph.close()
 

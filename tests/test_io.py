@@ -45,3 +45,14 @@ def test_neurons_round_trip(tmp_path):
     r = io.read_neurons(tmp_path / "pos.txt")
     assert np.allclose(r["xyz"], xyz, atol=5e-7) and np.array_equal(r["signal"], sig)
     assert np.allclose(r["box_min"], d["box_min"]) and np.allclose(r["box_max"], d["box_max"])
+
+
+def test_reference_positions_head_parses():
+    # header and first 200 neurons of the reference's input/positions.txt: the same numbers as the
+    # positions fixture, and the subdomain box of its header
+    r = io.read_neurons(os.path.join(GOLDEN, "positions_head.txt"))
+    d = np.load(os.path.join(GOLDEN, "positions_51842.npz"))
+    assert r["xyz"].shape == (200, 3)
+    assert np.array_equal(r["xyz"], d["xyz"][:200]) and np.array_equal(r["signal"], d["signal"][:200])
+    assert np.array_equal(r["box_min"], d["box_min"]) and np.array_equal(r["box_max"], d["box_max"])
+    assert r["area"][0] == "lh_bankssts"
