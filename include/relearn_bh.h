@@ -267,7 +267,9 @@ int rl_net_get_edges(rl_bh_ctx* ctx, uint64_t capacity, uint64_t* in_offsets, ui
 
 typedef struct {
     uint32_t search_waves_per_cu;  /* persistent waves of the cooperative search per compute unit, 1..24 */
-    uint32_t reserved[7];
+    uint32_t search_chunk;         /* sources a search wave takes from the work queue at a time, 1..64
+                                      (0: 1) */
+    uint32_t reserved[6];
 } rl_bh_tuning;
 
 /* Launch tuning of the search (no counterpart in the reference, whose CPU loop has none).  Results
@@ -275,8 +277,8 @@ typedef struct {
 int rl_bh_set_tuning(rl_bh_ctx* ctx, const rl_bh_tuning* tuning);
 int rl_bh_get_tuning(rl_bh_ctx* ctx, rl_bh_tuning* tuning);
 
-/* Times the search of the prepared update `step` (after rl_bh_prepare_*) at candidate tunings and
- * keeps the fastest.  Only the search runs; the vacancies do not change, and afterwards the search
+/* Times the search of the prepared update `step` (after rl_bh_prepare_*) at candidate tunings (waves
+ * per CU 8..24, then work-queue chunks 1..8 at the best wave count) and keeps the fastest.  Only the search runs; the vacancies do not change, and afterwards the search
  * result of `step` is readable as after rl_bh_find_target_neurons.  best may be NULL. */
 int rl_bh_autotune(rl_bh_ctx* ctx, uint32_t step, rl_bh_tuning* best);
 

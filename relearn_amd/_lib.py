@@ -76,7 +76,8 @@ class StepStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("search_waves_per_cu", C.c_uint32),
-        ("reserved", C.c_uint32 * 7),
+        ("search_chunk", C.c_uint32),
+        ("reserved", C.c_uint32 * 6),
     ]
 
 
@@ -353,21 +354,22 @@ class BarnesHutHIP:
                         int(counts[i])))
         return out
 
-    def set_tuning(self, search_waves_per_cu):
+    def set_tuning(self, search_waves_per_cu, search_chunk=1):
         t = Tuning()
         t.search_waves_per_cu = int(search_waves_per_cu)
+        t.search_chunk = int(search_chunk)
         self._check(self.L.rl_bh_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
         t = Tuning()
         self._check(self.L.rl_bh_get_tuning(self.h, C.byref(t)))
-        return {"search_waves_per_cu": t.search_waves_per_cu}
+        return {"search_waves_per_cu": t.search_waves_per_cu, "search_chunk": t.search_chunk}
 
     def autotune(self, step=0):
         """Keeps the fastest search tuning for the prepared update; returns it."""
         t = Tuning()
         self._check(self.L.rl_bh_autotune(self.h, int(step), C.byref(t)))
-        return {"search_waves_per_cu": t.search_waves_per_cu}
+        return {"search_waves_per_cu": t.search_waves_per_cu, "search_chunk": t.search_chunk}
 
     def graph_create(self, steps=0, max_vacant_axons=None):
         """Captures a plasticity period (steps neuron steps, element commit, connectivity update)."""

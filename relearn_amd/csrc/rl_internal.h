@@ -194,6 +194,7 @@ struct rl_bh_ctx {
     uint32_t coop_blocks = 0;        // persistent search waves: cus * waves_per_cu
     uint32_t cus = 0;
     uint32_t waves_per_cu = 16;
+    uint32_t search_chunk = 1;       // sources per work-queue grab of a search wave
     // (slot, source, node, level) of descents that need the general per-lane traversal (slots_cap)
     uint4* slow_items = nullptr;
     void* tmp = nullptr;

@@ -48,6 +48,7 @@ struct CoopArgs {
     const uint32_t* work;
     const uint64_t* nwork;        // device count of work; XCD segment x is [nwork x / 8, nwork (x + 1) / 8)
     unsigned long long* cursor;   // kXcds counters
+    uint32_t chunk;               // sources taken from a segment per cursor increment
     uint32_t* t_my;               // per wave: cap slots, packed child range of the slot's node
     uint32_t* t_fexp;             // first expanded child slot | expanded-child count << 28
     double* t_mass;
@@ -631,19 +632,27 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(5, 8
     unsigned long long acc_exec = 0;
     const uint64_t nw = *a.nwork;
     const uint32_t step = *s.step;
+    const uint32_t chunk = a.chunk;
+    uint64_t w_next = 0, w_lim = 0;   // the wave's current chunk of its segment
     while (true) {
-        uint64_t w = 0;
-        if (ln == 0) {
-            w = nw * static_cast<uint64_t>(seg) / kXcds + atomicAdd(&a.cursor[seg], 1ULL);
-        }
-        w = __shfl(w, 0, kWave);
-        if (w >= nw * static_cast<uint64_t>(seg + 1) / kXcds) {
-            if (++tried >= kXcds) {
-                break;
+        if (w_next >= w_lim) {
+            const uint64_t seg_end = nw * static_cast<uint64_t>(seg + 1) / kXcds;
+            uint64_t w0 = 0;
+            if (ln == 0) {
+                w0 = nw * static_cast<uint64_t>(seg) / kXcds + atomicAdd(&a.cursor[seg], static_cast<unsigned long long>(chunk));
             }
-            seg = (seg + 1) % kXcds;
-            continue;
+            w0 = __shfl(w0, 0, kWave);
+            if (w0 >= seg_end) {
+                if (++tried >= kXcds) {
+                    break;
+                }
+                seg = (seg + 1) % kXcds;
+                continue;
+            }
+            w_next = w0;
+            w_lim = std::min<uint64_t>(w0 + chunk, seg_end);
         }
+        const uint64_t w = w_next++;
         const uint32_t src = __builtin_amdgcn_readfirstlane(a.work[w]);
         const uint32_t na = __builtin_amdgcn_readfirstlane(s.vac_axon[src]);
         const uint32_t slot0 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(s.slot_off[src]));
@@ -782,6 +791,7 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa) {
     a.work = c->coop_work;
     a.nwork = sa.nwork;
     a.cursor = c->coop_cursor;
+    a.chunk = std::max<uint32_t>(c->search_chunk, 1U);
     a.t_my = c->coop_node;
     a.t_fexp = c->coop_fexp;
     a.t_mass = c->coop_mass;

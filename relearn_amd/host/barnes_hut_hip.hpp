@@ -172,11 +172,11 @@ public:
     }
 
     // Chooses the search launch tuning on the prepared update (after prepare_update_connectivity) by
-    // timing it; the search results do not depend on it.  Returns the persistent waves per CU kept.
-    std::uint32_t autotune() {
+    // timing it; the search results do not depend on it.  Returns the tuning kept.
+    rl_bh_tuning autotune() {
         rl_bh_tuning t{};
         check(rl_bh_autotune(ctx_, step_, &t));
-        return t.search_waves_per_cu;
+        return t;
     }
 
     // The algorithm is selected under the CPU path's name (relearn.cpp:313).

@@ -182,20 +182,22 @@ def test_search_tuning_does_not_change_results(precision):
     g.init(net.xyz, net.signal)
     slots = int(net.vac_axon.sum())
     ref = None
-    for w in (1, 5, 16, 24):
-        g.set_tuning(w)
-        assert g.get_tuning()["search_waves_per_cu"] == w
+    for w, k in ((1, 1), (5, 1), (16, 1), (24, 1), (16, 4), (3, 64)):
+        g.set_tuning(w, k)
+        assert g.get_tuning() == {"search_waves_per_cu": w, "search_chunk": k}
         g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
         g.find_target_neurons(2)
         cur = g.get_search_slots(slots)
         if ref is None:
             ref = cur
         for a, b in zip(ref, cur):
-            assert np.array_equal(a, b), w
+            assert np.array_equal(a, b), (w, k)
     best = g.autotune(2)
-    assert 1 <= best["search_waves_per_cu"] <= 24
+    assert 1 <= best["search_waves_per_cu"] <= 24 and 1 <= best["search_chunk"] <= 8
     for a, b in zip(ref, g.get_search_slots(slots)):
         assert np.array_equal(a, b)
     with pytest.raises(BHError):
         g.set_tuning(0)
+    with pytest.raises(BHError):
+        g.set_tuning(16, 65)
     g.close()
