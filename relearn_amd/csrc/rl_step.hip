@@ -74,6 +74,10 @@ struct StepArgs {
     const uint32_t* in_src;
     const int32_t* in_w;
     double conductance;
+    // 1 / h and -1 / tau_ca, divided once on the host (IEEE division is correctly rounded on both
+    // sides, so the values are the ones the reference's loops use)
+    double scale;
+    double tau_inv;
 };
 
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
@@ -130,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
         }
         input = input + static_cast<double>(acc) * a.conductance;
     }
-    const double scale = 1.0 / static_cast<double>(p.h);
+    const double scale = a.scale;
     const double ab = p.a * p.b;
     uint8_t spiked = 0;
     for (uint32_t s = 0; s < p.h; s++) {
@@ -148,7 +152,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     a.fired[i] = spiked;
     a.x[i] = xv;
     a.u[i] = uv;
-    const double tau_inv = -1.0 / p.tau_ca;
+    const double tau_inv = a.tau_inv;
     const double local_beta = p.beta_ca * static_cast<double>(spiked == 1);
     for (uint32_t s = 0; s < p.h; s++) {
         c += scale * (c * tau_inv + local_beta);
@@ -319,6 +323,8 @@ StepArgs step_args(rl_bh_ctx* c) {
     a.target_ca = s.target_ca;
     a.delta = s.delta;
     a.enabled = c->enabled;
+    a.scale = 1.0 / static_cast<double>(s.p.h);
+    a.tau_inv = -1.0 / s.p.tau_ca;
     if (s.p.synaptic_input != 0) {
         a.in_off = c->net.off;
         a.in_src = c->net.src;
