@@ -42,6 +42,25 @@ def test_pick_draw_addresses_distinct(oracle):
     assert oracle.pick_draw(7, 1, 5, 0, 1) != oracle.pick_draw(8, 1, 5, 0, 1)
 
 
+def test_counter_draws_are_uniform(oracle):
+    # the pick draws and commit priorities of consecutive counters (neurons, axon ordinals, levels,
+    # steps) are uniform 32-bit words: Kolmogorov-Smirnov on 2e5 draws, every bit balanced, no
+    # correlation between neighbouring counters
+    from scipy import stats
+    draws = {
+        "neuron": [oracle.pick_draw(7, i, 5, 0, 1) for i in range(100_000)],
+        "axon": [oracle.pick_draw(7, 3, 5, j, 1) for j in range(50_000)] + [oracle.pick_draw(7, 4, 5, j, 2) for j in range(50_000)],
+        "step": [oracle.pick_draw(9, 11, s, 0, 1) for s in range(100_000)],
+        "priority": [oracle.commit_priority(7, i, 5, 0) >> 32 for i in range(100_000)],
+    }
+    for name, d in draws.items():
+        u = np.asarray(d, dtype=np.float64) / 2.0**32
+        assert stats.kstest(u, "uniform").pvalue > 1e-4, name
+        bits = (np.asarray(d, dtype=np.uint64)[:, None] >> np.arange(32, dtype=np.uint64)) & np.uint64(1)
+        assert np.all(np.abs(bits.mean(0) - 0.5) < 5 * 0.5 / np.sqrt(len(d))), name
+        assert abs(np.corrcoef(u[:-1], u[1:])[0, 1]) < 5 / np.sqrt(len(d)), name
+
+
 def test_draw_to_uniform_boost_mapping(oracle):
     # boost uniform_real_distribution on a 32-bit engine: k / 2^32 * next, always < next
     for nxt in (1.0, np.nextafter(3.7, 4.0), 1e-300, 12345.678):
