@@ -823,6 +823,8 @@ int rl_bh_graph_create(rl_bh_ctx* c, uint32_t steps, uint64_t max_vacant_axons) 
     if (status == RL_OK) status = enqueue_search(c);
     if (status == RL_OK) status = enqueue_commit(c, nullptr);
     if (status == RL_OK) status = enqueue_net_append(c);
+    // the next period's steps push over the pairs as this period left them
+    if (status == RL_OK && c->neurons.ready && c->neurons.p.synaptic_input != 0) status = enqueue_syn_sync(c, 0);
     c->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
@@ -855,11 +857,18 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     RL_TRY(set_device(c));
     hipStream_t s = c->stream;
     RL_TRY(normalize_fired(c));
+    const bool synaptic = c->neurons.ready && c->neurons.p.synaptic_input != 0 && c->net.ready;
+    if (synaptic) {
+        RL_TRY(syn_prepare(c));
+    }
     RL_TRY(set_steps(c, first_step, update));
     if (stats != nullptr) {
         RL_HIP(c, hipEventRecord(c->ev0, s));
     }
     RL_HIP(c, hipGraphLaunch(c->graph_exec, s));
+    if (synaptic) {
+        c->net.syn_valid = true;   // the period ends by rebuilding the pushes over its pairs
+    }
     c->prepared = false;
     c->searched_valid = false;
     c->nreq = 0;

@@ -83,8 +83,18 @@ struct NetGraph {
     uint32_t* ocnt = nullptr;      // n: out-entries per source of a deletion phase
     uint64_t* ooff = nullptr;      // n + 1
     uint2* olist = nullptr;        // cap: (entry, target) of each out-entry, grouped by source
+    // Synaptic input by push: the non-zero pairs grouped by source, and per step parity the summed
+    // weights that the neurons which fired pushed to their targets for the next step.
+    uint64_t* out_off = nullptr;   // n + 1
+    uint32_t* out_cnt = nullptr;   // n: zero between builds
+    uint32_t* out_tgt = nullptr;   // cap
+    int32_t* out_w = nullptr;      // cap
+    int32_t* acc[2] = { nullptr, nullptr };  // n each
     uint64_t cap = 0;
     bool ready = false;
+    // acc[parity] holds the pushes of fired_buf[parity] over the current pairs, acc[parity ^ 1] is
+    // zero, and the out-edge lists match the graph
+    bool syn_valid = false;
 };
 
 struct Neurons {
@@ -294,6 +304,8 @@ int enqueue_net_append(rl_bh_ctx* c);   // appends the accepted requests of the 
 int enqueue_net_delete(rl_bh_ctx* c, int type);  // deletes the synapses of the type's shrunken elements
 int net_append(rl_bh_ctx* c);           // eager: grows the capacity if needed, then appends
 int net_append_retry(rl_bh_ctx* c);     // after a captured append over capacity: grow, then append
+int enqueue_syn_sync(rl_bh_ctx* c, int parity);  // out-edge lists from the graph, acc from fired_buf[parity]
+int syn_prepare(rl_bh_ctx* c);          // enqueue_syn_sync at the current parity unless syn_valid
 void net_free(rl_bh_ctx* c);
 
 }  // namespace rl

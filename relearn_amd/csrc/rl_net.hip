@@ -165,6 +165,53 @@ __global__ void k_net_offsets(uint64_t n, const uint64_t* __restrict__ noff, uin
     }
 }
 
+// ---------------------------------------------------------------- synaptic input by push
+
+// Non-zero pairs per source (out_cnt starts at zero).
+__global__ void k_out_count(uint64_t n, const uint64_t* __restrict__ off, const uint32_t* __restrict__ src,
+                            const int32_t* __restrict__ w, uint32_t* __restrict__ out_cnt) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < n; t += stride) {
+        for (uint64_t e = off[t]; e < off[t + 1]; e++) {
+            if (w[e] != 0) {
+                atomicAdd(&out_cnt[src[e]], 1U);
+            }
+        }
+    }
+}
+
+// The pairs grouped by source at out_off (order inside a source arbitrary); out_cnt returns to zero.
+__global__ void k_out_scatter(uint64_t n, const uint64_t* __restrict__ off, const uint32_t* __restrict__ src,
+                              const int32_t* __restrict__ w, const uint64_t* __restrict__ out_off,
+                              uint32_t* __restrict__ out_cnt, uint32_t* __restrict__ out_tgt, int32_t* __restrict__ out_w) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < n; t += stride) {
+        for (uint64_t e = off[t]; e < off[t + 1]; e++) {
+            const int32_t we = w[e];
+            if (we != 0) {
+                const uint32_t s = src[e];
+                const uint64_t pos = out_off[s] + atomicSub(&out_cnt[s], 1U) - 1U;
+                out_tgt[pos] = static_cast<uint32_t>(t);
+                out_w[pos] = we;
+            }
+        }
+    }
+}
+
+// acc[t] += the weights of the pairs whose source has fired set (acc starts at zero).
+__global__ void k_syn_push_all(uint64_t n, const uint8_t* __restrict__ fired, const uint64_t* __restrict__ out_off,
+                               const uint32_t* __restrict__ out_tgt, const int32_t* __restrict__ out_w,
+                               int32_t* __restrict__ acc) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t s = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; s < n; s += stride) {
+        if (fired[s] != 0) {
+            for (uint64_t e = out_off[s]; e < out_off[s + 1]; e++) {
+                atomicAdd(&acc[out_tgt[e]], out_w[e]);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- deletion
 
 struct DelArgs {
@@ -368,6 +415,8 @@ int net_alloc_edges(rl_bh_ctx* c, uint64_t cap) {
     RL_TRY(dalloc(c, &g.src2, cap));
     RL_TRY(dalloc(c, &g.w2, cap));
     RL_TRY(dalloc(c, &g.olist, cap));
+    RL_TRY(dalloc(c, &g.out_tgt, cap));
+    RL_TRY(dalloc(c, &g.out_w, cap));
     g.cap = cap;
     return RL_OK;
 }
@@ -439,8 +488,15 @@ void net_free(rl_bh_ctx* c) {
     dfree(&g.ocnt);
     dfree(&g.ooff);
     dfree(&g.olist);
+    dfree(&g.out_off);
+    dfree(&g.out_cnt);
+    dfree(&g.out_tgt);
+    dfree(&g.out_w);
+    dfree(&g.acc[0]);
+    dfree(&g.acc[1]);
     g.cap = 0;
     g.ready = false;
+    g.syn_valid = false;
 }
 
 int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const uint32_t* sources,
@@ -478,6 +534,7 @@ int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const 
                 pairs.emplace_back(sources[e], weights[e]);
             }
             std::sort(pairs.begin(), pairs.end());
+            int64_t in_weight = 0;   // the synaptic input sums a neuron's in-weights in 32 bits
             for (size_t i = 0; i < pairs.size();) {
                 int64_t sum = 0;
                 size_t j = i;
@@ -485,8 +542,9 @@ int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const 
                     sum += pairs[j].second;
                 }
                 if (sum != 0) {
-                    if (sum > INT32_MAX || sum < INT32_MIN) {
-                        return fail(c, RL_EINVAL, "rl_net_init: pair weight out of range");
+                    in_weight += sum < 0 ? -sum : sum;
+                    if (sum > INT32_MAX || sum < INT32_MIN || in_weight > INT32_MAX) {
+                        return fail(c, RL_EINVAL, "rl_net_init: pair weight or summed in-weight of a neuron out of range");
                     }
                     src.push_back(pairs[i].first);
                     w.push_back(static_cast<int32_t>(sum));
@@ -508,6 +566,10 @@ int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const 
     RL_TRY(dalloc(c, &g.npre, n + 1));
     RL_TRY(dalloc(c, &g.ocnt, n));
     RL_TRY(dalloc(c, &g.ooff, n + 1));
+    RL_TRY(dalloc(c, &g.out_off, n + 1));
+    RL_TRY(dalloc(c, &g.out_cnt, n));
+    RL_TRY(dalloc(c, &g.acc[0], n));
+    RL_TRY(dalloc(c, &g.acc[1], n));
     RL_TRY(net_alloc_edges(c, std::max<uint64_t>(capacity, 1)));
     hipStream_t s = c->stream;
     RL_HIP(c, hipMemcpyAsync(g.off, off.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
@@ -518,6 +580,7 @@ int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const 
     RL_HIP(c, hipMemsetAsync(g.ocnt, 0, n * sizeof(uint32_t), s));
     RL_HIP(c, hipStreamSynchronize(s));
     g.ready = true;
+    g.syn_valid = false;
     return RL_OK;
 }
 
@@ -553,6 +616,40 @@ int net_get(rl_bh_ctx* c, uint64_t capacity, uint64_t* in_offsets, uint32_t* sou
     return RL_OK;
 }
 
+// Rebuilds the out-edge lists from the graph and the accumulators from the fired status of
+// fired_buf[parity]: the state the step kernel expects at that parity.  Allocates nothing and reads
+// no count on the host, so a captured period can end with it.
+int enqueue_syn_sync(rl_bh_ctx* c, int parity) {
+    NetGraph& g = c->net;
+    const uint64_t n = c->n;
+    if (!g.ready || n == 0 || !c->neurons.ready) {
+        return RL_OK;
+    }
+    hipStream_t s = c->stream;
+    RL_TRY(fill_u32(c, g.out_cnt, 0U, n, nullptr, 0));
+    k_out_count<<<kStreamBlocks, kBlock, 0, s>>>(n, g.off, g.src, g.w, g.out_cnt);
+    RL_HIP(c, hipGetLastError());
+    RL_TRY(scan_counts(c, g.out_cnt, n, g.out_off));
+    k_out_scatter<<<kStreamBlocks, kBlock, 0, s>>>(n, g.off, g.src, g.w, g.out_off, g.out_cnt, g.out_tgt, g.out_w);
+    RL_HIP(c, hipGetLastError());
+    RL_TRY(fill_u32(c, reinterpret_cast<uint32_t*>(g.acc[0]), 0U, n, nullptr, 0));
+    RL_TRY(fill_u32(c, reinterpret_cast<uint32_t*>(g.acc[1]), 0U, n, nullptr, 0));
+    k_syn_push_all<<<kStreamBlocks, kBlock, 0, s>>>(n, c->neurons.fired_buf[parity], g.out_off, g.out_tgt, g.out_w,
+                                                    g.acc[parity]);
+    RL_HIP(c, hipGetLastError());
+    return RL_OK;
+}
+
+int syn_prepare(rl_bh_ctx* c) {
+    NetGraph& g = c->net;
+    if (!g.ready || g.syn_valid || c->n == 0 || !c->neurons.ready) {
+        return RL_OK;
+    }
+    RL_TRY(enqueue_syn_sync(c, c->neurons.parity));
+    g.syn_valid = true;
+    return RL_OK;
+}
+
 int enqueue_net_append(rl_bh_ctx* c) {
     if (!c->net.ready || c->n == 0) {
         return RL_OK;
@@ -582,11 +679,16 @@ static int net_grow(rl_bh_ctx* c, uint64_t have, uint64_t need) {
     dfree(&g.src2);
     dfree(&g.w2);
     dfree(&g.olist);
+    dfree(&g.out_tgt);
+    dfree(&g.out_w);
     g.src = src;
     g.w = w;
+    g.syn_valid = false;
     RL_TRY(dalloc(c, &g.src2, cap));
     RL_TRY(dalloc(c, &g.w2, cap));
     RL_TRY(dalloc(c, &g.olist, cap));
+    RL_TRY(dalloc(c, &g.out_tgt, cap));
+    RL_TRY(dalloc(c, &g.out_w, cap));
     g.cap = cap;
     return RL_OK;
 }
@@ -611,6 +713,7 @@ int net_append(rl_bh_ctx* c) {
     if (!c->net.ready || c->n == 0) {
         return RL_OK;
     }
+    c->net.syn_valid = false;
     RL_TRY(enqueue_net_count(c));
     return net_merge_grow(c);
 }
@@ -621,6 +724,7 @@ int net_append_retry(rl_bh_ctx* c) {
     if (!c->net.ready || c->n == 0) {
         return RL_OK;
     }
+    c->net.syn_valid = false;
     RL_TRY(net_merge_grow(c));
     unsigned long long over = 0;
     RL_HIP(c, hipMemcpyAsync(&over, c->counters + kCtrNetOver, sizeof(over), hipMemcpyDeviceToHost, c->stream));

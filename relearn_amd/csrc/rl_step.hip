@@ -64,21 +64,49 @@ struct StepArgs {
     double* u;
     double* ca;
     const double* input;
-    const uint8_t* fired_prev;   // fired status of the previous step (synaptic input, disabled carry)
+    const uint8_t* fired_prev;   // fired status of the previous step (disabled carry)
     uint8_t* fired;
     const double* target_ca;
     double* delta;
     const uint8_t* enabled;
-    // synaptic input: in-edges CSR by target, or null
-    const uint64_t* in_off;
-    const uint32_t* in_src;
-    const int32_t* in_w;
+    // synaptic input, or null: acc_cur holds the summed weights that the sources which fired in the
+    // previous step pushed; a neuron that fires in this step pushes its out-pairs into acc_next
+    int32_t* acc_cur;
+    int32_t* acc_next;
+    const uint64_t* out_off;
+    const uint32_t* out_tgt;
+    const int32_t* out_w;
     double conductance;
     // 1 / h and -1 / tau_ca, divided once on the host (IEEE division is correctly rounded on both
     // sides, so the values are the ones the reference's loops use)
     double scale;
     double tau_inv;
 };
+
+// A neuron that fired adds the weight of each of its out-pairs to the target's input of the next
+// step (integer atomics: the sums do not depend on their order).  Few neurons fire per step, so
+// this replaces a gather over every in-edge of every neuron.
+// Pairs are read four at a time, so that their loads overlap.
+__device__ __forceinline__ void push_out_pairs(const StepArgs& a, uint64_t i) {
+    const uint64_t e0 = a.out_off[i];
+    const uint64_t e1 = a.out_off[i + 1];
+    for (uint64_t e = e0; e < e1; e += 4) {
+        uint32_t t[4];
+        int32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool in = e + k < e1;
+            t[k] = in ? a.out_tgt[e + k] : 0U;
+            w[k] = in ? a.out_w[e + k] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (w[k] != 0) {
+                atomicAdd(&a.acc_next[t[k]], w[k]);
+            }
+        }
+    }
+}
 
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 #pragma clang fp contract(off)
@@ -99,40 +127,29 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     const double tgt = a.target_ca[i];
     const double dl[3] = { a.delta[i], a.delta[n + i], a.delta[2 * n + i] };
     const double caller_input = p.input_mode == 0 ? a.input[i] : 0.0;
-    uint64_t eb = 0, ee = 0;
-    if (a.in_off != nullptr) {
-        eb = a.in_off[i];
-        ee = a.in_off[i + 1];
-    }
+    const bool synaptic = a.acc_cur != nullptr;
+    const int32_t syn = synaptic ? a.acc_cur[i] : 0;
     if (en == 0) {
         a.fired[i] = fprev;  // the step leaves a disabled neuron's status as it was
+        if (synaptic) {
+            a.acc_cur[i] = 0;
+            if (fprev != 0) {
+                push_out_pairs(a, i);
+            }
+        }
         return;
     }
     const uint32_t step = a.step_base != nullptr ? *a.step_base + a.step : a.step;
     double input = p.input_mode == 0 ? caller_input
                                      : normal_input(a.net_seed[g], static_cast<uint32_t>(i - a.net_off[g]), step,
                                                     p.input_mean, p.input_stddev);
-    if (a.in_off != nullptr) {
+    if (synaptic) {
         // SynapticEquallyWeightedActivityInputCPU::update_local_input: the weights of the in-edges
-        // whose source fired in the previous step, times the conductance; the integer sum is what the
-        // reference's FP64 accumulation of integer weights yields.  CombinedActivityInputCPU adds it
-        // to the background input.  Four edges at a time, so that their gathers overlap.
-        int64_t acc = 0;
-        for (uint64_t e = eb; e < ee; e += 4) {
-            uint32_t src[4];
-            int32_t w[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const bool in = e + k < ee;
-                src[k] = in ? a.in_src[e + k] : 0U;
-                w[k] = in ? a.in_w[e + k] : 0;
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                acc += a.fired_prev[src[k]] != 0 ? w[k] : 0;
-            }
-        }
-        input = input + static_cast<double>(acc) * a.conductance;
+        // whose source fired in the previous step, times the conductance.  The sum was pushed by those
+        // sources; it is an integer, which the reference's FP64 accumulation of integer weights
+        // yields exactly.  CombinedActivityInputCPU adds it to the background input.
+        input = input + static_cast<double>(syn) * a.conductance;
+        a.acc_cur[i] = 0;
     }
     const double scale = a.scale;
     const double ab = p.a * p.b;
@@ -152,6 +169,9 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     a.fired[i] = spiked;
     a.x[i] = xv;
     a.u[i] = uv;
+    if (synaptic && spiked != 0) {
+        push_out_pairs(a, i);
+    }
     const double tau_inv = a.tau_inv;
     const double local_beta = p.beta_ca * static_cast<double>(spiked == 1);
     for (uint32_t s = 0; s < p.h; s++) {
@@ -298,6 +318,7 @@ int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const d
     }
     RL_HIP(c, hipStreamSynchronize(st));
     s.ready = true;
+    c->net.syn_valid = false;   // new fired status
     return RL_OK;
 }
 
@@ -326,9 +347,11 @@ StepArgs step_args(rl_bh_ctx* c) {
     a.scale = 1.0 / static_cast<double>(s.p.h);
     a.tau_inv = -1.0 / s.p.tau_ca;
     if (s.p.synaptic_input != 0) {
-        a.in_off = c->net.off;
-        a.in_src = c->net.src;
-        a.in_w = c->net.w;
+        a.acc_cur = c->net.acc[s.parity];
+        a.acc_next = c->net.acc[s.parity ^ 1];
+        a.out_off = c->net.out_off;
+        a.out_tgt = c->net.out_tgt;
+        a.out_w = c->net.out_w;
         a.conductance = s.p.synapse_conductance;
     }
     return a;
@@ -363,6 +386,9 @@ int neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
         return RL_OK;
     }
     RL_TRY(check_synaptic(c));
+    if (s.p.synaptic_input != 0) {
+        RL_TRY(syn_prepare(c));
+    }
     if (s.p.input_mode == 0) {
         RL_HIP(c, hipMemcpyAsync(s.input, input, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
@@ -385,6 +411,7 @@ int normalize_fired(rl_bh_ctx* c) {
     RL_HIP(c, hipGetLastError());
     s.parity = 0;
     s.fired = s.fired_buf[0];
+    c->net.syn_valid = false;   // the accumulators were those of parity 1
     return RL_OK;
 }
 
@@ -409,6 +436,10 @@ int enqueue_neuron_steps(rl_bh_ctx* c, uint32_t steps) {
         a.step = k;
         a.fired_prev = s.fired_buf[k & 1U];
         a.fired = s.fired_buf[(k + 1) & 1U];
+        if (a.acc_cur != nullptr) {
+            a.acc_cur = c->net.acc[k & 1U];
+            a.acc_next = c->net.acc[(k + 1) & 1U];
+        }
         k_step<<<grid_for(c->n), kBlock, 0, c->stream>>>(a);
         RL_HIP(c, hipGetLastError());
     }
@@ -453,6 +484,9 @@ int enqueue_elements_commit(rl_bh_ctx* c) {
 int neurons_commit(rl_bh_ctx* c, uint32_t update, uint32_t* deleted) {
     if (c->neurons.ready && c->n > 0) {
         RL_TRY(set_steps(c, 0, update));
+    }
+    if (c->net.ready) {
+        c->net.syn_valid = false;   // deletions change the pairs
     }
     RL_TRY(enqueue_elements_commit(c));
     if (deleted != nullptr && c->n > 0) {
