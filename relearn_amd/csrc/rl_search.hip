@@ -204,9 +204,50 @@ __device__ uint32_t list_pick(const SearchArgs& a, const Lane& L, uint32_t root,
     return t;
 }
 
+// Counter increments of a lane, added to the counter block once per thread block at the end of a
+// kernel: an atomic per source on the same counters serialises.
+struct Totals {
+    unsigned long long exec = 0;
+    unsigned long long ref = 0;
+    unsigned long long searched = 0;
+};
+
+__device__ __forceinline__ unsigned long long wave_total(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, 64);
+    }
+    return v;
+}
+
+// Sums the lanes' totals over the block (all threads must call it) and adds them to the counters.
+__device__ __forceinline__ void add_totals(unsigned long long* counters, const Totals& t) {
+    __shared__ unsigned long long part[3][kBlock / 64];
+    const unsigned long long v[3] = { wave_total(t.exec), wave_total(t.ref), wave_total(t.searched) };
+    const uint32_t wv = threadIdx.x / 64U;
+    if ((threadIdx.x & 63U) == 0) {
+        for (int k = 0; k < 3; k++) {
+            part[k][wv] = v[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s[3] = { 0, 0, 0 };
+        for (uint32_t w = 0; w < blockDim.x / 64U; w++) {
+            for (int k = 0; k < 3; k++) {
+                s[k] += part[k][w];
+            }
+        }
+        if (s[0] != 0) atomicAdd(&counters[kCtrTestsExec], s[0]);
+        if (s[1] != 0) atomicAdd(&counters[kCtrTestsRef], s[1]);
+        if (s[2] != 0) atomicAdd(&counters[kCtrSearches], s[2]);
+    }
+}
+
 // The whole search of source src by one lane; stk is the lane's LDS stack column (stride apart).
 template <bool kMixed, bool kFallback>
-__device__ __forceinline__ void search_source(const SearchArgs& a, uint32_t src, uint32_t* stk, uint32_t stride) {
+__device__ __forceinline__ void search_source(const SearchArgs& a, uint32_t src, uint32_t* stk, uint32_t stride,
+                                              Totals& tot) {
     const uint32_t na = a.vac_axon[src];
     const uint64_t slot0 = a.slot_off[src];
     if (a.enabled[src] == 0 || na == 0) {
@@ -234,7 +275,7 @@ __device__ __forceinline__ void search_source(const SearchArgs& a, uint32_t src,
         }
         a.found[src] = f;
         a.searched[src] = f == na ? na : 1;
-        atomicAdd(&a.counters[kCtrSearches], static_cast<unsigned long long>(f == na ? na : 1));
+        tot.searched += f == na ? na : 1;
         return;
     }
     // level 1: one list for all axons of the neuron
@@ -248,9 +289,9 @@ __device__ __forceinline__ void search_source(const SearchArgs& a, uint32_t src,
         a.slot_tests[slot0] = l1.tests_ref;
         a.found[src] = 0;
         a.searched[src] = 1;
-        atomicAdd(&a.counters[kCtrTestsExec], static_cast<unsigned long long>(exec));
-        atomicAdd(&a.counters[kCtrTestsRef], static_cast<unsigned long long>(l1.tests_ref));
-        atomicAdd(&a.counters[kCtrSearches], 1ULL);
+        tot.exec += exec;
+        tot.ref += l1.tests_ref;
+        tot.searched += 1;
         return;
     }
     const double next1 = nextafter(l1.sum, l1.sum + 0.00001);
@@ -321,9 +362,9 @@ __device__ __forceinline__ void search_source(const SearchArgs& a, uint32_t src,
     }
     a.found[src] = found;
     a.searched[src] = searched;
-    atomicAdd(&a.counters[kCtrTestsExec], static_cast<unsigned long long>(exec));
-    atomicAdd(&a.counters[kCtrTestsRef], static_cast<unsigned long long>(ref));
-    atomicAdd(&a.counters[kCtrSearches], static_cast<unsigned long long>(searched));
+    tot.exec += exec;
+    tot.ref += ref;
+    tot.searched += searched;
 }
 
 // Sources a.order[0 .. *a.nwork), grid-stride, one lane per source.
@@ -332,16 +373,19 @@ __global__ void __launch_bounds__(kBlock) k_search(SearchArgs a) {
     extern __shared__ uint32_t lds_stack[];
     const uint64_t count = *a.nwork;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    Totals tot;
     for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < count; t += stride) {
-        search_source<kMixed, kFallback>(a, a.order[t], lds_stack + threadIdx.x, blockDim.x);
+        search_source<kMixed, kFallback>(a, a.order[t], lds_stack + threadIdx.x, blockDim.x, tot);
     }
+    add_totals(a.counters, tot);
 }
 
 // Descents that the cooperative kernels hand over (a level that needs expansion or the reference's
 // fallbacks): one lane continues the search of one slot from (node, level) with the general
 // traversal, as the descent loop of k_search does.
 template <bool kMixed>
-__device__ __forceinline__ void descend_item(const SearchArgs& a, const uint4 it, uint32_t* stk, uint32_t stride) {
+__device__ __forceinline__ void descend_item(const SearchArgs& a, const uint4 it, uint32_t* stk, uint32_t stride,
+                                             Totals& tot) {
     const uint32_t slot = it.x, src = it.y;
     uint32_t node = it.z;
     Lane L;
@@ -380,7 +424,7 @@ __device__ __forceinline__ void descend_item(const SearchArgs& a, const uint4 it
     }
     a.slot_tests[slot] += tests;
     a.slot_target[slot] = ok ? node : kNone;
-    atomicAdd(&a.counters[kCtrTestsExec], static_cast<unsigned long long>(exec));
+    tot.exec += exec;
 }
 
 template <bool kMixed>
@@ -389,9 +433,11 @@ __global__ void __launch_bounds__(kBlock) k_descend_lane(SearchArgs a, const uin
     extern __shared__ uint32_t lds_stack[];
     const uint64_t count = *n_items;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    Totals tot;
     for (uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; t < count; t += stride) {
-        descend_item<kMixed>(a, items[t], lds_stack + threadIdx.x, blockDim.x);
+        descend_item<kMixed>(a, items[t], lds_stack + threadIdx.x, blockDim.x, tot);
     }
+    add_totals(a.counters, tot);
 }
 
 __global__ void k_emit_requests(uint64_t n, const uint32_t* __restrict__ found, const uint64_t* __restrict__ req_off,

@@ -740,44 +740,59 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // Per source, in axon order: a search that fails ends the source's searches (find_target_neurons,
 // BarnesHutBase.h:283-289).  Counts the searches, the successful ones and the reference's tests.
-__global__ void __launch_bounds__(256) k_finalize(SearchArgs s, const uint32_t* __restrict__ work,
-                                                  const uint64_t* __restrict__ n_work) {
-    const uint64_t nw = *n_work;
-    if (blockIdx.x * static_cast<uint64_t>(blockDim.x) >= nw) {
-        return;  // the grid covers every neuron; only the active sources have work
-    }
-    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+// Neurons in index order, grid-stride, so that neighbouring lanes read neighbouring slots (the work
+// list is in Morton order, whose slots are scattered).  The totals are summed per block and added
+// with one atomic per block: atomics of every wave on the same two counters serialise (they took
+// 6 ms at 16M).
+__global__ void __launch_bounds__(256) k_finalize(SearchArgs s) {
+    __shared__ unsigned long long part[2][256 / kWave];
     unsigned long long ref = 0, searched_n = 0;
-    if (t < nw) {
-        const uint32_t src = work[t];
-        if (s.found[src] != kOverflowMark) {
-            const uint32_t na = s.vac_axon[src];
-            const uint64_t slot0 = s.slot_off[src];
-            uint32_t found = 0, searched = 0;
-            uint32_t jj = 0;
-            for (; jj < na; jj++) {
-                searched++;
-                ref += s.slot_tests[slot0 + jj];
-                if (s.slot_target[slot0 + jj] == kNone) {
-                    break;
-                }
-                found++;
-            }
-            // draws after the first failure were resolved speculatively: the reference never makes them
-            for (jj = jj + 1; jj < na; jj++) {
-                s.slot_target[slot0 + jj] = kNone;
-                s.slot_tests[slot0 + jj] = 0;
-            }
-            s.found[src] = found;
-            s.searched[src] = searched;
-            searched_n = searched;
+    const bool over = s.counters[kCtrOverCap] != 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t src = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; src < s.n; src += stride) {
+        // the active sources of k_active_flags: enabled, with vacant axons, and no over-capacity update
+        if (over || s.enabled[src] == 0 || s.vac_axon[src] == 0 || s.found[src] == kOverflowMark) {
+            continue;
         }
+        const uint32_t na = s.vac_axon[src];
+        const uint64_t slot0 = s.slot_off[src];
+        uint32_t found = 0, searched = 0;
+        uint32_t jj = 0;
+        for (; jj < na; jj++) {
+            searched++;
+            ref += s.slot_tests[slot0 + jj];
+            if (s.slot_target[slot0 + jj] == kNone) {
+                break;
+            }
+            found++;
+        }
+        // draws after the first failure were resolved speculatively: the reference never makes them
+        for (jj = jj + 1; jj < na; jj++) {
+            s.slot_target[slot0 + jj] = kNone;
+            s.slot_tests[slot0 + jj] = 0;
+        }
+        s.found[src] = found;
+        s.searched[src] = searched;
+        searched_n += searched;
     }
     ref = wave_sum(ref);
     searched_n = wave_sum(searched_n);
-    if ((threadIdx.x & (kWave - 1)) == 0 && searched_n != 0) {
-        atomicAdd(&s.counters[kCtrTestsRef], ref);
-        atomicAdd(&s.counters[kCtrSearches], searched_n);
+    const uint32_t wv = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        part[0][wv] = ref;
+        part[1][wv] = searched_n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long r = 0, sn = 0;
+        for (uint32_t k = 0; k < 256 / kWave; k++) {
+            r += part[0][k];
+            sn += part[1][k];
+        }
+        if (sn != 0) {
+            atomicAdd(&s.counters[kCtrTestsRef], r);
+            atomicAdd(&s.counters[kCtrSearches], sn);
+        }
     }
 }
 
@@ -818,7 +833,7 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa) {
     if (timed) RL_HIP(c, hipEventRecord(c->ev3, s));
     RL_TRY(launch_descend_lane(c, sa, c->slow_items, c->counters + kCtrSlow));
     const uint64_t n = std::max<uint64_t>(c->n, 1);
-    k_finalize<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(sa, c->coop_work, sa.nwork);
+    k_finalize<<<static_cast<unsigned>(std::min<uint64_t>((n + 255) / 256, kStreamBlocks)), 256, 0, s>>>(sa);
     RL_HIP(c, hipGetLastError());
     return RL_OK;
 }
