@@ -249,12 +249,15 @@ int rl_neurons_get(rl_bh_ctx* ctx, double* x, double* u, double* ca, uint8_t* fi
  * [in_offsets[t], in_offsets[t+1]) of the n + 1 offsets.  in_offsets NULL starts with no edge.  From
  * then on every commit appends the synapses it creates, as Neurons::create_synapses adds them to the
  * graph (Neurons.cpp:440-477): weight +1 for an excitatory source, -1 for an inhibitory one
- * (Connector.cpp:33-121).  capacity: edges held without reallocation; the eager calls grow it, a
- * captured period reports RL_ELIMIT instead.  rl_step_params.synaptic_input = 1 adds
- * synapse_conductance times the summed weights of the in-edges whose source fired in the previous
- * step to the input of each step (SynapticEquallyWeightedActivityInputCPU::update_local_input,
+ * (Connector.cpp:33-121).  Repeated (source, target) pairs are merged into one entry whose weight
+ * is their sum, and pairs of weight 0 are dropped, as NetworkGraph keeps them.  capacity: entries
+ * held without reallocation; the eager calls grow it, and so does rl_bh_graph_launch with stats
+ * (see there).  rl_step_params.synaptic_input = 1 adds synapse_conductance times the summed weights
+ * of the in-edges whose source fired in the previous step to the input of each step
+ * (SynapticEquallyWeightedActivityInputCPU::update_local_input,
  * SynapticEquallyWeightedActivityInputCPU.cpp:20-50, combined with the background input as
- * CombinedActivityInputCPU.h does). */
+ * CombinedActivityInputCPU.h does); the sum is formed in 32-bit integers, so a neuron's summed
+ * |weights| must stay below 2^31 (checked here, RL_EINVAL). */
 int rl_net_init(rl_bh_ctx* ctx, uint64_t capacity, const uint64_t* in_offsets, const uint32_t* sources,
                 const int32_t* weights);
 
