@@ -1,5 +1,7 @@
-"""End-to-end parity of simulations with the oracle: 1000 neurons in a 1000 um cube (80/20 E/I), and
-the reference's real 51,842-neuron positions with its 425,706-synapse input network.
+"""End-to-end parity of simulations with the oracle: 1000 neurons in a 1000 um cube (80/20 E/I) with
+thresholds that make elements grow and shrink, the CPU reference configuration itself (1000 neurons,
+20,000 steps, the reference's thresholds), and the reference's real 51,842-neuron positions with its
+425,706-synapse input network.
 
 Each period runs 100 neuron steps with background input (the same caller arrays on both sides) and
 synaptic input over the network the updates build, the element commit of the three element types
@@ -37,11 +39,12 @@ def _pairs(off, src, w):
     return {k: v for k, v in out.items() if v != 0}
 
 
-def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed, rng, cond=1.5):
-    """Runs `periods` periods on the device and in the oracle side by side and compares them."""
+def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed, rng, cond=1.5,
+              eta=(0.02, 0.01, 0.0), target=0.1, nu=0.02, ca0=(0.05, 0.15), start_elements=True, check_every=1):
+    """Runs `periods` periods on the device and in the oracle side by side and compares them every
+    `check_every` periods and after the last."""
     n = len(xyz)
-    eta, target = (0.02, 0.01, 0.0), 0.1
-    pg = StepParams.izhikevich(input_mode=0, nu=0.02)
+    pg = StepParams.izhikevich(input_mode=0, nu=nu)
     pg.synaptic_input, pg.synapse_conductance = 1, cond
     po = oracle.StepParams()
     for f in ("a", "b", "c", "d", "v_spike", "k1", "k2", "k3", "h", "tau_ca", "beta_ca"):
@@ -57,11 +60,14 @@ def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed
     np.add.at(conn[0], es, np.abs(ew))
     np.add.at(conn[1], t_of[ew > 0], ew[ew > 0])
     np.add.at(conn[2], t_of[ew < 0], -ew[ew < 0])
-    grown = conn + (rng.poisson(1.0, (3, n)) if graph is None else rng.uniform(0.0, 1.5, (3, n)))
+    if start_elements:
+        grown = conn + (rng.poisson(1.0, (3, n)) if graph is None else rng.uniform(0.0, 1.5, (3, n)))
+    else:
+        grown = conn.copy()
     grown = grown.astype(np.float64)
     x = np.full(n, -65.0)
     u = np.full(n, -13.0)
-    ca = rng.uniform(0.05, 0.15, n)
+    ca = rng.uniform(ca0[0], ca0[1], n) if ca0 is not None else np.zeros(n)
     tca = np.full(n, target)
 
     g = BarnesHutHIP(box_min, box_max, seed=seed, precision=RL_PREC_FP64)
@@ -110,9 +116,11 @@ def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed
         og.add(rt[acc], rs[acc], np.where(rg[acc] != 0, -1, 1))
         st = g.update_connectivity(per)
         created_total += created
+        assert st["requests"] == len(rt) and st["created"] == created, per
+        if (per + 1) % check_every != 0 and per + 1 != periods:
+            continue
         _say("period", per, "requests", len(rt), "created", created, "deleted so far", deleted_total)
         # compare the whole state
-        assert st["requests"] == len(rt) and st["created"] == created, per
         sg = g.neurons_get()
         for key, ref in (("x", xo), ("u", uo), ("ca", co), ("fired", fo)):
             assert np.array_equal(sg[key], ref), (per, key)
@@ -136,6 +144,21 @@ def test_small_simulation_matches_oracle(oracle):
     created, deleted = _simulate(oracle, net.xyz, net.signal, net.box_min, net.box_max, None, periods, steps, seed,
                                  np.random.default_rng(17))
     assert created > 0 and deleted > 0
+
+
+def test_cfg0_full_simulation_matches_oracle(oracle):
+    # the CPU reference configuration: 1,000 neurons uniform in a 1000 um cube (seed 0, 80/20 E/I),
+    # Izhikevich defaults, Normal(5, 2) input, calcium tau 1e4 and beta 1e-3, growth rate 1e-4 and the
+    # reference's thresholds (eta 0.4 / 0.1 / 0.0, target calcium 0.7), elements and calcium from 0,
+    # synaptic input at conductance 0.03, plasticity every 100 steps, 20,000 steps; the state must
+    # stay identical for the whole run (calcium stays below the axons' eta, so no axon grows and no
+    # synapse forms: the reference's own outcome for this configuration)
+    n, periods, steps, seed = 1000, 200, 100, 0
+    _say("cfg0 simulation parity", n, "neurons,", periods, "periods of", steps, "steps")
+    net = uniform_cube(n, 1000.0, 0)
+    _simulate(oracle, net.xyz, net.signal, net.box_min, net.box_max, None, periods, steps, seed,
+              np.random.default_rng(19), cond=0.03, eta=(0.4, 0.1, 0.0), target=0.7, nu=1e-4, ca0=None,
+              start_elements=False, check_every=20)
 
 
 def test_real_network_simulation_matches_oracle(oracle):
