@@ -102,6 +102,8 @@ struct Neurons {
     double* u = nullptr;
     double* ca = nullptr;
     double* target_ca = nullptr;
+    bool target_uniform = false;  // every neuron has target_value: the step reads no per-neuron target
+    double target_value = 0.0;
     double* input = nullptr;
     uint8_t* fired_buf[2] = { nullptr, nullptr };  // fired status of the last two steps
     uint8_t* fired = nullptr;     // fired_buf[parity]: the last step's

@@ -7,7 +7,9 @@
 //                                       (source/neurons/synaptic_elements/SynapticElementsCPU.cpp:25-49,
 //                                        GrowthCurves.h:30-54)
 // and, at plasticity steps, ElementBaseBase::update (ElementBaseBase.h:420-487) per element type.
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "rl_device.h"
 #include "rl_internal.h"
@@ -54,7 +56,7 @@ __device__ __forceinline__ double normal_input(uint64_t seed, uint32_t neuron, u
 struct StepArgs {
     uint64_t n;
     rl_step_params p;
-    const uint32_t* net_of;      // network of each neuron
+    const uint32_t* net_of;      // network of each neuron, or null: one network
     const uint64_t* net_off;     // first neuron of each network
     const uint64_t* net_seed;    // draw seed of each network
     const double* net_nu;        // growth rates per network (3 each) or null: p.nu for all
@@ -66,7 +68,8 @@ struct StepArgs {
     const double* input;
     const uint8_t* fired_prev;   // fired status of the previous step (disabled carry)
     uint8_t* fired;
-    const double* target_ca;
+    const double* target_ca;     // per-neuron target calcium, or null: target_value for all
+    double target_value;
     double* delta;
     const uint8_t* enabled;
     // synaptic input, or null: acc_cur holds the summed weights that the sources which fired in the
@@ -120,11 +123,11 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     // after another
     const uint8_t en = a.enabled[i];
     const uint8_t fprev = a.fired_prev[i];
-    const uint32_t g = a.net_of[i];
+    const uint32_t g = a.net_of != nullptr ? a.net_of[i] : 0U;
     double xv = a.x[i];
     double uv = a.u[i];
     double c = a.ca[i];
-    const double tgt = a.target_ca[i];
+    const double tgt = a.target_ca != nullptr ? a.target_ca[i] : a.target_value;
     const double dl[3] = { a.delta[i], a.delta[n + i], a.delta[2 * n + i] };
     const double caller_input = p.input_mode == 0 ? a.input[i] : 0.0;
     const bool synaptic = a.acc_cur != nullptr;
@@ -307,6 +310,11 @@ int neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, const d
     RL_TRY(up(s.u, u, n));
     RL_TRY(up(s.ca, ca, n));
     RL_TRY(up(s.target_ca, target_ca, n));
+    // the same bits everywhere (not merely equal values: -0 and +0 stay per-neuron)
+    s.target_uniform = n > 0 && target_ca != nullptr &&
+                       std::all_of(target_ca, target_ca + n,
+                                   [&](double v) { return std::memcmp(&v, target_ca, sizeof(double)) == 0; });
+    s.target_value = n > 0 && target_ca != nullptr ? target_ca[0] : 0.0;
     RL_TRY(up(s.grown, grown, 3 * n));
     RL_HIP(c, hipMemsetAsync(s.delta, 0, 3 * n * sizeof(double), st));
     RL_HIP(c, hipMemsetAsync(s.fired_buf[0], 0, n, st));
@@ -329,7 +337,7 @@ StepArgs step_args(rl_bh_ctx* c) {
     StepArgs a{};
     a.n = c->n;
     a.p = s.p;
-    a.net_of = c->net_of;
+    a.net_of = c->num_networks > 1 ? c->net_of : nullptr;   // one network: every neuron is in network 0
     a.net_off = c->net_off;
     a.net_seed = c->net_seed;
     a.net_nu = s.net_nu;
@@ -341,7 +349,8 @@ StepArgs step_args(rl_bh_ctx* c) {
     a.input = s.input;
     a.fired_prev = s.fired_buf[s.parity];
     a.fired = s.fired_buf[s.parity ^ 1];
-    a.target_ca = s.target_ca;
+    a.target_ca = s.target_uniform ? nullptr : s.target_ca;
+    a.target_value = s.target_value;
     a.delta = s.delta;
     a.enabled = c->enabled;
     a.scale = 1.0 / static_cast<double>(s.p.h);

@@ -244,8 +244,10 @@ def _izh(oracle_mod):
     return p
 
 
-def test_neuron_step_parity(oracle):
-    _say("neuron step parity")
+@pytest.mark.parametrize("targets", ["uniform", "per-neuron"])
+def test_neuron_step_parity(oracle, targets):
+    # one target calcium for every neuron (the step then reads no per-neuron target) or one each
+    _say("neuron step parity", targets)
     from relearn_amd import StepParams
 
     n, steps = 64, 10
@@ -261,7 +263,7 @@ def test_neuron_step_parity(oracle):
     x = rng.uniform(-70, 20, n)
     u = rng.uniform(-15, 5, n)
     ca = rng.uniform(0, 1, n)
-    tca = np.full(n, 0.7)
+    tca = np.full(n, 0.7) if targets == "uniform" else rng.uniform(0.05, 0.9, n)
     en = np.ones(n, np.uint8)
     en[::9] = 0
     g = BarnesHutHIP((0, 0, 0), (1, 1, 1))
