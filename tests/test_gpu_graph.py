@@ -161,6 +161,62 @@ def test_graph_ensemble_period_matches_eager():
     graph.close()
 
 
+def _pairs(off, src, w):
+    t = np.repeat(np.arange(len(off) - 1, dtype=np.int64), np.diff(off.astype(np.int64)))
+    out = {}
+    for a, b, c in zip(t.tolist(), src.tolist(), w.tolist()):
+        out[(a, b)] = out.get((a, b), 0) + c
+    return {k: v for k, v in out.items() if v != 0}
+
+
+def test_graph_ensemble_with_synaptic_input_and_deletion_matches_eager():
+    # an ensemble whose periods push synaptic input over the graph the updates build, delete the
+    # synapses of shrinking elements and outgrow the graph's initial capacity
+    from relearn_amd.workloads import parameter_sweep
+
+    ens = parameter_sweep(num_networks=12, n_min=500, n_max=5000, seed=6)
+    n, steps, periods = ens.n, 15, 4
+    _say("graph ensemble period with synaptic input and deletion vs eager,", n, "neurons")
+    lo, hi = ens.boxes[:, :3].min(0), ens.boxes[:, 3:].max(0)
+    ctxs = []
+    for _ in range(2):
+        c = BarnesHutHIP(lo, hi, seed=5, precision=RL_PREC_FP32_MIXED)
+        c.init_networks(ens.offsets, ens.boxes, ens.xyz, ens.signal)
+        c.net_init(n // 4)
+        rng = np.random.default_rng(13)
+        p = StepParams.izhikevich(input_mode=1, nu=0.05)
+        p.synaptic_input, p.synapse_conductance = 1, 2.0
+        for t in range(3):
+            p.eta[t] = (0.02, 0.01, 0.0)[t]
+        # targets on both sides of the calcium range: some neurons grow elements, others shrink
+        c.neurons_init(p, rng.uniform(0.05, 0.15, n), ca=rng.uniform(0.05, 0.15, n),
+                       grown=rng.poisson(2.0, (3, n)).astype(np.float64).ravel())
+        c.neurons_set_network_growth(np.repeat(ens.nu * 200.0, 3))
+        ctxs.append(c)
+    eager, graph = ctxs
+    graph.graph_create(steps, 8 * n)
+    deleted = regrown = 0
+    for per in range(periods):
+        for k in range(steps):
+            eager.neurons_step(per * steps + k)
+        eager.neurons_commit_elements(per)
+        E = _update_result(eager, eager.update_connectivity(per))
+        G = _update_result(graph, graph.graph_launch(per * steps, per))
+        deleted += G["stats"]["deleted"]
+        regrown += G["stats"]["net_regrown"]
+        _say("period", per, "created", G["stats"]["created"], "deleted", G["stats"]["deleted"],
+             "net regrown", G["stats"]["net_regrown"])
+        _same(E, G)
+        se, sg = eager.neurons_get(), graph.neurons_get()
+        for key in ("x", "u", "ca", "fired", "delta", "grown", "connected"):
+            assert np.array_equal(se[key], sg[key]), (per, key)
+        assert _pairs(*eager.net_get_edges()) == _pairs(*graph.net_get_edges()), per
+    assert regrown > 0
+    _say("deleted over the run", deleted)
+    eager.close()
+    graph.close()
+
+
 def test_graph_needs_counter_input():
     net = uniform_cube(100, 100.0, 34)
     g = BarnesHutHIP(net.box_min, net.box_max)
