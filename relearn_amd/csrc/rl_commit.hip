@@ -8,9 +8,11 @@
 // serial walk accepts, in every group with v vacant dendrites, exactly the first v of its non-self
 // requests in processing order.  On the device:
 //   1. self requests fail; the others are counted per group (and get their priority);
-//   2. a counting sort (prefix sum of the group sizes, scatter) groups them;
-//   3. each request counts the members of its group that come before it in processing order,
-//      stopping at v, and is accepted when fewer than v do;
+//   2. a group with no more requests than vacancies accepts all of them (about half of the requests
+//      at 1M-16M neurons); the requests of the other groups are grouped by a counting sort (prefix
+//      sum of the group sizes, scatter);
+//   3. each grouped request counts the members of its group that come before it in processing
+//      order, stopping at v, and is accepted when fewer than v do;
 //   4. each group takes min(v, size) vacancies, each source one vacant axon per accepted request.
 // The processing order is the caller's permutation (parity with a given shuffle) or a counter-based
 // 64-bit priority per request, ties by request index.  Nothing here sorts by priority, allocates or
@@ -65,12 +67,18 @@ __global__ void k_commit_count(const uint64_t* __restrict__ n_req, const uint32_
     }
 }
 
-// Step 2, after the prefix sum goff of the group sizes.  Fills each group from its end; the order
-// inside a group is arbitrary (step 3 compares priorities, not positions) and gcnt returns to zero.
+// Step 2, after the prefix sum goff of the group sizes.  A request of a group that has no more
+// requests than vacancies is accepted here.  The others fill their group from its end; the order
+// inside a group is arbitrary (step 3 compares priorities, not positions).  Positions of accepted
+// groups keep the sentinel kUngrouped.
+constexpr uint32_t kUngrouped = 0xFFFFFFFFU;
+
 __global__ void k_commit_scatter(const uint64_t* __restrict__ n_req, const uint32_t* __restrict__ tgt,
                                  const uint32_t* __restrict__ src, const uint8_t* __restrict__ sig,
                                  const uint64_t* __restrict__ prio, const uint64_t* __restrict__ goff,
-                                 uint32_t* __restrict__ gcnt, uint64_t* __restrict__ sprio, uint32_t* __restrict__ sidx) {
+                                 const uint32_t* __restrict__ vexc, const uint32_t* __restrict__ vinh,
+                                 uint32_t* __restrict__ gcnt, uint64_t* __restrict__ sprio, uint32_t* __restrict__ sidx,
+                                 uint8_t* __restrict__ ok) {
     const uint64_t m = *n_req;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m; i += stride) {
@@ -78,7 +86,13 @@ __global__ void k_commit_scatter(const uint64_t* __restrict__ n_req, const uint3
         if (src[i] == t) {
             continue;
         }
-        const uint64_t g = 2 * static_cast<uint64_t>(t) + (sig[i] & 1U);
+        const uint32_t ty = sig[i] & 1U;
+        const uint64_t g = 2 * static_cast<uint64_t>(t) + ty;
+        const uint64_t size = goff[g + 1] - goff[g];
+        if (size <= static_cast<uint64_t>(ty ? vinh[t] : vexc[t])) {
+            ok[i] = 1;
+            continue;
+        }
         const uint64_t q = goff[g] + atomicSub(&gcnt[g], 1U) - 1U;
         sprio[q] = prio[i];
         sidx[q] = static_cast<uint32_t>(i);
@@ -96,6 +110,9 @@ __global__ void k_commit_rank(const uint64_t* __restrict__ n_grouped, const uint
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; q < mg; q += stride) {
         const uint32_t i = sidx[q];
+        if (i == kUngrouped) {
+            continue;   // a group accepted whole in step 2
+        }
         const uint64_t p = sprio[q];
         const uint32_t t = tgt[i];
         const uint32_t ty = sig[i] & 1U;
@@ -199,8 +216,10 @@ int enqueue_commit(rl_bh_ctx* c, const uint64_t* h_order) {
                                                     h_order == nullptr ? 1 : 0, c->c_prio, c->c_gcnt, c->req_ok);
     RL_HIP(c, hipGetLastError());
     RL_TRY(scan_counts(c, c->c_gcnt, groups, c->c_goff));
+    RL_TRY(fill_u32(c, c->c_sidx, kUngrouped, 0, c->c_goff + groups, c->slots_cap));
     k_commit_scatter<<<kStreamBlocks, kBlock, 0, s>>>(n_req, c->req_target, c->req_source, c->req_signal, c->c_prio,
-                                                      c->c_goff, c->c_gcnt, c->c_sprio, c->c_sidx);
+                                                      c->c_goff, c->vac[1], c->vac[2], c->c_gcnt, c->c_sprio, c->c_sidx,
+                                                      c->req_ok);
     RL_HIP(c, hipGetLastError());
     k_commit_rank<<<kStreamBlocks, kBlock, 0, s>>>(c->c_goff + groups, c->c_goff, c->c_sprio, c->c_sidx,
                                                    c->req_target, c->req_signal, c->vac[1], c->vac[2], c->req_ok);
