@@ -8,18 +8,25 @@
 // -march: the reference sets no -march either, so its FP64 arithmetic is the same IEEE
 // sequence of +,-,*,/ and the libstdc++ / glibc std::hypot, std::exp, std::sqrt.
 //
-// Parity pins (tests/test_oracle_*.py):
+// Parity pins (tests/test_oracle.py):
 //   * octree (a1-a4): closed-form invariants of the capacity-1 PR octree, brute-force recursive
 //     partition on small inputs, centroid recomputation in child order, the reference fixture
 //     input/positions.txt (51,842 real positions);
 //   * picker (a9): known answers of test/probability_picker/test_probability_picker.cpp:122-145 and
 //     lower-bound equivalence (:80-100);
-//   * Philox4x32-10: Random123 known-answer vectors; std::mt19937: 10000th output 4123659995;
+//   * Philox4x32-10: Random123 known-answer vectors, and the counter draws are uniform (KS, bit
+//     balance, lag-1 correlation); std::mt19937: 10000th output 4123659995;
 //   * Gaussian (a7), acceptance (a5): closed forms, monotonicity, brute-force list on tiny trees;
-//   * step (a15-a17): closed-form single-substep recurrences.
+//   * commit (a14): the serial loop over the same order, written out in the test;
+//   * step (a15-a17): closed-form single-substep recurrences, growth-curve zeros;
+//   * synaptic input: a dense matrix product of the weight matrix with the fired vector;
+//   * synapse deletion: conservation of units and element counts, all-or-nothing cases, a uniform
+//     distribution of the deleted subset, and independence from the order the pairs are stored in.
 // Parity unpinned: the reference's mt19937 seeding through boost::hash_combine (Boost 1.89.0),
-// boost::random::uniform_int_distribution inside ::shuffle and the Boost normal input stream;
-// these only feed the mt19937 CPU-baseline mode and never the parity checks.
+// boost::random::uniform_int_distribution inside ::shuffle, the Boost normal input stream and the
+// RandomHolder indices of RandomSynapseDeletionFinder; the first three only feed the mt19937
+// CPU-baseline mode, and the deletion draw here is a counter-based uniform subset of the same size,
+// so none of them is a parity check against the reference's own stream.
 
 #include "bh_oracle.h"
 
