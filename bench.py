@@ -408,6 +408,22 @@ def read_traffic(kernel, precision, neurons):
         return None
 
 
+def read_valu_busy(kernel, precision_label):
+    """VALU busy fraction of the search kernel from the committed rocprofv3 counter summary
+    (SQ_INSTS_VALU x 4 cycles over all SIMD cycles), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{precision_label}_16M_counters.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("- VALU busy"):
+                try:
+                    return float(line.rsplit(":", 1)[1])
+                except ValueError:
+                    return None
+    return None
+
+
 def cpu_baseline(net, sample, threads, seed):
     from oracle import pyoracle
 
@@ -539,6 +555,8 @@ def main(argv=None):
             / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9,
             "kernel_ms": main_r["dominant_kernel_ms"],
             "traffic_gbs": None if traffic is None else traffic / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9,
+            # what bounds the kernel: instruction issue (rocprofv3 counters of the same kernel at 16M)
+            "valu_busy": read_valu_busy(kname, main_name) if args.neurons == 16_000_000 else None,
         },
     }
     for name, r in res.items():
