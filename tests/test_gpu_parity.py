@@ -171,6 +171,35 @@ def test_search_parity_real_positions(oracle, flags):
     g.close()
 
 
+@pytest.mark.parametrize("precision", [RL_PREC_FP64, RL_PREC_FP32_MIXED])
+def test_search_parity_gaussian_with_mu(oracle, precision):
+    # GaussianDistributionKernel with mu != 0 (Gaussian.h:62-67): the attraction peaks at distance mu
+    _say("search with mu = 400 um, precision", precision)
+    net = uniform_cube(30_000, 10_000.0 * (0.03 ** (1 / 3)), 23)
+    t = oracle.Tree(net.xyz, net.box_min, net.box_max)
+    t.update(net.vac_exc, net.vac_inh)
+    ref = t.find_targets(net.signal, net.vac_axon, mu=400.0, sigma=750.0, seed=17, step=2, threads=0)
+    t.close()
+    g = BarnesHutHIP(net.box_min, net.box_max, mu=400.0, sigma=750.0, seed=17, precision=precision)
+    g.init(net.xyz, net.signal)
+    g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+    g.find_target_neurons(2)
+    tgt, tests, cnt = g.get_search_slots(int(net.vac_axon.sum()))
+    if precision == RL_PREC_FP64:
+        bad, near = _check_search(ref, tgt, tests, cnt, net)
+        _say("near-boundary searches", near, "mismatches", len(bad))
+        assert not bad
+    else:
+        valid = (np.asarray(ref["target"]) != 0xFFFFFFFF) & (tgt != 0xFFFFFFFF)
+        div = np.mean(np.asarray(ref["target"])[valid] != tgt[valid])
+        _say("pick divergence fp32-mixed vs oracle", div)
+        assert div <= 1e-3
+        # the acceptance decisions are the FP64 ones: where the picks agree, so do the test counts
+        same = valid & (np.asarray(ref["target"]) == tgt)
+        assert np.array_equal(np.asarray(ref["node_tests"])[same], tests[same])
+    g.close()
+
+
 @pytest.mark.parametrize("flags", [0, RL_FLAG_EXACT_ORDER])
 def test_search_fp32_mixed_divergence(oracle, flags):
     _say("search fp32 mixed vs fp64", flags)
