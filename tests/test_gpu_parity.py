@@ -171,6 +171,22 @@ def test_search_parity_real_positions(oracle, flags):
     g.close()
 
 
+@pytest.mark.parametrize("flags", [0, RL_FLAG_EXACT_ORDER])
+@pytest.mark.parametrize("theta", [0.1, 0.5])
+def test_search_parity_other_theta(oracle, theta, flags):
+    # the ends of the acceptance parameter's range (BarnesHutBase.h:74-77).  At 0.1 the level-1
+    # lists are long: sources whose expanded nodes overflow a wave's slot table go to the per-lane
+    # kernel, so that hand-over is checked too.
+    _say("search fp64 theta", theta, "flags", flags)
+    net = uniform_cube(20_000, 10_000.0 * (0.02 ** (1 / 3)), 29)
+    ref, g, nreq, tgt, tests, cnt = _search_pair(oracle, net, seed=31, step=4, theta=theta, flags=flags)
+    bad, near = _check_search(ref, tgt, tests, cnt, net)
+    _say("near-boundary searches", near, "mismatches", len(bad), bad[:5])
+    assert not bad
+    assert nreq == int(cnt.sum())
+    g.close()
+
+
 @pytest.mark.parametrize("precision", [RL_PREC_FP64, RL_PREC_FP32_MIXED])
 def test_search_parity_gaussian_with_mu(oracle, precision):
     # GaussianDistributionKernel with mu != 0 (Gaussian.h:62-67): the attraction peaks at distance mu
