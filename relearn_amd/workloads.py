@@ -82,10 +82,10 @@ def parameter_sweep(num_networks: int = 256, n_min: int = 2_000, n_max: int = 20
                     np.concatenate([nt.vac_inh for nt in nets]), nu)
 
 
-# The configurations of the connectivity-update workloads: (neurons, cube side in micrometres)
+# The uniform-cube workloads: (neurons, cube side in micrometres)
 CONFIGS = {
-    "cfg0_1k": (1_000, 1_000.0),
-    "cfg1_1m": (1_000_000, 10_000.0),
-    "cfg2_16m": (16_000_000, 25_000.0),
-    "cfg4_4m": (4_000_000, 16_000.0),
+    "cube_1k": (1_000, 1_000.0),
+    "cube_1m": (1_000_000, 10_000.0),
+    "cube_16m": (16_000_000, 25_000.0),
+    "cube_4m": (4_000_000, 16_000.0),
 }

@@ -383,10 +383,10 @@ def _check_search_vec(ref, tgt, tests, cnt, net):
     return int(cnt_bad.sum()), int(t_bad.sum()), int(e_bad.sum()), int(near_n.sum())
 
 
-def test_cfg1_million_neurons_fp64(oracle):
+def test_million_neurons_fp64(oracle):
     """1M neurons uniform in a 10,000 um cube, Poisson(2)/(1.5)/(1.5) vacancies, theta 0.3, sigma 750:
     bit-exact octree counts and centroids, identical picks except draws within 1e-12 of a boundary."""
-    _say("cfg1: 1M neurons fp64 tree + search parity")
+    _say("1M neurons fp64 tree + search parity")
     net = uniform_cube(1_000_000, 10_000.0, 0)
     t = oracle.Tree(net.xyz, net.box_min, net.box_max)
     t.update(net.vac_exc, net.vac_inh)
@@ -395,13 +395,13 @@ def test_cfg1_million_neurons_fp64(oracle):
     g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
     a, b = t.export(), g.export_tree()
     _compare_trees(a, b)
-    _say("cfg1 tree identical", len(a["level"]), "nodes")
+    _say("1M-neuron tree identical", len(a["level"]), "nodes")
     ref = t.find_targets(net.signal, net.vac_axon, theta=0.3, sigma=750.0, seed=1, step=0, threads=0)
     nreq = g.find_target_neurons(0)
     slots = int(net.vac_axon.astype(np.uint64).sum())
     tgt, tests, cnt = g.get_search_slots(slots)
     cb, tb, eb, near = _check_search_vec(ref, tgt, tests, cnt, net)
-    _say("cfg1 near-boundary neurons", near, "count/target/test mismatches", cb, tb, eb)
+    _say("1M near-boundary neurons", near, "count/target/test mismatches", cb, tb, eb)
     assert cb == 0 and tb == 0 and eb == 0
     assert near <= 1e-3 * len(cnt)
     assert nreq == int(cnt.sum())

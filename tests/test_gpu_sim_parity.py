@@ -146,7 +146,7 @@ def test_small_simulation_matches_oracle(oracle):
     assert created > 0 and deleted > 0
 
 
-def test_cfg0_full_simulation_matches_oracle(oracle):
+def test_reference_cpu_configuration_simulation_matches_oracle(oracle):
     # the CPU reference configuration: 1,000 neurons uniform in a 1000 um cube (seed 0, 80/20 E/I),
     # Izhikevich defaults, Normal(5, 2) input, calcium tau 1e4 and beta 1e-3, growth rate 1e-4 and the
     # reference's thresholds (eta 0.4 / 0.1 / 0.0, target calcium 0.7), elements and calcium from 0,
@@ -154,7 +154,7 @@ def test_cfg0_full_simulation_matches_oracle(oracle):
     # stay identical for the whole run (calcium stays below the axons' eta, so no axon grows and no
     # synapse forms: the reference's own outcome for this configuration)
     n, periods, steps, seed = 1000, 200, 100, 0
-    _say("cfg0 simulation parity", n, "neurons,", periods, "periods of", steps, "steps")
+    _say("1000-neuron, 20,000-step simulation parity", n, "neurons,", periods, "periods of", steps, "steps")
     net = uniform_cube(n, 1000.0, 0)
     _simulate(oracle, net.xyz, net.signal, net.box_min, net.box_max, None, periods, steps, seed,
               np.random.default_rng(19), cond=0.03, eta=(0.4, 0.1, 0.0), target=0.7, nu=1e-4, ca0=None,
