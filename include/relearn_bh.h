@@ -76,6 +76,11 @@ typedef struct {
     uint64_t net_capacity;      /* network graph capacity after the replay (rl_bh_graph_launch) */
     uint64_t net_regrown;       /* 1: the replay's synapses did not fit the network graph, which grew and
                                    took them; the period was captured again (rl_bh_graph_launch) */
+    uint64_t lane_sources;      /* sources whose level-1 list went to the per-lane search: a cooperative
+                                   slot table that overflowed, or a Gaussian sum of 0 (the FP64 re-evaluation
+                                   of an FP32 sum of 0, or the n/r fallback of Kernel.h:109-126) */
+    uint64_t slow_descents;     /* descents below level 1 handed to the per-lane traversal (expansion or a
+                                   Gaussian sum of 0 below the picked node) */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a
@@ -104,6 +109,25 @@ int rl_bh_init(rl_bh_ctx* ctx, uint64_t n, const double* xyz, const uint8_t* sig
  * num_networks = 1 with the box of the configuration. */
 int rl_bh_init_networks(rl_bh_ctx* ctx, uint32_t num_networks, const uint64_t* offsets, const double* boxes,
                         const double* xyz, const uint8_t* signal);
+
+/* Algorithm::create_neurons(creation_count) (Algorithm.h:117) = OctreeAlgorithm::create_neurons
+ * (source/algorithm/Internal/OctreeAlgorithm.h:73-85), run on a creation event (Neurons::create_neurons,
+ * source/neurons/Neurons.cpp:324-343, from Simulation.cpp:425): appends `count` neurons with ids
+ * n .. n + count - 1 at positions xyz (count x 3 AoS, inside the box) with signal types `signal`,
+ * inserts them into the octree and re-initialises the leaves.  The octree of distinct positions does not
+ * depend on the insertion order, so the tree equals the one rl_bh_init builds over all neurons.  The
+ * new neurons are enabled, with no vacant or connected elements.  When the context holds neuron state
+ * (rl_neurons_init), the new neurons start with x, u, ca, target_ca (count each; NULL x, u, ca give 0,
+ * target_ca is then required) and grown (3 x count, type-major; NULL gives 0), as the reference's
+ * calculators initialise them (CalciumCalculator::create_neurons, ElementBaseBase::create_neurons), and
+ * the state of the existing neurons (deltas and fired status included) carries over; per-neuron element
+ * parameters (rl_neurons_set_element_params) of the new neurons take the per-type values of
+ * rl_step_params.  A network graph (rl_net_init) keeps its edges and capacity; the new neurons have
+ * none (NetworkGraph::create_neurons).  A captured period is released.  One-network contexts only
+ * (RL_ESTATE otherwise).  Positions and signal types are checked before anything changes; a failure
+ * after that (RL_EDUPPOS, RL_EOOM) leaves the context to be initialised again. */
+int rl_bh_create_neurons(rl_bh_ctx* ctx, uint64_t count, const double* xyz, const uint8_t* signal, const double* x,
+                         const double* u, const double* ca, const double* target_ca, const double* grown);
 
 /* Draw-stream seed of network g of an ensemble (network 0 keeps the configured seed). */
 uint64_t rl_bh_network_seed(uint64_t seed, uint32_t network);
@@ -149,7 +173,10 @@ int rl_bh_get_search_slots(rl_bh_ctx* ctx, uint64_t capacity, uint32_t* target, 
 /* ForwardConnector::process_requests + process_responses (source/algorithm/Connector.cpp:33-197):
  * resolves conflicts in the order given by `order` (a permutation of the request indices, the
  * shuffled order of Connector.cpp:71) or, when order is NULL, by a counter-based priority
- * (seed, source, step, j); commits the vacancies on the device.  *created gets the number of synapses. */
+ * (seed, source, step, j); commits the vacancies on the device.  *created gets the number of synapses.
+ * The requests of one search are committed once: a second call before the next search (or after a
+ * replay with stats, which commits its own requests) returns RL_ESTATE and changes nothing; the
+ * requests and accept flags stay readable. */
 int rl_bh_commit(rl_bh_ctx* ctx, uint32_t step, const uint64_t* order, uint64_t* created);
 
 /* Accept flags of the last commit (1 = SynapseCreationResponse::Succeeded), request order. */
@@ -186,8 +213,8 @@ int rl_bh_last_search_ms(rl_bh_ctx* ctx, double* ms);
 int rl_bh_candidate_lists(rl_bh_ctx* ctx, uint64_t num_sources, const uint32_t* sources, uint64_t capacity,
                           uint32_t* nodes, double* probabilities, uint32_t* counts);
 
-/* Counters of the last search: searches, requests, node_tests, node_tests_ref, ms_search and
- * ms_search_kernel are filled (created and the other times are 0). */
+/* Counters of the last search: searches, requests, node_tests, node_tests_ref, ms_search,
+ * ms_search_kernel, lane_sources and slow_descents are filled (created and the other times are 0). */
 int rl_bh_last_search_stats(rl_bh_ctx* ctx, rl_bh_step_stats* stats);
 
 /* ------------------------------------------------------------------ neuron step */
@@ -205,6 +232,8 @@ typedef struct {
     double synapse_conductance;  /* conductance per unit synapse weight (SynapticEquallyWeightedActivityInput) */
     uint32_t synaptic_input; /* 1: add the synaptic input of the network graph (rl_net_init) */
     uint32_t reserved;
+    double retract_ratio[3]; /* vacant retract ratio per element type (--retract-ratio, relearn.cpp:617-618;
+                                ElementBaseBase::update, ElementBaseBase.h:420-490); 0 in the reference's default */
 } rl_step_params;
 
 /* Allocates and uploads the per-neuron state of the step kernel (IzhikevichModel x/u, calcium,
@@ -237,6 +266,16 @@ int rl_neurons_commit_elements_at(rl_bh_ctx* ctx, uint32_t update, uint32_t* del
  * excitatory, inhibitory dendrites), replacing rl_step_params.nu for those networks; NULL restores
  * rl_step_params.nu everywhere (SynapticElements growth rate, SynapticElementsBase.h:617-625). */
 int rl_neurons_set_network_growth(rl_bh_ctx* ctx, const double* nu);
+
+/* Per-neuron synaptic element parameters, as the reference's --synaptic-elements-file sets them
+ * (SynapticElementsIO::load_function_from_file, source/io/SynapticElementsIO.h:60; applied in
+ * relearn.cpp:1158-1170 through set_minimum_calcium_calculator and set_vacant_retract_ratio_calculator):
+ * min_calcium[t * n + i] replaces rl_step_params.eta[t] for neuron i in the growth curve
+ * (SynapticElementsCPU.cpp:25-49), retract_ratio[t * n + i] replaces rl_step_params.retract_ratio[t] in
+ * the element commit (ElementBaseBase.h:420-490); t = axon, excitatory, inhibitory dendrite.  Either
+ * pointer may be NULL, which restores the per-type values of rl_step_params for that parameter.  Values
+ * must be finite; retract ratios lie in [0, 1]. */
+int rl_neurons_set_element_params(rl_bh_ctx* ctx, const double* min_calcium, const double* retract_ratio);
 
 /* Copies the neuron state: any pointer may be NULL; delta/grown are 3*n (type-major). */
 int rl_neurons_get(rl_bh_ctx* ctx, double* x, double* u, double* ca, uint8_t* fired, double* delta,

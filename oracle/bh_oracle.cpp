@@ -1261,7 +1261,8 @@ static double gaussian_growth_curve(double current, double eta, double epsilon) 
 }
 
 int orc_neuron_step(uint64_t n, const orc_step_params* p, double* x, double* u, double* ca, const double* input,
-                    uint8_t* fired, const double* target_ca, double* delta, const uint8_t* enabled) {
+                    uint8_t* fired, const double* target_ca, double* delta, const uint8_t* enabled,
+                    const double* min_calcium) {
     if (p == nullptr || (n > 0 && (x == nullptr || u == nullptr || ca == nullptr || input == nullptr || fired == nullptr ||
                                    target_ca == nullptr || delta == nullptr)) ||
         p->h == 0) {
@@ -1307,13 +1308,16 @@ int orc_neuron_step(uint64_t n, const orc_step_params* p, double* x, double* u, 
         ca[i] = c;
     }
     // update_number_elements_openmp (source/neurons/synaptic_elements/SynapticElementsCPU.cpp:25-49) for the
-    // axon, excitatory and inhibitory dendrite types (:73-75), added with add_to_delta.
+    // axon, excitatory and inhibitory dendrite types (:73-75), added with add_to_delta.  The minimum
+    // calcium is per neuron (get_minimum_calcium, :62; the --synaptic-elements-file values) or one value
+    // per type (the constant calculators of relearn.cpp:1199-1201).
     for (int ty = 0; ty < 3; ty++) {
         for (uint64_t i = 0; i < n; i++) {
             double inc = 0.0;
             if (enabled == nullptr || enabled[i] != 0) {
-                const double clamped_target = std::max(target_ca[i], p->eta[ty]);
-                const double dz = gaussian_growth_curve(ca[i], p->eta[ty], clamped_target);
+                const double eta = min_calcium != nullptr ? min_calcium[static_cast<uint64_t>(ty) * n + i] : p->eta[ty];
+                const double clamped_target = std::max(target_ca[i], eta);
+                const double dz = gaussian_growth_curve(ca[i], eta, clamped_target);
                 inc = dz * p->nu[ty];
             }
             delta[static_cast<uint64_t>(ty) * n + i] += inc;

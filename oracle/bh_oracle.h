@@ -156,11 +156,12 @@ typedef struct {
 /*
  * a15-a17: one simulation step for neurons 0..n-1: Izhikevich update with the given input,
  * calcium update, growth-curve deltas added to delta[3*n] (type-major: axon, exc, inh).
- * enabled may be NULL.  target_ca per neuron.
+ * enabled may be NULL.  target_ca per neuron.  min_calcium: 3*n per-neuron minimum calcium
+ * (type-major), or NULL for p->eta.
  */
 int orc_neuron_step(uint64_t n, const orc_step_params* p, double* x, double* u, double* ca,
                     const double* input, uint8_t* fired, const double* target_ca,
-                    double* delta, const uint8_t* enabled);
+                    double* delta, const uint8_t* enabled, const double* min_calcium);
 
 /* a18: ElementBaseBase::update for one element type over n neurons; returns deletions per neuron. */
 /* Synaptic input of the equally weighted model plus the background input: for every neuron the

@@ -130,7 +130,7 @@ def lib():
         L.orc_reference_shuffle.argtypes = [C.c_uint64, C.c_uint32, _u64p]
         L.orc_reference_shuffle.restype = C.c_int
         L.orc_neuron_step.argtypes = [C.c_uint64, C.POINTER(StepParams), _f64p, _f64p, _f64p, _f64p, _u8p, _f64p,
-                                      _f64p, _u8p]
+                                      _f64p, _u8p, _f64p]
         L.orc_neuron_step.restype = C.c_int
         L.orc_graph_create.argtypes = [C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32), C.POINTER(C.c_void_p)]
         L.orc_graph_create.restype = C.c_int
@@ -346,15 +346,17 @@ def reference_shuffle(m, mt_seed):
     return perm[:m]
 
 
-def neuron_step(params: StepParams, x, u, ca, inp, fired, target_ca, delta, enabled=None):
+def neuron_step(params: StepParams, x, u, ca, inp, fired, target_ca, delta, enabled=None, min_calcium=None):
     n = len(x)
     en = None if enabled is None else np.ascontiguousarray(enabled, dtype=np.uint8)
+    mc = None if min_calcium is None else np.ascontiguousarray(min_calcium, dtype=np.float64).ravel()
+    assert mc is None or len(mc) == 3 * n
     for a in (x, u, ca, inp, target_ca, delta):
         assert a.dtype == np.float64 and a.flags.c_contiguous
     assert fired.dtype == np.uint8
     _check(lib().orc_neuron_step(n, C.byref(params), _ptr(x, _f64p), _ptr(u, _f64p), _ptr(ca, _f64p),
                                  _ptr(inp, _f64p), _ptr(fired, _u8p), _ptr(target_ca, _f64p), _ptr(delta, _f64p),
-                                 _ptr(en, _u8p)), "orc_neuron_step")
+                                 _ptr(en, _u8p), _ptr(mc, _f64p)), "orc_neuron_step")
 
 
 class Graph:

@@ -33,7 +33,8 @@ EXPORTED = [
     "rl_bh_last_search_ms", "rl_bh_last_search_stats", "rl_neurons_init", "rl_neurons_step", "rl_neurons_commit_elements",
     "rl_neurons_get", "rl_neurons_set_network_growth", "rl_bh_graph_create", "rl_bh_graph_launch",
     "rl_bh_graph_destroy", "rl_bh_set_tuning", "rl_bh_get_tuning", "rl_bh_autotune", "rl_net_init",
-    "rl_net_get_edges", "rl_bh_candidate_lists", "rl_neurons_commit_elements_at",
+    "rl_net_get_edges", "rl_bh_candidate_lists", "rl_neurons_commit_elements_at", "rl_neurons_set_element_params",
+    "rl_bh_create_neurons",
 ]
 
 
@@ -67,6 +68,8 @@ class StepStats(C.Structure):
         ("deleted", C.c_uint64),
         ("net_capacity", C.c_uint64),
         ("net_regrown", C.c_uint64),
+        ("lane_sources", C.c_uint64),
+        ("slow_descents", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -102,6 +105,7 @@ class StepParams(C.Structure):
         ("synapse_conductance", C.c_double),
         ("synaptic_input", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("retract_ratio", C.c_double * 3),
     ]
 
     @classmethod
@@ -145,7 +149,9 @@ def load():
     L.rl_bh_init.argtypes = [vp, C.c_uint64, _f64p, _u8p]
     L.rl_bh_init_networks.argtypes = [vp, C.c_uint32, _u64p, _f64p, _f64p, _u8p]
     L.rl_bh_network_seed.argtypes = [C.c_uint64, C.c_uint32]
+    L.rl_bh_create_neurons.argtypes = [vp, C.c_uint64, _f64p, _u8p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.rl_neurons_set_network_growth.argtypes = [vp, _f64p]
+    L.rl_neurons_set_element_params.argtypes = [vp, _f64p, _f64p]
     L.rl_bh_set_enabled.argtypes = [vp, _u8p]
     L.rl_bh_prepare_update_connectivity.argtypes = [vp, _u32p, _u32p, _u32p]
     L.rl_bh_prepare_resident.argtypes = [vp]
@@ -258,6 +264,18 @@ class BarnesHutHIP:
         self.n = xyz.shape[0]
         self._check(self.L.rl_bh_init_networks(self.h, len(off) - 1, _ptr(off, _u64p), _ptr(bx, _f64p),
                                                _ptr(xyz, _f64p), _ptr(sig, _u8p)))
+
+    # Algorithm::create_neurons
+    def create_neurons(self, xyz, signal, x=None, u=None, ca=None, target_ca=None, grown=None):
+        """Appends neurons (ids n .. n + count - 1) at positions xyz with the given signal types; with
+        neuron state, their initial x, u, calcium, target calcium and grown elements ((3, count))."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        sig = np.ascontiguousarray(signal, dtype=np.uint8)
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
+                for a in (x, u, ca, target_ca, grown)]
+        self._check(self.L.rl_bh_create_neurons(self.h, xyz.shape[0], _ptr(xyz, _f64p), _ptr(sig, _u8p),
+                                                *[_ptr(a, _f64p) for a in arrs]))
+        self.n += xyz.shape[0]
 
     def set_enabled(self, enabled):
         en = None if enabled is None else np.ascontiguousarray(enabled, dtype=np.uint8)
@@ -438,6 +456,13 @@ class BarnesHutHIP:
     def neurons_set_network_growth(self, nu):
         a = None if nu is None else np.ascontiguousarray(nu, dtype=np.float64).ravel()
         self._check(self.L.rl_neurons_set_network_growth(self.h, _ptr(a, _f64p)))
+
+    def neurons_set_element_params(self, min_calcium=None, retract_ratio=None):
+        """Per-neuron minimum calcium and vacant retract ratio, (3, n) type-major each, or None for the
+        per-type values of the step parameters."""
+        mc = None if min_calcium is None else np.ascontiguousarray(min_calcium, dtype=np.float64).ravel()
+        rr = None if retract_ratio is None else np.ascontiguousarray(retract_ratio, dtype=np.float64).ravel()
+        self._check(self.L.rl_neurons_set_element_params(self.h, _ptr(mc, _f64p), _ptr(rr, _f64p)))
 
     def neurons_step(self, step, inp=None):
         i = None if inp is None else np.ascontiguousarray(inp, dtype=np.float64)

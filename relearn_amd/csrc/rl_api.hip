@@ -374,6 +374,171 @@ int rl_bh_init_networks(rl_bh_ctx* c, uint32_t num_networks, const uint64_t* off
     return init_networks(c, num_networks, offsets, boxes, xyz, signal);
 }
 
+int rl_bh_create_neurons(rl_bh_ctx* c, uint64_t count, const double* xyz, const uint8_t* signal, const double* x,
+                         const double* u, const double* ca, const double* target_ca, const double* grown) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    if (c->n == 0 || c->num_networks != 1) {
+        return fail(c, RL_ESTATE, "rl_bh_create_neurons: needs a context initialised by rl_bh_init");
+    }
+    if (count == 0 || xyz == nullptr || signal == nullptr) {
+        return fail(c, RL_EINVAL, "rl_bh_create_neurons: count must be > 0 with positions and signal types");
+    }
+    const bool state = c->neurons.ready;
+    if (state && target_ca == nullptr) {
+        return fail(c, RL_EINVAL, "rl_bh_create_neurons: the neuron state needs the new neurons' target calcium");
+    }
+    RL_TRY(set_device(c));
+    const uint64_t n0 = c->n;
+    const uint64_t n1 = n0 + count;
+    if (n1 > kLinkMask) {
+        return fail(c, RL_ELIMIT, "rl_bh_create_neurons: more than 2^28 neurons");
+    }
+    hipStream_t s = c->stream;
+    // Everything the context keeps per neuron is read back, extended and handed to the same
+    // initialisation a fresh context runs: creation events are rare, and the octree of distinct
+    // positions does not depend on the order of insertion (OctreeNode::insert), so rebuilding it over
+    // all n0 + count neurons gives the tree that inserting the new ones would.
+    std::vector<double> pos(3 * n0);
+    std::vector<uint8_t> sig(n1), en(n1, 1);
+    std::vector<uint32_t> vac(3 * n1, 0), conn(3 * n1, 0);
+    RL_HIP(c, hipMemcpyAsync(pos.data(), c->pos, 3 * n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipMemcpyAsync(sig.data(), c->signal, n0, hipMemcpyDeviceToHost, s));
+    RL_HIP(c, hipMemcpyAsync(en.data(), c->enabled, n0, hipMemcpyDeviceToHost, s));
+    for (int t = 0; t < 3; t++) {
+        RL_HIP(c, hipMemcpyAsync(vac.data() + t * n1, c->vac[t], n0 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipMemcpyAsync(conn.data() + t * n1, c->conn[t], n0 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    // neuron state: x, u, ca, target, fired status of the last step, grown and deltas (3 per neuron)
+    const Neurons& ns = c->neurons;
+    std::vector<double> sx, su, sca, stgt, sgrown, sdelta, smin, sret;
+    std::vector<uint8_t> sfired;
+    const rl_step_params params = ns.p;
+    double* net_nu = nullptr;
+    if (state) {
+        sx.assign(n1, 0.0);
+        su.assign(n1, 0.0);
+        sca.assign(n1, 0.0);
+        stgt.assign(n1, 0.0);
+        sfired.assign(n1, 0);
+        sgrown.assign(3 * n1, 0.0);
+        sdelta.assign(3 * n1, 0.0);
+        RL_HIP(c, hipMemcpyAsync(sx.data(), ns.x, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipMemcpyAsync(su.data(), ns.u, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipMemcpyAsync(sca.data(), ns.ca, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipMemcpyAsync(stgt.data(), ns.target_ca, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipMemcpyAsync(sfired.data(), ns.fired, n0, hipMemcpyDeviceToHost, s));
+        for (int t = 0; t < 3; t++) {
+            RL_HIP(c, hipMemcpyAsync(sgrown.data() + t * n1, ns.grown + t * n0, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+            RL_HIP(c, hipMemcpyAsync(sdelta.data() + t * n1, ns.delta + t * n0, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        if (ns.min_ca != nullptr || ns.retract != nullptr) {
+            // new neurons take the per-type values of the step parameters
+            smin.assign(3 * n1, 0.0);
+            sret.assign(3 * n1, 0.0);
+            for (int t = 0; t < 3; t++) {
+                std::fill(smin.begin() + t * n1 + n0, smin.begin() + (t + 1) * n1, params.eta[t]);
+                std::fill(sret.begin() + t * n1 + n0, sret.begin() + (t + 1) * n1, params.retract_ratio[t]);
+                if (ns.min_ca != nullptr) {
+                    RL_HIP(c, hipMemcpyAsync(smin.data() + t * n1, ns.min_ca + t * n0, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+                } else {
+                    std::fill(smin.begin() + t * n1, smin.begin() + t * n1 + n0, params.eta[t]);
+                }
+                if (ns.retract != nullptr) {
+                    RL_HIP(c, hipMemcpyAsync(sret.data() + t * n1, ns.retract + t * n0, n0 * sizeof(double), hipMemcpyDeviceToHost, s));
+                } else {
+                    std::fill(sret.begin() + t * n1, sret.begin() + t * n1 + n0, params.retract_ratio[t]);
+                }
+            }
+        }
+        net_nu = ns.net_nu;
+    }
+    double nu_net[3] = { 0.0, 0.0, 0.0 };
+    if (net_nu != nullptr) {
+        RL_HIP(c, hipMemcpyAsync(nu_net, net_nu, 3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    // network graph: the new neurons have no edges (NetworkGraph::create_neurons)
+    const bool graph = c->net.ready;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> esrc;
+    std::vector<int32_t> ew;
+    uint64_t cap = 0;
+    if (graph) {
+        uint64_t edges = 0;
+        RL_TRY(net_get(c, 0, nullptr, nullptr, nullptr, &edges));
+        off.assign(n1 + 1, 0);
+        esrc.assign(std::max<uint64_t>(edges, 1), 0);
+        ew.assign(std::max<uint64_t>(edges, 1), 0);
+        RL_TRY(net_get(c, edges, off.data(), esrc.data(), ew.data(), &edges));
+        std::fill(off.begin() + n0 + 1, off.end(), edges);
+        cap = c->net.cap;
+    }
+    RL_HIP(c, hipStreamSynchronize(s));
+    std::vector<double> all(3 * n1);
+    for (uint64_t i = 0; i < n0; i++) {
+        for (int d = 0; d < 3; d++) {
+            all[3 * i + d] = pos[d * n0 + i];
+        }
+    }
+    std::memcpy(all.data() + 3 * n0, xyz, 3 * count * sizeof(double));
+    std::memcpy(sig.data() + n0, signal, count);
+    if (state) {
+        for (uint64_t k = 0; k < count; k++) {
+            sx[n0 + k] = x != nullptr ? x[k] : 0.0;
+            su[n0 + k] = u != nullptr ? u[k] : 0.0;
+            sca[n0 + k] = ca != nullptr ? ca[k] : 0.0;
+            stgt[n0 + k] = target_ca[k];
+            for (int t = 0; t < 3; t++) {
+                sgrown[t * n1 + n0 + k] = grown != nullptr ? grown[t * count + k] : 0.0;
+            }
+        }
+    }
+    const uint64_t offs[2] = { 0, n1 };
+    const double box[6] = { c->cfg.box_min[0], c->cfg.box_min[1], c->cfg.box_min[2],
+                            c->cfg.box_max[0], c->cfg.box_max[1], c->cfg.box_max[2] };
+    // validation of the new positions (box, signal types) happens here, before anything is released
+    for (uint64_t k = 0; k < count; k++) {
+        if (signal[k] > 1) {
+            return fail(c, RL_EINVAL, "rl_bh_create_neurons: signal type must be 0 or 1");
+        }
+        for (int d = 0; d < 3; d++) {
+            const double v = xyz[3 * k + d];
+            if (!(box[d] <= v && v <= box[3 + d])) {
+                return fail(c, RL_EOUTSIDE, "rl_bh_create_neurons: position outside the simulation box");
+            }
+        }
+    }
+    RL_TRY(init_networks(c, 1, offs, box, all.data(), sig.data()));
+    RL_HIP(c, hipMemcpyAsync(c->enabled, en.data(), n1, hipMemcpyHostToDevice, s));
+    for (int t = 0; t < 3; t++) {
+        RL_HIP(c, hipMemcpyAsync(c->vac[t], vac.data() + t * n1, n1 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    }
+    RL_HIP(c, hipStreamSynchronize(s));
+    if (state) {
+        RL_TRY(neurons_init(c, &params, sx.data(), su.data(), sca.data(), stgt.data(), sgrown.data(), conn.data()));
+        Neurons& nn = c->neurons;
+        RL_HIP(c, hipMemcpyAsync(nn.delta, sdelta.data(), 3 * n1 * sizeof(double), hipMemcpyHostToDevice, s));
+        RL_HIP(c, hipMemcpyAsync(nn.fired_buf[0], sfired.data(), n1, hipMemcpyHostToDevice, s));
+        RL_HIP(c, hipStreamSynchronize(s));
+        if (!smin.empty()) {
+            RL_TRY(neurons_set_element_params(c, smin.data(), sret.data()));
+        }
+        if (net_nu != nullptr) {
+            RL_TRY(neurons_set_network_growth(c, nu_net));
+        }
+    } else {
+        for (int t = 0; t < 3; t++) {
+            RL_HIP(c, hipMemcpyAsync(c->conn[t], conn.data() + t * n1, n1 * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        }
+        RL_HIP(c, hipStreamSynchronize(s));
+    }
+    if (graph) {
+        RL_TRY(net_init(c, cap, off.data(), esrc.data(), ew.data()));
+    }
+    return RL_OK;
+}
+
 int rl_bh_set_enabled(rl_bh_ctx* c, const uint8_t* enabled) {
     if (c == nullptr) {
         return RL_EINVAL;
@@ -583,6 +748,8 @@ int rl_bh_update_connectivity(rl_bh_ctx* c, uint32_t step, rl_bh_step_stats* sta
             stats->ms_commit = t23;
             stats->ms_search_kernel = c->last_kernel_ms;
             stats->ms_total = t01 + t12 + t23;
+            stats->lane_sources = c->last_lane_sources;
+            stats->slow_descents = c->last_slow_descents;
         }
     } while (false);
     for (auto& ev : e) {
@@ -648,6 +815,8 @@ int rl_bh_last_search_stats(rl_bh_ctx* c, rl_bh_step_stats* st) {
     st->node_tests_ref = c->last_tests_ref;
     st->ms_search = c->last_search_ms;
     st->ms_search_kernel = c->last_kernel_ms;
+    st->lane_sources = c->last_lane_sources;
+    st->slow_descents = c->last_slow_descents;
     return RL_OK;
 }
 
@@ -655,6 +824,11 @@ int rl_neurons_init(rl_bh_ctx* c, const rl_step_params* p, const double* x, cons
                     const double* target_ca, const double* grown, const uint32_t* connected) {
     if (c == nullptr || p == nullptr || (c->n > 0 && target_ca == nullptr) || p->h == 0 || p->input_mode > 1) {
         return c == nullptr ? RL_EINVAL : fail(c, RL_EINVAL, "rl_neurons_init: bad arguments");
+    }
+    for (int t = 0; t < 3; t++) {
+        if (!(p->retract_ratio[t] >= 0.0 && p->retract_ratio[t] <= 1.0)) {
+            return fail(c, RL_EINVAL, "rl_neurons_init: retract ratios must lie in [0, 1]");
+        }
     }
     RL_TRY(set_device(c));
     return neurons_init(c, p, x, u, ca, target_ca, grown, connected);
@@ -666,6 +840,14 @@ int rl_neurons_set_network_growth(rl_bh_ctx* c, const double* nu) {
     }
     RL_TRY(set_device(c));
     return neurons_set_network_growth(c, nu);
+}
+
+int rl_neurons_set_element_params(rl_bh_ctx* c, const double* min_calcium, const double* retract_ratio) {
+    if (c == nullptr) {
+        return RL_EINVAL;
+    }
+    RL_TRY(set_device(c));
+    return neurons_set_element_params(c, min_calcium, retract_ratio);
 }
 
 int rl_neurons_step(rl_bh_ctx* c, uint32_t step, const double* input) {
@@ -894,7 +1076,11 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     c->last_tests = h[kCtrTestsExec];
     c->last_tests_ref = h[kCtrTestsRef];
     c->last_slow_descents = h[kCtrSlow];
+    c->last_lane_sources = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0 ? h[kCtrRedo] : h[kCtrOverflow];
+    stats->lane_sources = c->last_lane_sources;
+    stats->slow_descents = c->last_slow_descents;
     c->searched_valid = true;
+    c->committed = h[kCtrOverCap] == 0;   // the replay committed its requests (none when over capacity)
     if (h[kCtrOverCap] != 0) {
         return fail(c, RL_ELIMIT, "rl_bh_graph_launch: more vacant axons than max_vacant_axons; the update changed nothing");
     }
@@ -909,6 +1095,7 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
         c->slots = h[kCtrSlots];
         c->nreq = h[kCtrRequests];
         c->searched_valid = true;
+        c->committed = true;
     }
     stats->net_capacity = c->net.ready ? c->net.cap : 0;
     if (h[kCtrSticky] > (h[kCtrNetOver] != 0 ? 1ULL : 0ULL)) {

@@ -236,6 +236,9 @@ int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* creat
     if (!c->searched_valid) {
         return fail(c, RL_ESTATE, "commit called before find_target_neurons");
     }
+    if (c->committed) {
+        return fail(c, RL_ESTATE, "commit: the requests of the last search were already committed");
+    }
     if (created != nullptr) {
         *created = 0;
     }
@@ -244,6 +247,7 @@ int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* creat
     }
     RL_TRY(set_steps(c, 0, step));
     RL_TRY(enqueue_commit(c, h_order));
+    c->committed = true;
     unsigned long long hc = 0;
     RL_HIP(c, hipMemcpyAsync(&hc, c->counters + kCtrCreated, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
     RL_HIP(c, hipStreamSynchronize(c->stream));

@@ -113,6 +113,8 @@ struct Neurons {
     uint32_t* connected = nullptr;  // 3n
     uint32_t* deleted = nullptr;  // 3n
     double* net_nu = nullptr;     // growth rates per network (3 each), null: p.nu everywhere
+    double* min_ca = nullptr;     // 3n per-neuron minimum calcium (type-major), null: p.eta
+    double* retract = nullptr;    // 3n per-neuron vacant retract ratio, null: p.retract_ratio
     rl_step_params p{};
     bool ready = false;
 };
@@ -173,6 +175,8 @@ struct rl_bh_ctx {
     uint32_t* req_j = nullptr;
     uint8_t* req_ok = nullptr;
     bool searched_valid = false;
+    bool committed = false;          // the requests of the last search were committed (a second commit
+                                     // would apply them again)
     uint64_t last_searches = 0;
     uint64_t last_tests = 0;
     uint64_t last_tests_ref = 0;
@@ -295,6 +299,7 @@ int neurons_commit(rl_bh_ctx* c, uint32_t update, uint32_t* deleted);
 int neurons_get(rl_bh_ctx* c, double* x, double* u, double* ca, uint8_t* fired, double* delta, double* grown,
                 uint32_t* connected);
 int neurons_set_network_growth(rl_bh_ctx* c, const double* nu);
+int neurons_set_element_params(rl_bh_ctx* c, const double* min_calcium, const double* retract_ratio);
 void neurons_free(rl_bh_ctx* c);
 
 // network graph (rl_net.hip)

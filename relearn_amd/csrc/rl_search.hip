@@ -730,6 +730,7 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     c->nreq = 0;
     c->slots = 0;
     c->searched_valid = false;
+    c->committed = false;
     if (n == 0) {
         c->searched_valid = true;
         return RL_OK;

@@ -60,6 +60,7 @@ struct StepArgs {
     const uint64_t* net_off;     // first neuron of each network
     const uint64_t* net_seed;    // draw seed of each network
     const double* net_nu;        // growth rates per network (3 each) or null: p.nu for all
+    const double* min_ca;        // 3n per-neuron minimum calcium (type-major) or null: p.eta
     const uint32_t* step_base;   // device step base of a captured period (step = *step_base + step) or null
     uint32_t step;
     double* x;
@@ -182,18 +183,25 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     }
     a.ca[i] = c;
     for (int t = 0; t < 3; t++) {
-        const double clamped_target = tgt < p.eta[t] ? p.eta[t] : tgt;  // std::max(target, eta)
-        const double dz = growth_curve(c, p.eta[t], clamped_target);
+        const double eta = a.min_ca != nullptr ? a.min_ca[t * n + i] : p.eta[t];
+        const double clamped_target = tgt < eta ? eta : tgt;  // std::max(target, eta)
+        const double dz = growth_curve(c, eta, clamped_target);
         const double nu = a.net_nu != nullptr ? a.net_nu[3 * static_cast<uint64_t>(g) + t] : p.nu[t];
         a.delta[t * n + i] = dl[t] + dz * nu;
     }
 }
 
+// Vacant retract ratios of the element commit: per neuron (3n, type-major) or one per type.
+struct Retract {
+    const double* per_neuron;
+    double r[3];
+};
+
 // Element types [k0 / n, k1 / n) of every neuron: index k = type * n + neuron.
 __global__ void k_elements_update(uint64_t n, uint64_t k0, uint64_t k1, double* __restrict__ grown,
                                   double* __restrict__ delta, uint32_t* __restrict__ c0, uint32_t* __restrict__ c1,
                                   uint32_t* __restrict__ c2, uint32_t* __restrict__ v0, uint32_t* __restrict__ v1,
-                                  uint32_t* __restrict__ v2, uint32_t* __restrict__ deleted) {
+                                  uint32_t* __restrict__ v2, uint32_t* __restrict__ deleted, Retract rr) {
 #pragma clang fp contract(off)
     const uint64_t k = k0 + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (k >= k1) {
@@ -206,7 +214,7 @@ __global__ void k_elements_update(uint64_t n, uint64_t k0, uint64_t k1, double* 
     const double current_count = grown[k];
     const double current_delta = delta[k];
     const uint32_t connected_integral = connected[i];
-    const double retract = 0.0;
+    const double retract = rr.per_neuron != nullptr ? rr.per_neuron[k] : (t == 0 ? rr.r[0] : (t == 1 ? rr.r[1] : rr.r[2]));
     const double current_connected = static_cast<double>(connected_integral);
     const double current_vacant = current_count - current_connected;
     const double new_vacant = current_vacant + current_delta;
@@ -254,7 +262,40 @@ void neurons_free(rl_bh_ctx* c) {
     dfree(&s.delta);
     dfree(&s.deleted);
     dfree(&s.net_nu);
+    dfree(&s.min_ca);
+    dfree(&s.retract);
     s.ready = false;
+}
+
+int neurons_set_element_params(rl_bh_ctx* c, const double* min_calcium, const double* retract_ratio) {
+    Neurons& s = c->neurons;
+    if (!s.ready) {
+        return fail(c, RL_ESTATE, "rl_neurons_set_element_params before rl_neurons_init");
+    }
+    const size_t cnt = 3 * static_cast<size_t>(c->n);
+    for (size_t k = 0; min_calcium != nullptr && k < cnt; k++) {
+        if (!std::isfinite(min_calcium[k])) {
+            return fail(c, RL_EINVAL, "rl_neurons_set_element_params: minimum calcium must be finite");
+        }
+    }
+    for (size_t k = 0; retract_ratio != nullptr && k < cnt; k++) {
+        if (!(retract_ratio[k] >= 0.0 && retract_ratio[k] <= 1.0)) {
+            return fail(c, RL_EINVAL, "rl_neurons_set_element_params: retract ratios must lie in [0, 1]");
+        }
+    }
+    destroy_graph(c);  // a captured period holds the old pointers
+    dfree(&s.min_ca);
+    dfree(&s.retract);
+    if (min_calcium != nullptr && cnt > 0) {
+        RL_TRY(dalloc(c, &s.min_ca, cnt));
+        RL_HIP(c, hipMemcpyAsync(s.min_ca, min_calcium, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    if (retract_ratio != nullptr && cnt > 0) {
+        RL_TRY(dalloc(c, &s.retract, cnt));
+        RL_HIP(c, hipMemcpyAsync(s.retract, retract_ratio, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    RL_HIP(c, hipStreamSynchronize(c->stream));
+    return RL_OK;
 }
 
 int neurons_set_network_growth(rl_bh_ctx* c, const double* nu) {
@@ -341,6 +382,7 @@ StepArgs step_args(rl_bh_ctx* c) {
     a.net_off = c->net_off;
     a.net_seed = c->net_seed;
     a.net_nu = s.net_nu;
+    a.min_ca = s.min_ca;
     a.step_base = nullptr;
     a.step = 0;
     a.x = s.x;
@@ -371,6 +413,15 @@ __global__ void k_copy_u8(uint64_t n, const uint8_t* __restrict__ from, uint8_t*
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n; i += stride) {
         to[i] = from[i];
     }
+}
+
+Retract retract_of(const Neurons& s) {
+    Retract r{};
+    r.per_neuron = s.retract;
+    for (int t = 0; t < 3; t++) {
+        r.r[t] = s.p.retract_ratio[t];
+    }
+    return r;
 }
 
 int check_synaptic(rl_bh_ctx* c) {
@@ -475,7 +526,8 @@ int enqueue_elements_commit(rl_bh_ctx* c) {
     }
     if (!c->net.ready) {
         k_elements_update<<<grid_for(3 * n), kBlock, 0, c->stream>>>(n, 0, 3 * n, s.grown, s.delta, c->conn[0], c->conn[1],
-                                                                      c->conn[2], c->vac[0], c->vac[1], c->vac[2], s.deleted);
+                                                                      c->conn[2], c->vac[0], c->vac[1], c->vac[2], s.deleted,
+                                                                      retract_of(s));
         RL_HIP(c, hipGetLastError());
         return RL_OK;
     }
@@ -483,7 +535,7 @@ int enqueue_elements_commit(rl_bh_ctx* c) {
     for (int t = 0; t < 3; t++) {
         k_elements_update<<<grid_for(n), kBlock, 0, c->stream>>>(n, t * n, (t + 1) * n, s.grown, s.delta, c->conn[0],
                                                                   c->conn[1], c->conn[2], c->vac[0], c->vac[1], c->vac[2],
-                                                                  s.deleted);
+                                                                  s.deleted, retract_of(s));
         RL_HIP(c, hipGetLastError());
         RL_TRY(enqueue_net_delete(c, t));
     }

@@ -52,9 +52,11 @@ public:
 class BarnesHutHIP {
 public:
     // BarnesHut(bounding_box, space_filling_curve, acceptance_criterion) (BarnesHut.h:55-60) with the
-    // Gaussian kernel's sigma and mu (Gaussian.h:47-52) and the seed of the draw stream.
+    // Gaussian kernel's sigma and mu (Gaussian.h:47-52) and the seed of the draw stream.  The default
+    // precision is the CPU path's FP64, since the adapter stands in under the CPU name `barnes-hut`;
+    // FP32-mixed is an explicit choice.
     BarnesHutHIP(const BoundingBox& box, double acceptance_criterion, double sigma = 750.0, double mu = 0.0,
-                 std::uint64_t seed = 0, Precision precision = Precision::FP32Mixed, int device = 0) {
+                 std::uint64_t seed = 0, Precision precision = Precision::FP64, int device = 0) {
         rl_bh_config cfg{};
         for (int d = 0; d < 3; d++) {
             cfg.box_min[d] = box.min[d];
@@ -104,6 +106,20 @@ public:
         enabled_.assign(n_, 1);
     }
 
+    // Algorithm::create_neurons (Algorithm.h:117; OctreeAlgorithm.h:73-85): the positions and signal
+    // types set before hold the new neurons n .. n + creation_count - 1 as well; they are inserted into
+    // the octree, enabled and without vacancies until the next prepare_update_connectivity.
+    void create_neurons(std::uint64_t creation_count) {
+        const std::uint64_t n1 = n_ + creation_count;
+        if (creation_count == 0 || positions_.size() != n1 || signal_.size() != n1) {
+            throw Error("BarnesHutHIP::create_neurons: positions and signal types must be set for the new neurons");
+        }
+        check(rl_bh_create_neurons(ctx_, creation_count, positions_[n_].data(), signal_.data() + n_, nullptr, nullptr,
+                                   nullptr, nullptr, nullptr));
+        n_ = n1;
+        enabled_.resize(n_, 1);
+    }
+
     // Algorithm::disable_neurons (Algorithm.h:125): disabled neurons neither search nor receive.
     void disable_neurons(std::span<const std::uint64_t> neuron_ids) {
         for (const auto id : neuron_ids) {
@@ -133,7 +149,11 @@ public:
     }
 
     // Algorithm::update_connectivity (Algorithm.h:149): search, conflict resolution and commit; the
-    // device vacancies are updated, the created synapses returned.
+    // device vacancies are updated, the created synapses returned.  The caller's synaptic elements still
+    // hold the vacancies of the prepare: ForwardConnector connects one dendrite of the target (of the
+    // type of the source's signal) and one axon of the source per created synapse (Connector.cpp:105,
+    // 182), which the caller does with the returned synapses (weight +1: excitatory dendrite, -1:
+    // inhibitory), so that the vacancies of its next prepare equal get_vacancies.
     ConnectivityUpdateResult update_connectivity(std::uint64_t number_neurons) {
         if (number_neurons != n_) {
             throw Error("BarnesHutHIP::update_connectivity: number of neurons differs from init");

@@ -7,6 +7,11 @@
 // u32 vacant_excitatory[n], u32 vacant_inhibitory[n].
 // Output file: per step u64 count, then count x (u32 target, u32 source, i32 weight); then the final
 // vacancies u32[3n].  Prints one JSON line per step with the created count.
+//
+// The caller keeps its own synaptic elements, as the reference's Neurons do: after every update it
+// connects one dendrite of each synapse's target and one axon of its source (ForwardConnector,
+// Connector.cpp:105,182), and the vacancies it hands to the next prepare are its own.  It checks them
+// against the device's committed vacancies (get_vacancies) and exits with status 3 when they differ.
 #include <array>
 #include <cstdint>
 #include <cstdio>
@@ -89,8 +94,23 @@ int main(int argc, char** argv) {
                 write_from(out, &src, 1);
                 write_from(out, &w, 1);
             }
-            // the caller's synaptic elements follow the device's committed vacancies
-            bh.get_vacancies(va, ve, vi);
+            // SynapticElements::connect_elements for each created synapse: the target's dendrite of the
+            // source's signal type, the source's axon
+            for (const auto& syn : r.local_synapses) {
+                std::vector<std::uint32_t>& dend = syn.weight > 0 ? ve : vi;
+                if (dend[syn.target] == 0 || va[syn.source] == 0) {
+                    std::cerr << "update_example: a synapse without a vacant element\n";
+                    return 3;
+                }
+                dend[syn.target]--;
+                va[syn.source]--;
+            }
+            std::vector<std::uint32_t> da(n), de(n), di(n);
+            bh.get_vacancies(da, de, di);
+            if (da != va || de != ve || di != vi) {
+                std::cerr << "update_example: caller-side vacancies differ from the device's after step " << s << "\n";
+                return 3;
+            }
             std::printf("{\"step\": %u, \"created\": %llu}\n", s, static_cast<unsigned long long>(r.number_created_synapses));
         }
         write_from(out, va.data(), n);

@@ -160,3 +160,36 @@ def test_one_neuron_with_thousands_of_axons(oracle, precision):
         assert np.mean(np.asarray(ref["target"]) != tgt) < 1e-2
     assert np.all(tgt != 17)
     g.close()
+
+
+def test_second_commit_of_one_search_is_refused():
+    # the requests of one search are committed once: a second rl_bh_commit returns RL_ESTATE and leaves
+    # the vacancies as the first left them (it would otherwise take every accepted dendrite and axon a
+    # second time); the requests and accept flags stay readable, and a new search allows a commit again
+    from relearn_amd._lib import BHError
+
+    _say("second commit refused")
+    net = uniform_cube(3000, 1000.0, 61)
+    g = BarnesHutHIP(net.box_min, net.box_max, seed=2)
+    g.init(net.xyz, net.signal)
+    g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+    m = g.find_target_neurons(0)
+    created = g.commit(0)
+    assert m > 0 and created > 0
+    vac = [v.copy() for v in g.get_vacancies()]
+    with pytest.raises(BHError) as err:
+        g.commit(0)
+    assert err.value.code == -6
+    assert all(np.array_equal(a, b) for a, b in zip(vac, g.get_vacancies()))
+    assert len(g.get_requests(m)[0]) == m and int(g.get_commit_result(m).sum()) == created
+    # a new search of the committed state can be committed
+    g.prepare_resident()
+    g.find_target_neurons(1)
+    g.commit(1)
+    # a replay with stats commits its own requests: a commit after it is refused too
+    g.graph_create(0, int(net.vac_axon.sum()))
+    g.graph_launch(0, 2, stats=True)
+    with pytest.raises(BHError) as err:
+        g.commit(2)
+    assert err.value.code == -6
+    g.close()

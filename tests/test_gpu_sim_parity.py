@@ -40,9 +40,17 @@ def _pairs(off, src, w):
 
 
 def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed, rng, cond=1.5,
-              eta=(0.02, 0.01, 0.0), target=0.1, nu=0.02, ca0=(0.05, 0.15), start_elements=True, check_every=1):
+              eta=(0.02, 0.01, 0.0), target=0.1, nu=0.02, ca0=(0.05, 0.15), start_elements=True, check_every=1,
+              retract=(0.0, 0.0, 0.0), min_ca=None, retract_per_neuron=None, ca_init=None, target_ca=None,
+              create=None):
     """Runs `periods` periods on the device and in the oracle side by side and compares them every
-    `check_every` periods and after the last."""
+    `check_every` periods and after the last.
+
+    retract: vacant retract ratio per element type; min_ca / retract_per_neuron: (3, n) per-neuron
+    minimum calcium and retract ratios (rl_neurons_set_element_params); ca_init / target_ca: per-neuron
+    initial and target calcium; create: (period, xyz, signal) of a creation event at the start of that
+    period (rl_bh_create_neurons), the new neurons starting at x = c, u = 0, calcium 0, the target
+    calcium `target` and no elements."""
     n = len(xyz)
     pg = StepParams.izhikevich(input_mode=0, nu=nu)
     pg.synaptic_input, pg.synapse_conductance = 1, cond
@@ -52,6 +60,11 @@ def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed
     for t in range(3):
         pg.eta[t] = po.eta[t] = eta[t]
         po.nu[t] = pg.nu[t]
+        pg.retract_ratio[t] = retract[t]
+    rr = np.repeat(np.asarray(retract, np.float64)[:, None], n, axis=1)
+    if retract_per_neuron is not None:
+        rr = np.array(retract_per_neuron, dtype=np.float64)
+    mc = None if min_ca is None else np.array(min_ca, dtype=np.float64)
     og = oracle.Graph(n) if graph is None else oracle.Graph(n, *graph)
     # connected counts that match the initial graph, a fraction of an element vacant on top
     conn = np.zeros((3, n), np.uint32)
@@ -68,12 +81,16 @@ def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed
     x = np.full(n, -65.0)
     u = np.full(n, -13.0)
     ca = rng.uniform(ca0[0], ca0[1], n) if ca0 is not None else np.zeros(n)
-    tca = np.full(n, target)
+    if ca_init is not None:
+        ca = np.array(ca_init, dtype=np.float64)
+    tca = np.full(n, target) if target_ca is None else np.array(target_ca, dtype=np.float64)
 
     g = BarnesHutHIP(box_min, box_max, seed=seed, precision=RL_PREC_FP64)
     g.init(xyz, signal)
     g.net_init(len(es) + 50 * n, *(graph if graph is not None else (None, None, None)))
     g.neurons_init(pg, tca, x=x, u=u, ca=ca, grown=grown.ravel(), connected=conn.ravel())
+    if mc is not None or retract_per_neuron is not None:
+        g.neurons_set_element_params(mc, None if retract_per_neuron is None else rr)
 
     tree = oracle.Tree(xyz, box_min, box_max)
     xo, uo, co = x.copy(), u.copy(), ca.copy()
@@ -83,17 +100,44 @@ def _simulate(oracle, xyz, signal, box_min, box_max, graph, periods, steps, seed
     vac = np.zeros((3, n), np.uint32)
     created_total = deleted_total = 0
     for per in range(periods):
+        if create is not None and create[0] == per:
+            # creation event (Neurons::create_neurons): both sides append the neurons; the oracle
+            # rebuilds its tree and graph over all of them
+            cx, cs = np.asarray(create[1], np.float64), np.asarray(create[2], np.uint8)
+            k = len(cx)
+            g.create_neurons(cx, cs, x=np.full(k, pg.c), u=np.zeros(k), ca=np.zeros(k), target_ca=np.full(k, target))
+            xyz = np.concatenate([xyz, cx])
+            signal = np.concatenate([signal, cs])
+            tree.close()
+            tree = oracle.Tree(xyz, box_min, box_max)
+            eo, es, ew = og.export()
+            og.close()
+            og = oracle.Graph(n + k, np.concatenate([eo, np.full(k, eo[-1], eo.dtype)]), es, ew)
+            xo = np.concatenate([xo, np.full(k, pg.c)])
+            uo = np.concatenate([uo, np.zeros(k)])
+            co = np.concatenate([co, np.zeros(k)])
+            fo = np.concatenate([fo, np.zeros(k, np.uint8)])
+            tca = np.concatenate([tca, np.full(k, target)])
+            do = np.concatenate([do.reshape(3, n), np.zeros((3, k))], axis=1).ravel()
+            go = np.concatenate([go, np.zeros((3, k))], axis=1)
+            conn = np.concatenate([conn, np.zeros((3, k), np.uint32)], axis=1)
+            vac = np.concatenate([vac, np.zeros((3, k), np.uint32)], axis=1)
+            rr = np.concatenate([rr, np.repeat(np.asarray(retract, np.float64)[:, None], k, axis=1)], axis=1)
+            if mc is not None:
+                mc = np.concatenate([mc, np.repeat(np.asarray(eta, np.float64)[:, None], k, axis=1)], axis=1)
+            n += k
+            _say("created", k, "neurons before period", per, "now", n)
         off, src, w = og.export()
         for k in range(steps):
             bg = rng.normal(5.0, 2.0, n)
             inp = oracle.synaptic_input(off, src, w, fo, cond, bg)
-            oracle.neuron_step(po, xo, uo, co, inp, fo, tca, do, None)
+            oracle.neuron_step(po, xo, uo, co, inp, fo, tca, do, None, min_calcium=mc)
             g.neurons_step(per * steps + k, bg)
         dv = do.reshape(3, n)
         dref = np.zeros((3, n), np.uint32)
         for t in range(3):
             gt, dt, ct, vt = go[t].copy(), dv[t].copy(), conn[t].copy(), vac[t].copy()
-            dref[t] = oracle.elements_update(gt, dt, ct, vt)
+            dref[t] = oracle.elements_update(gt, dt, ct, vt, retract_ratio=rr[t].copy())
             go[t], dv[t], conn[t], vac[t] = gt, dt, ct, vt
             deleted_total += og.delete(t, dref[t], signal, conn, vac, step=per, seed=seed)
         do = dv.ravel().copy()
@@ -174,3 +218,63 @@ def test_real_network_simulation_matches_oracle(oracle):
     created, deleted = _simulate(oracle, d["xyz"], d["signal"], d["box_min"], d["box_max"], graph, 2, 20, 8,
                                  np.random.default_rng(18), cond=0.03)
     assert deleted > 0
+
+
+def test_simulation_with_retract_ratio_matches_oracle(oracle):
+    # --retract-ratio (relearn.cpp:617-618): a fraction of the vacant elements retracts at every element
+    # commit (ElementBaseBase::update, ElementBaseBase.h:420-490), one ratio per element type
+    n, periods, steps, seed = 1000, 4, 100, 21
+    _say("simulation parity with retract ratios 0.1 / 0.25 / 0.5")
+    net = uniform_cube(n, 1000.0, 2)
+    created, deleted = _simulate(oracle, net.xyz, net.signal, net.box_min, net.box_max, None, periods, steps, seed,
+                                 np.random.default_rng(23), retract=(0.1, 0.25, 0.5))
+    assert created > 0
+
+
+def test_simulation_with_creation_event_matches_oracle(oracle):
+    # Neurons::create_neurons (Simulation.cpp:425) between periods: 200 neurons join a 1000-neuron
+    # network; the octree, the searches, the element state and the network graph must go on matching
+    n, periods, steps, seed = 1000, 4, 100, 25
+    _say("simulation parity with a creation event of 200 neurons")
+    net = uniform_cube(n, 1000.0, 4)
+    rng = np.random.default_rng(26)
+    cx = rng.random((200, 3)) * 1000.0
+    cs = (rng.random(200) < 0.2).astype(np.uint8)
+    created, deleted = _simulate(oracle, net.xyz, net.signal, net.box_min, net.box_max, None, periods, steps, seed,
+                                 np.random.default_rng(27), create=(2, cx, cs))
+    assert created > 0
+
+
+def test_reference_text_files_drive_a_simulation(oracle, tmp_path):
+    # The reference's own input files (the header and first 200 neurons of input/positions.txt, the
+    # first 40 synapses of input/rank_0_in_network.txt) and a calcium file and a synaptic-elements file
+    # in the layouts of CalciumIO / SynapticElementsIO, all read through relearn_amd.io into rl_bh_init,
+    # rl_net_init, rl_neurons_init and rl_neurons_set_element_params; three periods against the oracle.
+    from relearn_amd import io
+
+    pos = io.read_neurons(os.path.join(GOLDEN, "positions_head.txt"))
+    n = len(pos["xyz"])
+    off, src, w, _ = io.read_in_synapses(os.path.join(GOLDEN, "network_head_in.txt"), n)
+    rng = np.random.default_rng(31)
+    groups = io.groups_from_areas(pos["area"])
+    assert "lh_middletemporal" in groups
+    lines = ["# <description> <initial calcium> <target calcium>", "default 0.05 0.1", "lh_middletemporal 0.12 0.08"]
+    listed = [i for i in range(n) if i % 3 == 0 and i not in groups["lh_middletemporal"]]
+    for i in listed:
+        lines.append(f"0:{i + 1} {rng.uniform(0.05, 0.15):.6f} {rng.uniform(0.08, 0.12):.6f}")
+    (tmp_path / "calcium.txt").write_text("\n".join(lines) + "\n")
+    ca_init, ca_tgt = io.read_calcium(tmp_path / "calcium.txt", n, groups)
+    assert np.all(ca_init[groups["lh_middletemporal"]] == 0.12)
+    params = {f: np.zeros((3, n)) for f in io.ELEMENT_FIELDS}
+    params["min_calcium"][0] = rng.uniform(0.01, 0.04, n)
+    params["min_calcium"][1] = rng.uniform(0.005, 0.02, n)
+    params["retract_ratio"][:] = rng.uniform(0.0, 0.3, (3, n))
+    params["nu"][:] = 0.02
+    params["max_elements"][:] = 2.0
+    io.write_synaptic_elements(tmp_path / "elements.txt", params)
+    se = io.read_synaptic_elements(tmp_path / "elements.txt", n)
+    _say("reference text files: 200 neurons, 40 synapses, per-neuron calcium and element parameters")
+    created, deleted = _simulate(oracle, pos["xyz"], pos["signal"], pos["box_min"], pos["box_max"], (off, src, w), 3,
+                                 100, 33, rng, ca_init=ca_init, target_ca=ca_tgt, min_ca=se["min_calcium"],
+                                 retract_per_neuron=se["retract_ratio"])
+    assert created > 0

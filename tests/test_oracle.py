@@ -334,6 +334,61 @@ def test_find_targets_statistics(oracle):
     assert abs(np.mean(d) - np.sum(dall * w) / np.sum(w)) < 0.1 * np.sum(dall * w) / np.sum(w)
 
 
+def _fallback_tree(oracle, counts):
+    # source 0 at (1000, 1000, 1000); A, B, C at distances 3000, 4000 and 5000 um along the axes
+    xyz = np.array([[1000.0, 1000.0, 1000.0], [4000.0, 1000.0, 1000.0], [1000.0, 5000.0, 1000.0],
+                    [1000.0, 1000.0, 6000.0]])
+    ve = np.asarray(counts, np.uint32)
+    t = oracle.Tree(xyz, [0, 0, 0], [10000.0] * 3)
+    t.update(ve, ve)
+    return t, xyz
+
+
+def test_candidate_list_inverse_distance_fallback(oracle):
+    # Kernel::create_probability_interval (Kernel.h:109-126): when every Gaussian underflows
+    # (sigma = 1 um against 3000-5000 um) the probabilities become n / d.  Hand-computed: A (n = 2,
+    # d = 3000) 2/3000, B (n = 1, d = 4000) 1/4000, C (n = 3, d = 5000) 3/5000.
+    t, xyz = _fallback_tree(oracle, [5, 2, 1, 3])
+    e = t.export()
+    nodes, prob, total, fb, tests = t.candidate_list(0, xyz[0], 0, 0, 1e-6, 1.0)
+    assert fb
+    got = {int(e["link"][i]): p for i, p in zip(nodes, prob)}
+    assert set(got) == {1, 2, 3}
+    assert got[1] == pytest.approx(2 / 3000, rel=1e-15)
+    assert got[2] == pytest.approx(1 / 4000, rel=1e-15)
+    assert got[3] == pytest.approx(3 / 5000, rel=1e-15)
+    assert total == pytest.approx(2 / 3000 + 1 / 4000 + 3 / 5000, rel=1e-15)
+    # a Gaussian that does not underflow keeps the Gaussian (sigma = 3000: A = 2 e^-1)
+    nodes, prob, total, fb, _ = t.candidate_list(0, xyz[0], 0, 0, 1e-6, 3000.0)
+    assert not fb
+    got = {int(e["link"][i]): p for i, p in zip(nodes, prob)}
+    assert got[1] == pytest.approx(2 * math.exp(-1.0), rel=1e-15)
+    # only the source's own leaf has dendrites: the list is empty whatever the kernel
+    t2, _ = _fallback_tree(oracle, [5, 0, 0, 0])
+    nodes, prob, total, fb, _ = t2.candidate_list(0, xyz[0], 0, 0, 1e-6, 1.0)
+    assert len(nodes) == 0 and total == 0.0
+
+
+def test_find_targets_inverse_distance_picks(oracle):
+    # With the n / d fallback the picks of one source split as n_A / d_A : n_B / d_B.  A (n = 1,
+    # d = 1000) and B (n = 3, d = 3000) are equally likely; a 1/d^2 weight would give 3 : 1 and a
+    # weight without n 3 : 1 the other way.
+    xyz = np.array([[1000.0, 1000.0, 1000.0], [2000.0, 1000.0, 1000.0], [1000.0, 4000.0, 1000.0]])
+    ve = np.array([0, 1, 3], np.uint32)
+    t = oracle.Tree(xyz, [0, 0, 0], [10000.0] * 3)
+    t.update(ve, ve)
+    va = np.array([4000, 0, 0], np.uint32)
+    r = t.find_targets(np.zeros(3, np.uint8), va, seed=5, threads=1, theta=1e-6, sigma=1.0)
+    assert r["count"][0] == 4000 and r["stats"]["fallbacks"] > 0
+    frac_a = float(np.mean(r["target"] == 1))
+    assert abs(frac_a - 0.5) < 0.035   # 4.4 standard errors at 4000 draws
+    # sigma large enough for the Gaussian: no fallback, A wins exp(-(1/3)^2) : 3 exp(-1)
+    r = t.find_targets(np.zeros(3, np.uint8), va, seed=5, threads=1, theta=1e-6, sigma=3000.0)
+    assert r["stats"]["fallbacks"] == 0
+    pa, pb = math.exp(-1.0 / 9.0), 3.0 * math.exp(-1.0)
+    assert abs(float(np.mean(r["target"] == 1)) - pa / (pa + pb)) < 0.035
+
+
 def test_find_targets_deterministic_across_threads(oracle):
     rng = np.random.default_rng(12)
     n = 3000
