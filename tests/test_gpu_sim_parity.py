@@ -254,7 +254,13 @@ def test_reference_text_files_drive_a_simulation(oracle, tmp_path):
 
     pos = io.read_neurons(os.path.join(GOLDEN, "positions_head.txt"))
     n = len(pos["xyz"])
-    off, src, w, _ = io.read_in_synapses(os.path.join(GOLDEN, "network_head_in.txt"), n)
+    # the head of the in-network file names a few sources beyond the first n neurons: keep the synapses
+    # whose two ends are both among them, as a rank that loads this subset would
+    with open(os.path.join(GOLDEN, "network_head_in.txt")) as fh:
+        kept = [ln for ln in fh if ln.startswith("#") or max(int(ln.split()[1]), int(ln.split()[3])) <= n]
+    (tmp_path / "in_network.txt").write_text("".join(kept))
+    off, src, w, _ = io.read_in_synapses(tmp_path / "in_network.txt", n)
+    assert len(src) >= 30
     rng = np.random.default_rng(31)
     groups = io.groups_from_areas(pos["area"])
     assert "lh_middletemporal" in groups
