@@ -354,6 +354,30 @@ int rl_bh_graph_launch(rl_bh_ctx* ctx, uint32_t first_step, uint32_t update, rl_
 /* Releases the captured period (rl_bh_destroy does too). */
 int rl_bh_graph_destroy(rl_bh_ctx* ctx);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the conflict resolution and
+ * vacancy commit of rl_bh_commit in counter order, context-free, on caller device buffers.  It calls
+ * no hipMalloc/hipFree: its grouping scratch is a static device array of the library (about 26 MiB per
+ * GPU, held until the process ends), which limits a call to n <= 262144 neurons and m <= 1048576
+ * requests (RL_ELIMIT beyond).  Request i is (d_target[i], d_source[i], d_signal[i], d_axon_ordinal[i]),
+ * ids 0..n-1 of one network whose draw-stream seed is network_seed (rl_bh_network_seed(seed, 0) for a
+ * context created with seed); requests are processed in the order of the counter-based priority
+ * (network_seed, source, step, axon ordinal), ties by index, as rl_bh_commit with order NULL.  Self
+ * requests and requests naming an id >= n fail.  Accepted requests take one vacant dendrite of the
+ * target (signal 0 excitatory, 1 inhibitory) and one vacant axon of the source; the caller's requests
+ * of a source must not outnumber its vacant axons.  d_connected_* may be NULL (not counted).
+ * d_ok[i] = 1 for an accepted request.  All work is ordered on hip_stream (a hipStream_t, NULL = the
+ * default stream); with created != NULL the call waits for it and stores the number of accepted
+ * requests, with created == NULL it returns without waiting (capturable).  Calls that overlap on one
+ * GPU share the scratch array and race; rl_bh_last_error(NULL) describes a failure. */
+int rl_bh_commit_requests_noalloc(uint64_t n, uint64_t m, const uint32_t* d_target, const uint32_t* d_source,
+                                  const uint8_t* d_signal, const uint32_t* d_axon_ordinal, uint64_t network_seed,
+                                  uint32_t step, uint32_t* d_vacant_axons, uint32_t* d_vacant_excitatory_dendrites,
+                                  uint32_t* d_vacant_inhibitory_dendrites, uint32_t* d_connected_axons,
+                                  uint32_t* d_connected_excitatory_dendrites,
+                                  uint32_t* d_connected_inhibitory_dendrites, uint8_t* d_ok, void* hip_stream,
+                                  uint64_t* created);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,7 +34,7 @@ EXPORTED = [
     "rl_neurons_get", "rl_neurons_set_network_growth", "rl_bh_graph_create", "rl_bh_graph_launch",
     "rl_bh_graph_destroy", "rl_bh_set_tuning", "rl_bh_get_tuning", "rl_bh_autotune", "rl_net_init",
     "rl_net_get_edges", "rl_bh_candidate_lists", "rl_neurons_commit_elements_at", "rl_neurons_set_element_params",
-    "rl_bh_create_neurons",
+    "rl_bh_create_neurons", "rl_bh_commit_requests_noalloc",
 ]
 
 
@@ -182,6 +182,8 @@ def load():
     L.rl_net_get_edges.argtypes = [vp, C.c_uint64, _u64p, _u32p, C.POINTER(C.c_int32), _u64p]
     L.rl_bh_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.rl_bh_autotune.argtypes = [vp, C.c_uint32, C.POINTER(Tuning)]
+    L.rl_bh_commit_requests_noalloc.argtypes = [C.c_uint64, C.c_uint64] + [vp] * 4 + [C.c_uint64, C.c_uint32] + \
+        [vp] * 8 + [_u64p]
     for name in EXPORTED:
         f = getattr(L, name)
         if name not in ("rl_bh_last_error", "rl_bh_network_seed"):
@@ -492,3 +494,20 @@ class BarnesHutHIP:
 def network_seed(seed, network):
     """Draw-stream seed of network `network` of an ensemble."""
     return int(load().rl_bh_network_seed(seed, network))
+
+
+def commit_requests_noalloc(n, m, target, source, signal, axon_ordinal, network_seed, step, vacant, connected, ok,
+                            stream=0, wait=True):
+    """rl_bh_commit_requests_noalloc on device buffers given as integer addresses: target, source, signal,
+    axon_ordinal (m requests), vacant = (axon, excitatory, inhibitory) vacancy arrays and connected = the
+    same three connected-count arrays or None (n entries each), ok (m flags).  Returns the number of
+    accepted requests when wait is true, else None (the call then does not wait for the device)."""
+    L = load()
+    created = C.c_uint64()
+    conn = list(connected) if connected is not None else [None, None, None]
+    st = L.rl_bh_commit_requests_noalloc(n, m, target, source, signal, axon_ordinal, network_seed, step,
+                                         vacant[0], vacant[1], vacant[2], conn[0], conn[1], conn[2], ok,
+                                         stream or None, C.byref(created) if wait else None)
+    if st != 0:
+        raise BHError(st, L.rl_bh_last_error(None).decode())
+    return int(created.value) if wait else None

@@ -18,6 +18,15 @@ int fail(rl_bh_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
+namespace {
+thread_local char g_global_error[256] = "";
+}  // namespace
+
+void set_global_error(const char* msg) {
+    std::strncpy(g_global_error, msg, sizeof(g_global_error) - 1);
+    g_global_error[sizeof(g_global_error) - 1] = 0;
+}
+
 int hip_fail(rl_bh_ctx* c, hipError_t e, const char* what) {
     const int code = (e == hipErrorOutOfMemory) ? RL_EOOM : RL_EHIP;
     return fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
@@ -52,7 +61,6 @@ int ensure_tmp(rl_bh_ctx* c, size_t bytes) {
 
 namespace {
 
-thread_local std::string g_create_error;
 
 void free_all(rl_bh_ctx* c) {
     destroy_graph(c);
@@ -136,37 +144,37 @@ using namespace rl;
 extern "C" {
 
 int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
-    g_create_error.clear();
+    g_global_error[0] = 0;
     if (out == nullptr || cfg == nullptr) {
-        g_create_error = "rl_bh_create: null argument";
+        set_global_error("rl_bh_create: null argument");
         return RL_EINVAL;
     }
     *out = nullptr;
     // BarnesHut::BarnesHut (BarnesHut.h:55-60) and test_acceptance_criterion's theta range
     // (BarnesHutBase.h:74-77); GaussianDistributionKernel (Gaussian.h:50) requires sigma > 0.
     if (!(cfg->theta > 0.0) || !(cfg->theta <= 0.5) || !(cfg->sigma > 0.0) || !std::isfinite(cfg->mu)) {
-        g_create_error = "rl_bh_create: theta must be in (0, 0.5] and sigma > 0";
+        set_global_error("rl_bh_create: theta must be in (0, 0.5] and sigma > 0");
         return RL_EINVAL;
     }
     if (cfg->precision != RL_PREC_FP64 && cfg->precision != RL_PREC_FP32_MIXED) {
-        g_create_error = "rl_bh_create: unknown precision";
+        set_global_error("rl_bh_create: unknown precision");
         return RL_EINVAL;
     }
     const double big = std::numeric_limits<double>::max() / 4.0;
     for (int d = 0; d < 3; d++) {
         if (!(cfg->box_min[d] <= cfg->box_max[d]) || !(std::fabs(cfg->box_min[d]) < big) ||
             !(std::fabs(cfg->box_max[d]) < big)) {
-            g_create_error = "rl_bh_create: invalid simulation box";
+            set_global_error("rl_bh_create: invalid simulation box");
             return RL_EINVAL;
         }
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-        g_create_error = "rl_bh_create: no HIP device";
+        set_global_error("rl_bh_create: no HIP device");
         return RL_EHIP;
     }
     if (cfg->device < 0 || cfg->device >= ndev) {
-        g_create_error = "rl_bh_create: device ordinal out of range";
+        set_global_error("rl_bh_create: device ordinal out of range");
         return RL_EINVAL;
     }
     auto* c = new rl_bh_ctx();
@@ -174,7 +182,7 @@ int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
     if (hipSetDevice(cfg->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
-        g_create_error = "rl_bh_create: stream/event creation failed";
+        set_global_error("rl_bh_create: stream/event creation failed");
         delete c;
         return RL_EHIP;
     }
@@ -212,7 +220,7 @@ int rl_bh_destroy(rl_bh_ctx* c) {
 
 const char* rl_bh_last_error(const rl_bh_ctx* c) {
     if (c == nullptr) {
-        return g_create_error.c_str();
+        return g_global_error;
     }
     return c->st.msg.c_str();
 }

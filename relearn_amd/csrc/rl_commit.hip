@@ -43,7 +43,7 @@ __global__ void k_rank_from_order(uint64_t m, const uint64_t* __restrict__ order
 
 // Step 1.  The counter-based priority of request i is addressed like the draws: the network's seed,
 // the source's id inside its network, the update index and the axon ordinal.
-__global__ void k_commit_count(const uint64_t* __restrict__ n_req, const uint32_t* __restrict__ tgt,
+__global__ void k_commit_count(uint64_t n, const uint64_t* __restrict__ n_req, const uint32_t* __restrict__ tgt,
                                const uint32_t* __restrict__ src, const uint8_t* __restrict__ sig,
                                const uint32_t* __restrict__ rj, const uint32_t* __restrict__ net_of,
                                const uint64_t* __restrict__ net_off, const uint64_t* __restrict__ net_seed,
@@ -56,10 +56,10 @@ __global__ void k_commit_count(const uint64_t* __restrict__ n_req, const uint32_
         const uint32_t s = src[i];
         const uint32_t t = tgt[i];
         if (counter_order) {
-            const uint32_t g = net_of[s];
+            const uint32_t g = net_of != nullptr ? net_of[s] : 0U;   // no table: one network
             prio[i] = commit_priority(net_seed[g], s - static_cast<uint32_t>(net_off[g]), st, rj[i]);
         }
-        if (s == t) {
+        if (s == t || t >= n || s >= n) {   // self request, or an id outside the neurons: fails
             ok[i] = 0;
         } else {
             atomicAdd(&gcnt[2 * static_cast<uint64_t>(t) + (sig[i] & 1U)], 1U);
@@ -73,7 +73,7 @@ __global__ void k_commit_count(const uint64_t* __restrict__ n_req, const uint32_
 // groups keep the sentinel kUngrouped.
 constexpr uint32_t kUngrouped = 0xFFFFFFFFU;
 
-__global__ void k_commit_scatter(const uint64_t* __restrict__ n_req, const uint32_t* __restrict__ tgt,
+__global__ void k_commit_scatter(uint64_t n, const uint64_t* __restrict__ n_req, const uint32_t* __restrict__ tgt,
                                  const uint32_t* __restrict__ src, const uint8_t* __restrict__ sig,
                                  const uint64_t* __restrict__ prio, const uint64_t* __restrict__ goff,
                                  const uint32_t* __restrict__ vexc, const uint32_t* __restrict__ vinh,
@@ -83,7 +83,8 @@ __global__ void k_commit_scatter(const uint64_t* __restrict__ n_req, const uint3
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m; i += stride) {
         const uint32_t t = tgt[i];
-        if (src[i] == t) {
+        const uint32_t sr = src[i];
+        if (sr == t || t >= n || sr >= n) {
             continue;
         }
         const uint32_t ty = sig[i] & 1U;
@@ -154,7 +155,9 @@ __global__ void k_commit_groups(uint64_t groups, const uint64_t* __restrict__ go
         const uint32_t taken = size < v ? static_cast<uint32_t>(size) : v;
         if (taken > 0) {
             vac[t] = v - taken;
-            con[t] += taken;
+            if (con != nullptr) {
+                con[t] += taken;
+            }
             mine += taken;
         }
     }
@@ -211,13 +214,13 @@ int enqueue_commit(rl_bh_ctx* c, const uint64_t* h_order) {
         RL_HIP(c, hipGetLastError());
     }
     const uint64_t* n_req = c->req_off + n;
-    k_commit_count<<<kStreamBlocks, kBlock, 0, s>>>(n_req, c->req_target, c->req_source, c->req_signal, c->req_j,
+    k_commit_count<<<kStreamBlocks, kBlock, 0, s>>>(n, n_req, c->req_target, c->req_source, c->req_signal, c->req_j,
                                                     c->net_of, c->net_off, c->net_seed, c->d_steps + 1,
                                                     h_order == nullptr ? 1 : 0, c->c_prio, c->c_gcnt, c->req_ok);
     RL_HIP(c, hipGetLastError());
     RL_TRY(scan_counts(c, c->c_gcnt, groups, c->c_goff));
     RL_TRY(fill_u32(c, c->c_sidx, kUngrouped, 0, c->c_goff + groups, c->slots_cap));
-    k_commit_scatter<<<kStreamBlocks, kBlock, 0, s>>>(n_req, c->req_target, c->req_source, c->req_signal, c->c_prio,
+    k_commit_scatter<<<kStreamBlocks, kBlock, 0, s>>>(n, n_req, c->req_target, c->req_source, c->req_signal, c->c_prio,
                                                       c->c_goff, c->vac[1], c->vac[2], c->c_gcnt, c->c_sprio, c->c_sidx,
                                                       c->req_ok);
     RL_HIP(c, hipGetLastError());
@@ -258,3 +261,184 @@ int commit(rl_bh_ctx* c, uint32_t step, const uint64_t* h_order, uint64_t* creat
 }
 
 }  // namespace rl
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free commit on caller device buffers (rl_bh_commit_requests_noalloc).
+//
+// The grouping scratch of the commit (group counts and offsets, scan tile sums, priorities, grouped
+// requests, a small header) lives in one file-scope device array, `scratch`, sized for
+// kNoallocMaxNeurons neurons and kNoallocMaxRequests requests.  Every GPU that runs the library holds
+// its own copy until the process ends.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol): the device address of each GPU's copy comes from
+// hipGetSymbolAddress with that GPU current, once per GPU, and only that address is used.
+#include <mutex>
+
+namespace rl {
+namespace {
+
+constexpr uint64_t kNoallocMaxNeurons = 1ULL << 18;
+constexpr uint64_t kNoallocMaxRequests = 1ULL << 20;
+
+constexpr size_t align256(size_t b) {
+    return (b + 255U) & ~static_cast<size_t>(255U);
+}
+
+// byte offsets inside the array
+constexpr size_t kNaHdr = 0;   // NoallocHeader
+constexpr size_t kNaGcnt = 256;
+constexpr size_t kNaGoff = kNaGcnt + align256(2 * kNoallocMaxNeurons * sizeof(uint32_t));
+constexpr size_t kNaPart = kNaGoff + align256((2 * kNoallocMaxNeurons + 1) * sizeof(uint64_t));
+constexpr size_t kNaPrio = kNaPart + align256(((2 * kNoallocMaxNeurons + 4095) / 4096 + 1) * sizeof(uint64_t));
+constexpr size_t kNaSprio = kNaPrio + align256(kNoallocMaxRequests * sizeof(uint64_t));
+constexpr size_t kNaSidx = kNaSprio + align256(kNoallocMaxRequests * sizeof(uint64_t));
+constexpr size_t kNaBytes = kNaSidx + align256(kNoallocMaxRequests * sizeof(uint32_t));
+
+struct NoallocHeader {
+    uint64_t m;                    // requests of the call (the kernels read the count from here)
+    uint64_t net_off;              // 0: the requests' source ids are the ids in the network
+    uint64_t seed;                 // network seed of the counter priorities
+    unsigned long long created;    // accepted requests
+    uint32_t step;                 // plasticity update addressing the priorities
+};
+static_assert(sizeof(NoallocHeader) <= 256, "header slot");
+
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+__global__ void k_noalloc_setup(NoallocHeader* h, uint64_t m, uint64_t seed, uint32_t step) {
+    if (threadIdx.x == 0) {
+        h->m = m;
+        h->net_off = 0;
+        h->seed = seed;
+        h->created = 0;
+        h->step = step;
+    }
+}
+
+// Step 4, axon side, for requests in any order: every accepted request takes one vacant axon of its
+// source (the caller's requests of one source never outnumber its vacant axons).
+__global__ void k_commit_axons_by_request(const uint64_t* __restrict__ n_req, const uint32_t* __restrict__ src,
+                                          const uint8_t* __restrict__ ok, uint32_t* __restrict__ vax,
+                                          uint32_t* __restrict__ cax) {
+    const uint64_t m = *n_req;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m; i += stride) {
+        if (ok[i] != 0) {
+            const uint32_t s = src[i];
+            atomicSub(&vax[s], 1U);
+            if (cax != nullptr) {
+                atomicAdd(&cax[s], 1U);
+            }
+        }
+    }
+}
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+unsigned char* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// Device address of the current GPU's copy of `scratch`, looked up on the first call on that GPU.
+int noalloc_scratch(unsigned char** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_global_error("rl_bh_commit_requests_noalloc: hipGetDevice failed");
+        return RL_EHIP;
+    }
+    if (dev < 0 || dev >= kNoallocMaxDevices) {
+        set_global_error("rl_bh_commit_requests_noalloc: device ordinal beyond 63");
+        return RL_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_global_error("rl_bh_commit_requests_noalloc: hipGetSymbolAddress of the scratch array failed");
+            return RL_EHIP;
+        }
+        g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return RL_OK;
+}
+
+#define RL_NA_HIP(expr)                                                                   \
+    do {                                                                                  \
+        const hipError_t rl_e_ = (expr);                                                  \
+        if (rl_e_ != hipSuccess) {                                                        \
+            set_global_error("rl_bh_commit_requests_noalloc: HIP launch or copy failed"); \
+            return RL_EHIP;                                                               \
+        }                                                                                 \
+    } while (0)
+
+}  // namespace
+}  // namespace rl
+
+extern "C" int rl_bh_commit_requests_noalloc(uint64_t n, uint64_t m, const uint32_t* d_target,
+                                             const uint32_t* d_source, const uint8_t* d_signal,
+                                             const uint32_t* d_axon_ordinal, uint64_t network_seed, uint32_t step,
+                                             uint32_t* d_vacant_axons, uint32_t* d_vacant_excitatory_dendrites,
+                                             uint32_t* d_vacant_inhibitory_dendrites, uint32_t* d_connected_axons,
+                                             uint32_t* d_connected_excitatory_dendrites,
+                                             uint32_t* d_connected_inhibitory_dendrites, uint8_t* d_ok,
+                                             void* hip_stream, uint64_t* created) {
+    using namespace rl;
+    if (created != nullptr) {
+        *created = 0;
+    }
+    if (n > kNoallocMaxNeurons || m > kNoallocMaxRequests) {
+        set_global_error("rl_bh_commit_requests_noalloc: more than 262144 neurons or 1048576 requests "
+                         "(the size of the static scratch array)");
+        return RL_ELIMIT;
+    }
+    if (m == 0) {
+        return RL_OK;
+    }
+    if (d_target == nullptr || d_source == nullptr || d_signal == nullptr || d_axon_ordinal == nullptr ||
+        d_vacant_axons == nullptr || d_vacant_excitatory_dendrites == nullptr ||
+        d_vacant_inhibitory_dendrites == nullptr || d_ok == nullptr || n == 0) {
+        set_global_error("rl_bh_commit_requests_noalloc: null request, vacancy or result array, or n = 0");
+        return RL_EINVAL;
+    }
+    unsigned char* base = nullptr;
+    RL_TRY(noalloc_scratch(&base));
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    auto* hdr = reinterpret_cast<NoallocHeader*>(base + kNaHdr);
+    auto* gcnt = reinterpret_cast<uint32_t*>(base + kNaGcnt);
+    auto* goff = reinterpret_cast<uint64_t*>(base + kNaGoff);
+    auto* part = reinterpret_cast<uint64_t*>(base + kNaPart);
+    auto* prio = reinterpret_cast<uint64_t*>(base + kNaPrio);
+    auto* sprio = reinterpret_cast<uint64_t*>(base + kNaSprio);
+    auto* sidx = reinterpret_cast<uint32_t*>(base + kNaSidx);
+    const uint64_t groups = 2 * n;
+    // the array keeps the last call's contents: everything read before it is written is set here
+    k_noalloc_setup<<<1, 64, 0, s>>>(hdr, m, network_seed, step);
+    RL_NA_HIP(hipGetLastError());
+    RL_NA_HIP(fill_u32_on(s, gcnt, 0U, groups, nullptr, 0));
+    k_commit_count<<<kStreamBlocks, kBlock, 0, s>>>(n, &hdr->m, d_target, d_source, d_signal, d_axon_ordinal, nullptr,
+                                                    &hdr->net_off, &hdr->seed, &hdr->step, 1, prio, gcnt, d_ok);
+    RL_NA_HIP(hipGetLastError());
+    RL_NA_HIP(scan_counts_on(s, gcnt, groups, goff, part));
+    RL_NA_HIP(fill_u32_on(s, sidx, kUngrouped, 0, goff + groups, kNoallocMaxRequests));
+    k_commit_scatter<<<kStreamBlocks, kBlock, 0, s>>>(n, &hdr->m, d_target, d_source, d_signal, prio, goff,
+                                                      d_vacant_excitatory_dendrites, d_vacant_inhibitory_dendrites,
+                                                      gcnt, sprio, sidx, d_ok);
+    RL_NA_HIP(hipGetLastError());
+    k_commit_rank<<<kStreamBlocks, kBlock, 0, s>>>(goff + groups, goff, sprio, sidx, d_target, d_signal,
+                                                   d_vacant_excitatory_dendrites, d_vacant_inhibitory_dendrites, d_ok);
+    RL_NA_HIP(hipGetLastError());
+    k_commit_groups<<<kStreamBlocks, kBlock, 0, s>>>(groups, goff, d_vacant_excitatory_dendrites,
+                                                     d_vacant_inhibitory_dendrites, d_connected_excitatory_dendrites,
+                                                     d_connected_inhibitory_dendrites, &hdr->created);
+    RL_NA_HIP(hipGetLastError());
+    k_commit_axons_by_request<<<kStreamBlocks, kBlock, 0, s>>>(&hdr->m, d_source, d_ok, d_vacant_axons,
+                                                               d_connected_axons);
+    RL_NA_HIP(hipGetLastError());
+    if (created != nullptr) {
+        unsigned long long hc = 0;
+        RL_NA_HIP(hipMemcpyAsync(&hc, &hdr->created, sizeof(hc), hipMemcpyDeviceToHost, s));
+        RL_NA_HIP(hipStreamSynchronize(s));
+        *created = hc;
+    }
+    return RL_OK;
+}

@@ -227,6 +227,9 @@ struct rl_bh_ctx {
 namespace rl {
 
 int fail(rl_bh_ctx* c, int code, const std::string& msg);
+// message of the last failing context-free call of this thread (rl_bh_last_error(NULL)); a fixed
+// buffer, so setting it allocates nothing
+void set_global_error(const char* msg);
 int hip_fail(rl_bh_ctx* c, hipError_t e, const char* what);
 int ensure_tmp(rl_bh_ctx* c, size_t bytes);
 
@@ -269,6 +272,9 @@ void destroy_graph(rl_bh_ctx* c);
 // scans and fills (rl_scan.hip)
 uint64_t scan_scratch_entries(uint64_t n);
 int scan_counts(rl_bh_ctx* c, const uint32_t* in, uint64_t n, uint64_t* out);  // out[0..n], out[n] = total
+// the same on a given stream with scan_scratch_entries(n) tile sums at part (16-byte aligned operands)
+hipError_t scan_counts_on(hipStream_t s, const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* part);
+hipError_t fill_u32_on(hipStream_t s, uint32_t* p, uint32_t value, uint64_t n, const uint64_t* n_dev, uint64_t cap);
 int fill_u32(rl_bh_ctx* c, uint32_t* p, uint32_t value, uint64_t n, const uint64_t* n_dev, uint64_t cap);
 int set_steps(rl_bh_ctx* c, uint32_t neuron_step, uint32_t update);
 

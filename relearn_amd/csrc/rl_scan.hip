@@ -172,31 +172,42 @@ uint64_t scan_scratch_entries(uint64_t n) {
     return (n + kScanTile - 1) / kScanTile + 1;
 }
 
+hipError_t scan_counts_on(hipStream_t s, const uint32_t* in, uint64_t n, uint64_t* out, uint64_t* part) {
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles > 0) {
+        k_scan_reduce<<<static_cast<unsigned>(tiles), kScanBlock, 0, s>>>(in, n, part);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            return e;
+        }
+    }
+    k_scan_top<<<1, kTopBlock, 0, s>>>(part, tiles, out + n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && tiles > 0) {
+        k_scan_down<<<static_cast<unsigned>(tiles), kScanBlock, 0, s>>>(in, n, part, out);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
 int scan_counts(rl_bh_ctx* c, const uint32_t* in, uint64_t n, uint64_t* out) {
     if ((reinterpret_cast<uintptr_t>(in) & 15U) != 0 || (reinterpret_cast<uintptr_t>(out) & 15U) != 0) {
         return fail(c, RL_EINVAL, "scan_counts: operands must be 16-byte aligned");
     }
-    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
-    if (tiles + 1 > c->scan_part_cap) {
+    if (scan_scratch_entries(n) > c->scan_part_cap) {
         return fail(c, RL_ESTATE, "scan_counts: scratch smaller than the scan");
     }
-    hipStream_t s = c->stream;
-    if (tiles > 0) {
-        k_scan_reduce<<<static_cast<unsigned>(tiles), kScanBlock, 0, s>>>(in, n, c->scan_part);
-        RL_HIP(c, hipGetLastError());
-    }
-    k_scan_top<<<1, kTopBlock, 0, s>>>(c->scan_part, tiles, out + n);
-    RL_HIP(c, hipGetLastError());
-    if (tiles > 0) {
-        k_scan_down<<<static_cast<unsigned>(tiles), kScanBlock, 0, s>>>(in, n, c->scan_part, out);
-        RL_HIP(c, hipGetLastError());
-    }
+    RL_HIP(c, scan_counts_on(c->stream, in, n, out, c->scan_part));
     return RL_OK;
 }
 
+hipError_t fill_u32_on(hipStream_t s, uint32_t* p, uint32_t value, uint64_t n, const uint64_t* n_dev, uint64_t cap) {
+    k_fill_u32<<<kStreamBlocks, 256, 0, s>>>(p, value, n, n_dev, cap);
+    return hipGetLastError();
+}
+
 int fill_u32(rl_bh_ctx* c, uint32_t* p, uint32_t value, uint64_t n, const uint64_t* n_dev, uint64_t cap) {
-    k_fill_u32<<<kStreamBlocks, 256, 0, c->stream>>>(p, value, n, n_dev, cap);
-    RL_HIP(c, hipGetLastError());
+    RL_HIP(c, fill_u32_on(c->stream, p, value, n, n_dev, cap));
     return RL_OK;
 }
 
