@@ -29,10 +29,9 @@ B_NODE = {"fp64": 40, "fp32_mixed": 24}   # bytes read per acceptance test (reco
 B_IO = 32                                  # source position + output ids per search
 # Simulation workloads: Normal(5, 2) input alone drives the Izhikevich neurons to about 0.0117 spikes
 # per step, so calcium settles near 0.117 and never reaches the reference's axon growth threshold
-# 0.4, and with no axons no synapse forms to add input; these thresholds and target let elements
-# grow and connect.
-SIM_ETA = (0.02, 0.01, 0.0)
-SIM_TARGET_CA = 0.2
+# 0.4, and with no axons no synapse forms to add input; the "growing" thresholds and target let
+# elements grow and connect, the "reference" ones are the configuration as stated.
+SIM_THRESHOLDS = {"growing": ((0.02, 0.01, 0.0), 0.2), "reference": ((0.4, 0.1, 0.0), 0.7)}
 
 
 def log(*a):
@@ -254,12 +253,13 @@ def run_sim(args):
         desc = f"{n} neurons uniform in a {args.sim_side:g} um cube, growth rate {1e-4 * args.nu_scale:g}"
     log(f"sim workload: {desc}, generated in {time.perf_counter() - t0:.1f} s")
     periods, period = args.sim_periods, args.period
+    sim_eta, sim_target = SIM_THRESHOLDS[args.thresholds]
     every = max(1, periods // 10)
     out = {"metric": "simulated neuron steps/s (plasticity every period steps, one graph replay per period)",
            "unit": "neuron-steps/s", "n_gpus": 1, "steps": periods, "warmup": 0, "higher_is_better": True,
            "data": "synthetic",
            "config": {"workload": f"{desc}; Izhikevich + calcium + Gaussian growth, input Normal(5, 2), "
-                                  f"growth thresholds eta {SIM_ETA}, target calcium {SIM_TARGET_CA}, theta 0.3, "
+                                  f"growth thresholds eta {sim_eta}, target calcium {sim_target} ({args.thresholds}), theta 0.3, "
                                   f"sigma 750 um, {periods} periods of {period} steps, elements start at 0, "
                                   f"equally weighted synaptic input (conductance {args.conductance:g}) over the "
                                   "network the updates build",
@@ -281,12 +281,12 @@ def run_sim(args):
         g.prepare_update_connectivity(zero, zero, zero)
         p = StepParams.izhikevich(input_mode=1, nu=1e-4 * args.nu_scale)
         for t in range(3):
-            p.eta[t] = SIM_ETA[t]
+            p.eta[t] = sim_eta[t]
         # synaptic input over the device network graph, which every update extends
         p.synaptic_input = 1
         p.synapse_conductance = args.conductance
         g.net_init(24 * n)
-        g.neurons_init(p, np.full(n, SIM_TARGET_CA), x=np.full(n, -65.0), u=np.full(n, -13.0))
+        g.neurons_init(p, np.full(n, sim_target), x=np.full(n, -65.0), u=np.full(n, -13.0))
         if args.workload == "sim-ensemble":
             g.neurons_set_network_growth(np.repeat(ens.nu * args.nu_scale, 3))
         cap = 4 * n
@@ -408,19 +408,51 @@ def read_traffic(kernel, precision, neurons):
         return None
 
 
-def read_valu_busy(kernel, precision_label):
-    """VALU busy fraction of the search kernel from the committed rocprofv3 counter summary
-    (SQ_INSTS_VALU x 4 cycles over all SIMD cycles), or None."""
+def read_counters(kernel, precision_label):
+    """Limiter figures of the search kernel from the committed rocprofv3 counter summary
+    profiles/<kernel>_<precision>_16M_counters.md (lines '- <name>: <value>' under 'Derived'), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{precision_label}_16M_counters.md")
     if not os.path.exists(path):
         return None
+    out = {"resource": "valu", "profile": os.path.relpath(path, ROOT)}
+    keys = {"VALU busy": "valu_busy", "L2 hit rate": "l2_hit_rate", "VALU instructions per active source":
+            "valu_insts_per_source", "SALU instructions per active source": "salu_insts_per_source",
+            "commit": "profiled_commit"}
     with open(path) as f:
         for line in f:
-            if line.startswith("- VALU busy"):
-                try:
-                    return float(line.rsplit(":", 1)[1])
-                except ValueError:
-                    return None
+            if not line.startswith("- ") or ":" not in line:
+                continue
+            k, v = line[2:].split(":", 1)
+            for prefix, name in keys.items():
+                if k.strip().startswith(prefix):
+                    v = v.strip()
+                    try:
+                        out[name] = float(v)
+                    except ValueError:
+                        out[name] = v
+    return out
+
+
+def host_threads():
+    """Host cores this process may use: OMP_NUM_THREADS when set (the GPU host's per-GPU share), else the
+    CPUs of the process's affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
     return None
 
 
@@ -466,9 +498,14 @@ def main(argv=None):
     ap.add_argument("--nu-scale", type=float, default=10.0, help="simulation: growth rates x this factor")
     ap.add_argument("--conductance", type=float, default=0.03,
                     help="simulation: synapse conductance (the reference's default 0.03)")
+    ap.add_argument("--thresholds", choices=["growing", "reference"], default="growing",
+                    help="simulation: growth thresholds eta 0.02/0.01/0 with target calcium 0.2 (axons grow), "
+                         "or the reference's 0.4/0.1/0 with target 0.7 (calcium stays below the axons' threshold)")
     ap.add_argument("--networks", type=int, default=256, help="networks of the ensemble workload")
     ap.add_argument("--no-unbatched", action="store_true", help="ensemble: skip the one-context-per-network run")
     args = ap.parse_args(argv)
+    # OpenMP threads of the CPU baseline stay on neighbouring cores (read when the runtime loads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.calibrate_copy:
         return copy_calibration(args.calibrate_copy)
 
@@ -513,12 +550,14 @@ def main(argv=None):
     main_name = "fp32_mixed" if "fp32_mixed" in res else ("fp64" if "fp64" in res else "fp64_plain")
     main_r = res[main_name]
     b_node = B_NODE["fp32_mixed" if main_name == "fp32_mixed" else "fp64"]
-    # algorithmic (logical) bytes: every acceptance test of the reference's traversals reads one node
+    # algorithmic (logical) bytes of one launch: every acceptance test the kernel executes reads one node
     # record, every search reads its source and writes its result
-    bytes_per_launch = main_r["node_tests_ref_per_step"] * b_node + main_r["searches_per_step"] * B_IO
-    achieved = bytes_per_launch / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9
+    bytes_per_launch = main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO
+    kernel_s = 1e-3 * main_r["dominant_kernel_ms"]
+    achieved = bytes_per_launch / kernel_s / 1e9
     kname = "k_level1" if args.kernel == "coop" and main_name != "fp64_plain" else "k_search"
     traffic = read_traffic(kname, main_name, args.neurons)
+    counters = read_counters(kname, main_name) if args.neurons == 16_000_000 else None
     out = {
         "metric": METRIC,
         "value": main_r["searches_per_s"],
@@ -549,14 +588,20 @@ def main(argv=None):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "bytes_model": f"reference acceptance tests x {b_node} B + searches x {B_IO} B (logical bytes; "
-                           "node records are re-read from L2/MALL, so the frac can exceed 1)",
-            "achieved_executed_tests": (main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO)
-            / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9,
+            "bytes_model": f"executed acceptance tests x {b_node} B + searches x {B_IO} B per launch (logical "
+                           "bytes: node records are re-read from L2, so this is the kernel's demand, not HBM "
+                           "traffic)",
             "kernel_ms": main_r["dominant_kernel_ms"],
-            "traffic_gbs": None if traffic is None else traffic / (1e-3 * main_r["dominant_kernel_ms"]) / 1e9,
-            # what bounds the kernel: instruction issue (rocprofv3 counters of the same kernel at 16M)
-            "valu_busy": read_valu_busy(kname, main_name) if args.neurons == 16_000_000 else None,
+            "node_tests_per_launch": main_r["node_tests_per_step"],
+            "searches_per_launch": main_r["searches_per_step"],
+            # the reference's traversal re-derives the level-1 list for every axon: its test count is
+            # larger than the kernel's, which derives it once per neuron
+            "achieved_reference_tests": (main_r["node_tests_ref_per_step"] * b_node + main_r["searches_per_step"] * B_IO)
+            / kernel_s / 1e9,
+            "hbm_physical_gbs": None if traffic is None else traffic / kernel_s / 1e9,
+            "hbm_physical_frac": None if traffic is None else traffic / kernel_s / 1e9 / HBM_PEAK_GBS,
+            # what limits the kernel, from its rocprofv3 counters: instruction issue, not HBM
+            "limiter": counters,
         },
     }
     for name, r in res.items():
@@ -586,9 +631,11 @@ def main(argv=None):
             "synapses_rel_diff": abs(b["created0"] - a["created0"]) / max(1, a["created0"]),
         }
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_threads()
         r, prep, ns = cpu_baseline(net, args.cpu_sample, threads, args.seed + 1)
         sps = r["searches"] / r["seconds"]
+        r1, _, ns1 = cpu_baseline(net, max(1, args.cpu_sample // 8), 1, args.seed + 2)
+        sps1 = r1["searches"] / r1["seconds"]
         out["cpu_baseline"] = {
             "value": sps,
             "unit": "searches/s",
@@ -596,11 +643,16 @@ def main(argv=None):
             "kind": "port",
             "sample": f"{r['searches']} searches of {ns} random source neurons on the full "
                       f"{args.neurons}-neuron pointer octree (mt19937 stream, omp parallel for + critical "
-                      f"appends); tree build + upward pass {prep:.1f} s not in the rate",
-            "ms_per_step_extrapolated": 1e3 * main_r["searches_per_step"] / sps,
+                      f"appends, OMP_PROC_BIND=close); tree build + upward pass {prep:.1f} s not in the rate",
+            "cpu_model": cpu_model(),
+            "host_cpus_visible": os.cpu_count(),
+            "single_thread_value": sps1,
+            "single_thread_sample": f"{r1['searches']} searches of {ns1} source neurons, 1 thread",
+            "search_ms_per_step_extrapolated": 1e3 * main_r["searches_per_step"] / sps,
             "gpu_speedup": out["value"] / sps,
         }
-        log(f"cpu baseline: {sps:.4g} searches/s on {threads} threads ({r['searches']} searches)")
+        log(f"cpu baseline: {sps:.4g} searches/s on {threads} threads ({r['searches']} searches), "
+            f"{sps1:.4g} on 1 thread")
     print(json.dumps(out), flush=True)
     return 0
 
