@@ -38,7 +38,9 @@ enum {
 };
 
 enum {
-    RL_FLAG_EXACT_ORDER = 1,  /* per-source search with sums in the CPU path's list order (slower) */
+    RL_FLAG_EXACT_ORDER = 1,      /* per-source search with sums in the CPU path's list order (slower) */
+    RL_FLAG_WAVE_PER_SOURCE = 2,  /* FP32-mixed: one wavefront per source (the FP64 search) instead of the
+                                     group walk (one lane per source, 64 neighbouring sources per wavefront) */
 };
 
 enum {

@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librelearn_bh.so")
 
-SOURCES = ["rl_api.hip", "rl_tree.hip", "rl_scan.hip", "rl_search.hip", "rl_search_coop.hip", "rl_commit.hip",
+SOURCES = ["rl_api.hip", "rl_tree.hip", "rl_scan.hip", "rl_search.hip", "rl_search_coop.hip", "rl_search_walk.hip",
+           "rl_commit.hip",
            "rl_step.hip", "rl_net.hip"]
 
 ARCH = os.environ.get("RELEARN_OFFLOAD_ARCH", "gfx950")

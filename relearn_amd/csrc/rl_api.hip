@@ -104,6 +104,15 @@ void free_all(rl_bh_ctx* c) {
     dfree(&c->coop_overflow);
     dfree(&c->redo);
     dfree(&c->coop_node);
+    dfree(&c->walk_flag);
+    dfree(&c->walk_off);
+    dfree(&c->walk_list);
+    dfree(&c->walk_seg_groups);
+    dfree(&c->walk_seg_goff);
+    dfree(&c->walk_seg_first);
+    dfree(&c->walk_seg_count);
+    dfree(&c->walk_cum);
+    dfree(&c->walk_child);
     dfree(&c->slow_items);
     dfree(&c->coop_fexp);
     dfree(&c->coop_mass);
@@ -327,12 +336,18 @@ int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const doubl
         int cus = 0;
         RL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
         c->cus = static_cast<uint32_t>(std::max(cus, 1));
+        if (uses_walk(c)) {
+            c->waves_per_cu = 24;   // the group walk's tables and LDS are sized for 24 waves per CU
+        }
         c->coop_blocks = c->cus * c->waves_per_cu;
         c->coop_cap = 2048;
         const size_t slots = static_cast<size_t>(c->cus) * kMaxWavesPerCu * c->coop_cap;
         RL_TRY(dalloc(c, &c->coop_node, slots));
         RL_TRY(dalloc(c, &c->coop_fexp, slots));
         RL_TRY(dalloc(c, &c->coop_mass, slots));
+        if (uses_walk(c)) {
+            RL_TRY(ensure_walk(c));
+        }
     }
     hipStream_t s = c->stream;
     RL_HIP(c, hipMemcpyAsync(c->net_off, c->h_net_off.data(), (G + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));

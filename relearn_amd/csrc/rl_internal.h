@@ -211,6 +211,19 @@ struct rl_bh_ctx {
     uint32_t cus = 0;
     uint32_t waves_per_cu = 16;
     uint32_t search_chunk = 1;       // sources per work-queue grab of a search wave
+    // group walk (the FP32-mixed search): per order position and type the active flags, their offsets,
+    // the grouped source list, the (type, network) segments, and per wave the walk's tables
+    uint32_t* walk_flag = nullptr;        // 2n
+    uint64_t* walk_off = nullptr;         // 2n + 1
+    uint32_t* walk_list = nullptr;        // n
+    uint32_t* walk_seg_groups = nullptr;  // 2 networks
+    uint64_t* walk_seg_goff = nullptr;    // 2 networks + 1
+    uint64_t* walk_seg_first = nullptr;   // 2 networks
+    uint64_t* walk_seg_count = nullptr;   // 2 networks
+    double* walk_cum = nullptr;           // walk_blocks_cap x walk_cap x 64
+    uint32_t* walk_child = nullptr;       // walk_blocks_cap x walk_cap x 8
+    uint32_t walk_cap = 0;
+    uint32_t walk_blocks_cap = 0;
     // (slot, source, node, level) of descents that need the general per-lane traversal (slots_cap)
     uint4* slow_items = nullptr;
     void* tmp = nullptr;
@@ -288,6 +301,14 @@ int candidate_lists(rl_bh_ctx* c, uint64_t num_sources, const uint32_t* sources,
                     double* probs, uint32_t* counts);
 struct SearchArgs;
 int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa);
+int launch_finalize(rl_bh_ctx* c, const SearchArgs& sa);
+int launch_search_walk(rl_bh_ctx* c, const SearchArgs& sa);
+int ensure_walk(rl_bh_ctx* c);   // group-walk buffers (FP32-mixed contexts), sized at init
+// the context's level-1 search: the group walk (FP32-mixed by default), the cooperative wave per
+// source, or one lane per source in the CPU path's order
+inline bool uses_walk(const rl_bh_ctx* c) {
+    return c->cfg.precision == RL_PREC_FP32_MIXED && (c->cfg.flags & (RL_FLAG_EXACT_ORDER | RL_FLAG_WAVE_PER_SOURCE)) == 0;
+}
 int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, const unsigned long long* count);
 
 // commit (rl_commit.hip)

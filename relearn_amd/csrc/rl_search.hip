@@ -692,7 +692,11 @@ int enqueue_search(rl_bh_ctx* c) {
         a.order = c->redo;
         a.nwork = reinterpret_cast<const uint64_t*>(c->counters + kCtrRedo);
     } else {
-        RL_TRY(launch_search_coop(c, a));
+        if (uses_walk(c)) {
+            RL_TRY(launch_search_walk(c, a));
+        } else {
+            RL_TRY(launch_search_coop(c, a));
+        }
         a.order = c->coop_overflow;
         a.nwork = reinterpret_cast<const uint64_t*>(c->counters + kCtrOverflow);
     }

@@ -535,7 +535,6 @@ __device__ ListResult list_of(const CoopArgs& a, const Lane& L, uint32_t root, C
     return lr;
 }
 
-constexpr uint32_t kOverflowMark = 0xFFFFFFFFU;   // found[src] of a source handed to the per-lane kernel
 
 // Per-draw descent state, identical in the 8 lanes of the draw's group.
 enum : int { kDrawActive = 0, kDrawDone = 1, kDrawFailed = 2, kDrawSlow = 3, kDrawIdle = 4 };
@@ -832,8 +831,12 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa) {
     RL_HIP(c, hipGetLastError());
     if (timed) RL_HIP(c, hipEventRecord(c->ev3, s));
     RL_TRY(launch_descend_lane(c, sa, c->slow_items, c->counters + kCtrSlow));
+    return launch_finalize(c, sa);
+}
+
+int launch_finalize(rl_bh_ctx* c, const SearchArgs& sa) {
     const uint64_t n = std::max<uint64_t>(c->n, 1);
-    k_finalize<<<static_cast<unsigned>(std::min<uint64_t>((n + 255) / 256, kStreamBlocks)), 256, 0, s>>>(sa);
+    k_finalize<<<static_cast<unsigned>(std::min<uint64_t>((n + 255) / 256, kStreamBlocks)), 256, 0, c->stream>>>(sa);
     RL_HIP(c, hipGetLastError());
     return RL_OK;
 }

@@ -7,6 +7,7 @@
 namespace rl {
 
 constexpr uint32_t kNone = 0xFFFFFFFFU;
+constexpr uint32_t kOverflowMark = 0xFFFFFFFFU;   // found[src] of a source handed to the per-lane kernel
 
 struct SearchArgs {
     uint64_t n;

@@ -1,0 +1,732 @@
+// Group walk: the FP32-mixed target search with one lane per source and one shared traversal.
+//
+// Active sources are grouped by (dendrite type searched, network) and, inside a group, taken 64 at a
+// time in Morton order, so that the 64 sources of a wavefront lie close together.  Their level-1
+// traversals (BarnesHutBase::get_nodes_to_consider, source/algorithm/BarnesHutInternal/
+// BarnesHutBase.h:129-206) then visit nearly the same nodes: the wavefront walks the union of the 64
+// traversals depth first, children in descending octant order (the reference's LIFO order), with one
+// node per step and a lane mask per stack entry holding the sources that expanded the entry's node.
+// Each lane evaluates the node only when it is in the mask (test_acceptance_criterion, :72-114, and
+// the Gaussian attraction, Kernel.h:61-75, in FP32 with FP64 decisions at the acceptance threshold
+// and at coincidences), so every lane sees exactly its own candidate list in the
+// reference's order and adds the probabilities of its accepted nodes to a running FP64 sum in that
+// order, as Kernel::create_probability_interval (Kernel.h:90-134) does.  The node data of a step is
+// the same for all lanes (one broadcast load); the per-lane work is the test, the exponential and
+// one FP64 addition.
+//
+// Every node that some lane expands gets a slot in the wave's table: the slots of its expanded
+// children and, per lane, the running sum when the walk leaves the node's subtree.  A draw r of lane
+// l is then resolved by lane l alone, top-down (ProbabilityPicker::pick_target, source/util/
+// ProbabilityPicker.h:46-112): from the root it scans the children of its current node in list order,
+// adds the probability of an accepted child, or jumps to the stored sum after an expanded child, and
+// stops at the first accepted child whose running sum reaches r or enters the first expanded child
+// whose stored sum does.  The running sums are the same FP64 additions in the same order as the
+// walk's, so the pick is the one of the sequential list.  Below a picked inner node the lane descends
+// (BarnesHutBase::find_target_neuron, :223-255): the node's children, normally all accepted, are
+// summed in list order, drawn from and picked; a level that needs expansion, or whose sum is 0 (the
+// reference's fallbacks), is handed to the per-lane general traversal.
+//
+// Sources whose list has candidates but a sum of 0 (FP32 underflow; the FP64 and n/r fallbacks), and
+// groups whose table or stack would overflow, go to the per-lane kernel whole.
+#include <algorithm>
+
+#include "rl_search_eval.h"
+
+namespace rl {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr uint32_t kGroup = 64;                  // sources per wavefront
+constexpr uint32_t kWalkDepth = kMaxLevels + 4;  // stack entries (one per tree level)
+constexpr int kXcds = kCursorSlots;
+constexpr uint32_t kWalkCap = 3584;              // table slots per wave (expanded nodes of one group)
+constexpr uint32_t kWalkWavesPerCu = 24;         // waves per CU the tables are sized for
+
+inline unsigned grid_for(uint64_t n, int block = kBlock) {
+    const uint64_t g = (n + block - 1) / block;
+    return static_cast<unsigned>(g == 0 ? 1 : g);
+}
+
+struct WalkArgs {
+    SearchArgs s;
+    const uint32_t* list;        // active sources by (type, network), Morton order inside
+    const uint64_t* seg_goff;    // nseg + 1: first group of each segment (segment = type * G + network)
+    const uint64_t* seg_first;   // nseg: list position of the segment's first source
+    const uint64_t* seg_count;   // nseg: sources of the segment
+    uint32_t nseg;
+    uint32_t networks;
+    unsigned long long* cursor;  // kXcds work cursors
+    uint32_t chunk;              // groups per cursor increment
+    double* t_cum;               // per wave: cap x 64 running sums at the exit of each slot's subtree
+    uint32_t* t_child;           // per wave: cap x 8 slots of the expanded children (kNone otherwise)
+    uint32_t cap;
+    uint32_t* overflow;          // sources for the per-lane kernel
+    unsigned long long* n_overflow;
+    uint4* slow;                 // (slot, source, node, level) of descents for the per-lane traversal
+    unsigned long long* n_slow;
+};
+
+constexpr uint32_t kBlockLevels = 24;            // stack levels whose children are staged in LDS
+
+// The stack below the top entry, and per stack level the records of the level's children (copied
+// from the tree by the load-to-LDS instructions when the entry is pushed, so that the steps over a
+// node's children read them from LDS instead of waiting for memory one child at a time).
+struct WalkShared {
+    uint32_t first[kWalkDepth];
+    uint32_t cur[kWalkDepth];
+    uint32_t slot[kWalkDepth];
+    uint64_t mask[kWalkDepth];
+    uint4 rec[kBlockLevels][8];
+    uint32_t meta[kBlockLevels][16];
+};
+
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __lane_id();
+}
+
+__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32U),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0U));
+}
+
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
+// The source data a lane needs for the FP32 test and, on the rare FP64 re-decision, for the exact one.
+struct WalkLane {
+    int32_t qx, qy, qz;
+    uint32_t src;
+};
+
+struct Group {
+    float k2q, inv_q;
+    const uint2* meta;
+    const uint4* rec32;
+    const double4* rec64;
+};
+
+typedef __attribute__((address_space(3))) void LdsVoid;
+
+// Starts copying the records of the children [first, first + nc) into LDS level `lv` (lanes < nc copy
+// a 16-byte position record each, lanes < 2 nc one word of a count/link record each).
+__device__ __forceinline__ void stage_children(WalkShared& sh, const Group& G, uint32_t lv, uint32_t first,
+                                               uint32_t nc, uint32_t ln) {
+    if (ln < nc) {
+        __builtin_amdgcn_global_load_lds((void*)(G.rec32 + first + ln), (LdsVoid*)&sh.rec[lv][0], 16, 0, 0);
+    }
+    if (ln < 2 * nc) {
+        __builtin_amdgcn_global_load_lds((void*)(reinterpret_cast<const uint32_t*>(G.meta) + 2 * first + ln),
+                                         (LdsVoid*)&sh.meta[lv][0], 4, 0, 0);
+    }
+}
+
+// s_waitcnt immediate: VM_CNT 0 (the staged records have landed in LDS), EXP_CNT and LGKM_CNT not waited for
+constexpr int kWaitVm0 = 0x0F70;
+
+// Node records read through the constant address space (scalar loads for wave-uniform indices).
+typedef unsigned int U32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int U32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) U32x2 KMeta;
+typedef const __attribute__((address_space(4))) U32x4 KRec;
+
+struct NodeEval {
+    int st;
+    double p;
+};
+
+enum : int { kUndecided = 3 };
+
+// The FP64 acceptance decision (test_acceptance_criterion, BarnesHutBase.h:72-114) of a node whose FP32
+// test is not conclusive: discard at distance 0, accept a leaf; for an inner node, accept iff the side
+// is below theta times the distance.  The comparison is made on squares (side^2 against theta^2 d^2,
+// each within a few units in the last place of the exact values) with a relative margin of 1e-13
+// around equality, outside of which it agrees with the reference's comparison of the side against
+// theta times the libstdc++ hypot of the centroid difference; inside the margin it is kUndecided and
+// the source goes to the per-lane search, which makes the reference's comparison itself.
+__device__ __forceinline__ int decide_exact(const SearchArgs& a, const Group& G, uint32_t src, uint32_t node,
+                                            bool leaf) {
+#pragma clang fp contract(off)
+    const double4 r = G.rec64[node];
+    const double dx = r.x - a.pos[src];
+    const double dy = r.y - a.pos[a.n + src];
+    const double dz = r.z - a.pos[2 * a.n + src];
+    if (dx == 0.0 && dy == 0.0 && dz == 0.0) {
+        return kDiscard;
+    }
+    if (leaf) {
+        return kAccept;
+    }
+    const double t = (a.theta * a.theta) * (dx * dx + dy * dy + dz * dz);
+    const double s2 = r.w * r.w;
+    if (s2 < t * (1.0 - 1.0e-13)) {
+        return kAccept;
+    }
+    if (s2 > t * (1.0 + 1.0e-13)) {
+        return kExpand;
+    }
+    return kUndecided;
+}
+
+// Acceptance test and probability of a loaded node for lanes with `act`: the FP32 test on exact
+// fixed-point differences, the Gaussian as exp2 of the scaled squared distance times the vacant count
+// in FP32.  Decisions inside the guard band and fixed-point coincidences are re-decided in FP64, so the
+// candidate lists are the FP64 path's; the probabilities stay FP32.  Other lanes get kDiscard, p = 0.
+template <bool kMu0>
+__device__ __forceinline__ NodeEval eval_walk(const SearchArgs& a, const Group& G, const WalkLane& W, uint32_t node,
+                                              uint2 mt, uint4 rq, bool act) {
+    const bool valid = act && mt.x != 0;
+    const bool leaf = (mt.y >> kLinkBits) == 0;
+    const int32_t ix = static_cast<int32_t>(rq.x) - W.qx;
+    const int32_t iy = static_cast<int32_t>(rq.y) - W.qy;
+    const int32_t iz = static_cast<int32_t>(rq.z) - W.qz;
+    const float dx = static_cast<float>(ix);
+    const float dy = static_cast<float>(iy);
+    const float dz = static_cast<float>(iz);
+    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    const float s2 = __uint_as_float(rq.w);
+    const float g = __builtin_fmaf(a.theta2f, d2, -s2);
+    const bool coinc = (ix | iy | iz) == 0;
+    const bool near = !leaf && fabsf(g) <= a.guard * s2;
+    float x;
+    if (kMu0) {
+        x = d2 * G.k2q;
+    } else {
+        const float dd = sqrtf(d2) * G.inv_q - a.muf;
+        x = dd * dd * a.k2f;
+    }
+    const float w = __builtin_amdgcn_exp2f(x) * static_cast<float>(mt.x);
+    NodeEval e;
+    e.st = valid ? ((leaf || g > 0.0f) ? kAccept : kExpand) : kDiscard;
+    e.p = e.st == kAccept ? static_cast<double>(w) : 0.0;
+    // the source's own leaf lies at distance 0: discarded without the FP64 test
+    const bool self = leaf && (mt.y & kLinkMask) == W.src;
+    const bool need = valid && !self && (coinc || near);
+    if (valid && self) {
+        e.st = kDiscard;
+        e.p = 0.0;
+    }
+    if (__builtin_amdgcn_ballot_w64(need) != 0) {
+        if (need) {
+            e.st = decide_exact(a, G, W.src, node, leaf);
+            e.p = e.st == kAccept ? static_cast<double>(w) : 0.0;
+        }
+    }
+    return e;
+}
+
+// Picker condition of ProbabilityPicker::pick_target: the first candidate whose running sum reaches r,
+// or, when r is not positive, the first candidate with a positive probability.
+__device__ __forceinline__ bool reaches(double sum, double r) {
+    return r > 0.0 ? sum >= r : sum > 0.0;
+}
+
+enum : int { kDescDone = 0, kDescFailed = 1, kDescSlow = 2 };
+
+// Descent below an accepted inner node of the lane's own list, level by level (find_target_neuron
+// repeated on the picked node): the children are tested, summed in list order (descending octant),
+// drawn from and picked.  A level with an expanded child or with a sum of 0 goes to the per-lane
+// general traversal (kDescSlow with node and level).  tests gets the reference's tests of the levels.
+template <bool kMu0>
+__device__ int descend(const SearchArgs& a, const Group& G, const WalkLane& W, uint64_t seed, uint32_t local,
+                       uint32_t step, uint32_t j, uint32_t& node, uint32_t& level, uint32_t& tests, uint32_t& exec) {
+    for (uint32_t guard = 0; guard < kMaxLevels + 2; guard++) {
+        if (node >= a.m) {
+            return kDescFailed;
+        }
+        const uint2 mn = G.meta[node];
+        const uint32_t nc = mn.y >> kLinkBits;
+        if (nc == 0) {
+            node = mn.y & kLinkMask;  // leaf: the target neuron
+            return kDescDone;
+        }
+        if (mn.x == 0) {
+            return kDescFailed;
+        }
+        const uint32_t first = mn.y & kLinkMask;
+        // the children in list order (descending octant): statuses and the sum
+        double tot = 0.0;
+        bool expand = false;
+        uint32_t accepted = 0;
+        for (uint32_t k = nc; k-- > 0;) {
+            const uint32_t y = first + k;
+            const NodeEval e = eval_walk<kMu0>(a, G, W, y, G.meta[y], G.rec32[y], true);
+            expand = expand || e.st == kExpand || e.st == kUndecided;
+            if (e.st == kAccept) {
+                tot += e.p;
+                accepted++;
+            }
+        }
+        exec += nc;
+        if (expand) {
+            return kDescSlow;
+        }
+        tests += nc;
+        if (accepted == 0) {
+            return kDescFailed;
+        }
+        if (tot == 0.0) {
+            return kDescSlow;
+        }
+        const double r = draw_to_uniform(pick_draw(seed, local, step, j, level), nextafter(tot, tot + 0.00001));
+        // the same children again: the first whose running sum reaches r
+        double cum = 0.0;
+        int sel = -1;
+        for (uint32_t k = nc; k-- > 0;) {
+            const uint32_t y = first + k;
+            const NodeEval e = eval_walk<kMu0>(a, G, W, y, G.meta[y], G.rec32[y], true);
+            exec++;
+            if (e.st == kAccept) {
+                const double cn = cum + e.p;
+                if (reaches(cn, r)) {
+                    sel = static_cast<int>(k);
+                    break;
+                }
+                cum = cn;
+            }
+        }
+        if (sel < 0) {
+            return kDescSlow;
+        }
+        node = first + static_cast<uint32_t>(sel);
+        level++;
+    }
+    return kDescFailed;
+}
+
+template <bool kMu0>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) k_walk(WalkArgs a) {
+    __shared__ WalkShared sh;
+    const SearchArgs& s = a.s;
+    const uint32_t ln = lane_id();
+    const uint64_t cbase = static_cast<uint64_t>(blockIdx.x) * a.cap;   // first table slot of this wave
+    double* const tcum = a.t_cum + cbase * kWave;
+    uint32_t* const tchild = a.t_child + cbase * 8;
+    int seg_x = static_cast<int>(blockIdx.x % kXcds);
+    int tried = 0;
+    unsigned long long acc_exec = 0;
+    const uint64_t nw = a.seg_goff[a.nseg];
+    const uint32_t step = *s.step;
+    const uint32_t chunk = a.chunk;
+    uint64_t w_next = 0, w_lim = 0;
+    while (true) {
+        if (w_next >= w_lim) {
+            const uint64_t seg_end = nw * static_cast<uint64_t>(seg_x + 1) / kXcds;
+            uint64_t w0 = 0;
+            if (ln == 0) {
+                w0 = nw * static_cast<uint64_t>(seg_x) / kXcds +
+                     atomicAdd(&a.cursor[seg_x], static_cast<unsigned long long>(chunk));
+            }
+            w0 = __shfl(w0, 0, kWave);
+            if (w0 >= seg_end) {
+                if (++tried >= kXcds) {
+                    break;
+                }
+                seg_x = (seg_x + 1) % kXcds;
+                continue;
+            }
+            w_next = w0;
+            w_lim = std::min<uint64_t>(w0 + chunk, seg_end);
+        }
+        const uint64_t w = w_next++;
+        // segment of group w: the last segment whose first group is <= w
+        uint32_t lo = 0, hi = a.nseg;   // seg_goff[lo] <= w < seg_goff[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (a.seg_goff[mid] <= w) {
+                lo = mid;
+            } else {
+                hi = mid;
+            }
+        }
+        const uint32_t seg = uniform_u32(lo);
+        const uint64_t gi = w - a.seg_goff[seg];
+        const uint32_t type = seg / a.networks;
+        const uint32_t net = seg - type * a.networks;
+        const uint64_t pos0 = a.seg_first[seg] + kGroup * gi;
+        const uint64_t left = a.seg_count[seg] - kGroup * gi;
+        const uint32_t gsize = static_cast<uint32_t>(std::min<uint64_t>(left, kGroup));
+        Group G;
+        G.meta = s.meta[type];
+        G.rec32 = s.rec32[type];
+        G.rec64 = s.rec64[type];
+        G.k2q = s.net_k2q[net];
+        G.inv_q = s.net_invq[net];
+        const uint64_t seed = s.net_seed[net];
+        const uint32_t net_first = static_cast<uint32_t>(s.net_off[net]);
+        const bool active = ln < gsize;
+        WalkLane W;
+        W.src = active ? a.list[pos0 + ln] : 0U;
+        W.qx = static_cast<int32_t>(s.qpos[W.src]);
+        W.qy = static_cast<int32_t>(s.qpos[s.n + W.src]);
+        W.qz = static_cast<int32_t>(s.qpos[2 * s.n + W.src]);
+        const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
+
+        // ---- the union walk: level-1 sums of all lanes ----
+        const uint2 mr = G.meta[net];
+        double cum = 0.0;
+        uint32_t tests = 0;
+        uint64_t has = 0;
+        uint64_t handoff = 0;   // sources with an FP64 decision too close to call here
+        bool over = false;
+        const uint32_t root_first = mr.y & kLinkMask;
+        const uint32_t root_nc = mr.x == 0 ? 0U : (mr.y >> kLinkBits);
+        if (root_nc != 0) {
+            // node records through the constant address space: below the LDS-staged levels the walk's
+            // node index is wave-uniform, so they are scalar loads into scalar registers
+            KMeta* const kmeta = (KMeta*)(G.meta);
+            KRec* const krec = (KRec*)(G.rec32);
+            uint32_t top_first = root_first, top_cur = root_nc, top_slot = 0;
+            uint64_t top_mask = amask;
+            uint32_t nslots = 1, sp = 0;
+            if (ln < 8) {
+                tchild[ln] = kNone;
+            }
+            stage_children(sh, G, 0, root_first, root_nc, ln);
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            // a lane's level-1 tests: the root's children and the children of every node it expands
+            tests = active ? root_nc : 0U;
+            while (true) {
+                if (top_cur == 0) {
+                    // the walk leaves the subtree of this slot's node
+                    tcum[static_cast<uint64_t>(top_slot) * kWave + ln] = cum;
+                    if (sp == 0) {
+                        break;
+                    }
+                    sp--;
+                    top_first = uniform_u32(sh.first[sp]);
+                    top_cur = uniform_u32(sh.cur[sp]);
+                    top_slot = uniform_u32(sh.slot[sp]);
+                    const uint64_t mk = sh.mask[sp];
+                    top_mask = (static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(mk >> 32U))) << 32U) |
+                               uniform_u32(static_cast<uint32_t>(mk));
+                    continue;
+                }
+                top_cur--;
+                const uint32_t y = top_first + top_cur;
+                uint2 mt;
+                uint4 rq;
+                if (sp < kBlockLevels) {
+                    mt = make_uint2(uniform_u32(sh.meta[sp][2 * top_cur]), uniform_u32(sh.meta[sp][2 * top_cur + 1]));
+                    rq = sh.rec[sp][top_cur];
+                } else {
+                    const U32x2 m2 = kmeta[y];
+                    const U32x4 r4 = krec[y];
+                    mt = make_uint2(m2.x, m2.y);
+                    rq = make_uint4(r4.x, r4.y, r4.z, r4.w);
+                }
+                if (mt.x == 0) {
+                    continue;   // no vacant dendrites below: discarded by every lane
+                }
+                const bool leaf = (mt.y >> kLinkBits) == 0;
+                const int32_t ix = static_cast<int32_t>(rq.x) - W.qx;
+                const int32_t iy = static_cast<int32_t>(rq.y) - W.qy;
+                const int32_t iz = static_cast<int32_t>(rq.z) - W.qz;
+                const float dx = static_cast<float>(ix);
+                const float dy = static_cast<float>(iy);
+                const float dz = static_cast<float>(iz);
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                // lane masks straight from the comparisons; combined with scalar mask arithmetic
+                const uint64_t coinc = __builtin_amdgcn_ballot_w64((ix | iy | iz) == 0);
+                float x;
+                if (kMu0) {
+                    x = d2 * G.k2q;
+                } else {
+                    const float dd = sqrtf(d2) * G.inv_q - s.muf;
+                    x = dd * dd * s.k2f;
+                }
+                const float w = __builtin_amdgcn_exp2f(x) * static_cast<float>(mt.x);
+                uint64_t acc, exm, need;
+                if (leaf) {
+                    // accepted unless at distance 0: the source's own leaf is dropped without a test in
+                    // FP64, another fixed-point coincidence is decided in FP64
+                    const uint64_t self = __builtin_amdgcn_ballot_w64((mt.y & kLinkMask) == W.src);
+                    need = top_mask & ~self & coinc;
+                    acc = top_mask & ~self & ~coinc;
+                    exm = 0;
+                } else {
+                    // accepted when theta^2 d^2 > side^2 clear of the guard band, expanded when below it
+                    const float s2 = __uint_as_float(rq.w);
+                    const float g = __builtin_fmaf(s.theta2f, d2, -s2);
+                    const uint64_t near = __builtin_amdgcn_ballot_w64(fabsf(g) <= s.guard * s2);
+                    const uint64_t pos = __builtin_amdgcn_ballot_w64(g > 0.0f);
+                    need = top_mask & (coinc | near);
+                    const uint64_t sure = top_mask & ~(coinc | near);
+                    acc = sure & pos;
+                    exm = sure & ~pos;
+                }
+                if (need != 0) {
+                    int st = kDiscard;
+                    if (__builtin_amdgcn_inverse_ballot_w64(need)) {
+                        st = decide_exact(s, G, W.src, y, leaf);
+                    }
+                    acc |= __builtin_amdgcn_ballot_w64(st == kAccept);
+                    exm |= __builtin_amdgcn_ballot_w64(st == kExpand);
+                    handoff |= __builtin_amdgcn_ballot_w64(st == kUndecided);
+                }
+                const float wa = __builtin_amdgcn_inverse_ballot_w64(acc) ? w : 0.0f;
+                cum += static_cast<double>(wa);
+                has |= acc;
+                if (exm != 0) {
+                    if (nslots >= a.cap || sp + 1 >= kWalkDepth) {
+                        over = true;
+                        break;
+                    }
+                    const uint32_t ns = nslots++;
+                    // the lane that initialised the parent's entry top_cur writes it (same lane, in order)
+                    if (ln == top_cur) {
+                        tchild[static_cast<uint64_t>(top_slot) * 8 + top_cur] = ns;
+                    }
+                    if (ln < 8) {
+                        tchild[static_cast<uint64_t>(ns) * 8 + ln] = kNone;
+                    }
+                    if (ln == 0) {
+                        sh.first[sp] = top_first;
+                        sh.cur[sp] = top_cur;
+                        sh.slot[sp] = top_slot;
+                        sh.mask[sp] = top_mask;
+                    }
+                    sp++;
+                    top_first = mt.y & kLinkMask;
+                    top_cur = mt.y >> kLinkBits;
+                    top_slot = ns;
+                    top_mask = exm;
+                    if (__builtin_amdgcn_inverse_ballot_w64(exm)) {
+                        tests += top_cur;
+                    }
+                    if (sp < kBlockLevels) {
+                        stage_children(sh, G, sp, top_first, top_cur, ln);
+                        __builtin_amdgcn_s_waitcnt(kWaitVm0);
+                    }
+                }
+            }
+        }
+        // the table written by other lanes is read below
+        __threadfence_block();
+        const uint32_t na = active ? s.vac_axon[W.src] : 0U;
+        const uint64_t slot0 = active ? s.slot_off[W.src] : 0U;
+        const uint32_t local = W.src - net_first;
+        const double total = cum;
+        const bool mine = ((has >> ln) & 1ULL) != 0;
+        acc_exec += tests;
+        // sources for the per-lane kernel: the whole group on an overflow, else lists summing to 0
+        const bool to_lane = active && (over || (mine && total == 0.0) || ((handoff >> ln) & 1ULL) != 0);
+        const uint64_t om = __builtin_amdgcn_ballot_w64(to_lane);
+        if (om != 0) {
+            unsigned long long o0 = 0;
+            if (ln == 0) {
+                o0 = atomicAdd(a.n_overflow, static_cast<unsigned long long>(__popcll(om)));
+            }
+            o0 = __shfl(o0, 0, kWave);
+            if (to_lane) {
+                a.overflow[o0 + rank_below(om)] = W.src;
+                s.found[W.src] = kOverflowMark;
+            }
+        }
+        if (active && !to_lane && !mine) {
+            s.slot_tests[slot0] = tests;   // an empty list: the first search fails, slot_target stays kNone
+        }
+        const bool draws = active && !to_lane && mine;
+        const double next1 = draws ? nextafter(total, total + 0.00001) : 0.0;
+        uint32_t maxna = draws ? na : 0U;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            maxna = max(maxna, static_cast<uint32_t>(__shfl_xor(static_cast<int>(maxna), o, kWave)));
+        }
+        maxna = uniform_u32(maxna);
+        uint32_t exec = 0;
+        for (uint32_t j = 0; j < maxna; j++) {
+            bool nav = draws && j < na;
+            const double r = nav ? draw_to_uniform(pick_draw(seed, local, step, j, 1U), next1) : 0.0;
+            uint32_t nfirst = root_first, ncur = root_nc, nslot = 0;
+            double ncum = 0.0;
+            uint32_t picked = kNone;
+            // level-1 pick: top-down through the table
+            for (uint32_t guard = 0; guard < 64U * (kMaxLevels + 2); guard++) {
+                if (__builtin_amdgcn_ballot_w64(nav) == 0) {
+                    break;
+                }
+                if (nav) {
+                    if (ncur == 0) {
+                        nav = false;   // unreachable for a consistent table
+                    } else {
+                        ncur--;
+                        const uint32_t y = nfirst + ncur;
+                        const uint2 mt = G.meta[y];
+                        const uint4 rq = G.rec32[y];
+                        exec++;
+                        const NodeEval e = eval_walk<kMu0>(s, G, W, y, mt, rq, true);
+                        if (e.st == kAccept) {
+                            const double cn = ncum + e.p;
+                            if (reaches(cn, r)) {
+                                picked = y;
+                                nav = false;
+                            }
+                            ncum = cn;
+                        } else if (e.st == kExpand) {
+                            const uint32_t sl = tchild[static_cast<uint64_t>(nslot) * 8 + ncur];
+                            if (sl >= a.cap) {
+                                nav = false;   // unreachable for a consistent table
+                            } else {
+                                const double ce = tcum[static_cast<uint64_t>(sl) * kWave + ln];
+                                if (reaches(ce, r)) {
+                                    nslot = sl;
+                                    nfirst = mt.y & kLinkMask;
+                                    ncur = mt.y >> kLinkBits;
+                                } else {
+                                    ncum = ce;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            // descent below the picked node
+            const bool drew = draws && j < na;
+            uint32_t node = picked, level = 2, dtests = tests;
+            int state = kDescFailed;
+            if (drew && picked != kNone) {
+                state = descend<kMu0>(s, G, W, seed, local, step, j, node, level, dtests, exec);
+            }
+            const uint64_t sm = __builtin_amdgcn_ballot_w64(drew && state == kDescSlow);
+            unsigned long long at0 = 0;
+            if (ln == 0 && sm != 0) {
+                at0 = atomicAdd(a.n_slow, static_cast<unsigned long long>(__popcll(sm)));
+            }
+            at0 = __shfl(at0, 0, kWave);
+            if (drew) {
+                const uint64_t slot = slot0 + j;
+                s.slot_tests[slot] = dtests;
+                if (state == kDescSlow) {
+                    a.slow[at0 + rank_below(sm)] = make_uint4(static_cast<uint32_t>(slot), W.src, node, level);
+                } else {
+                    s.slot_target[slot] = state == kDescDone ? node : kNone;
+                }
+            }
+        }
+        acc_exec += exec;
+    }
+    // executed tests of the wave, summed over the lanes
+    unsigned long long v = acc_exec;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, kWave);
+    }
+    if (ln == 0 && v != 0) {
+        atomicAdd(&s.counters[kCtrTestsExec], v);
+    }
+}
+
+// Per order position i: active source of type t -> flag[t n + i] = 1.
+__global__ void k_walk_flags(uint64_t n, const uint32_t* __restrict__ order, const uint8_t* __restrict__ enabled,
+                             const uint32_t* __restrict__ vac, const uint8_t* __restrict__ signal,
+                             const unsigned long long* __restrict__ counters, uint32_t* __restrict__ flag) {
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        const uint32_t src = order[i];
+        const bool act = counters[kCtrOverCap] == 0 && enabled[src] != 0 && vac[src] != 0;
+        const bool inh = signal[src] != 0;
+        flag[i] = act && !inh ? 1U : 0U;
+        flag[n + i] = act && inh ? 1U : 0U;
+    }
+}
+
+__global__ void k_walk_scatter(uint64_t n, const uint32_t* __restrict__ flag, const uint64_t* __restrict__ off,
+                               const uint32_t* __restrict__ order, uint32_t* __restrict__ list) {
+    const uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (k < 2 * n && flag[k] != 0) {
+        list[off[k]] = order[k < n ? k : k - n];
+    }
+}
+
+// Segments (type, network): first list position, source count and group count.
+__global__ void k_walk_segments(uint64_t n, uint32_t networks, const uint64_t* __restrict__ net_off,
+                                const uint64_t* __restrict__ off, uint64_t* __restrict__ seg_first,
+                                uint64_t* __restrict__ seg_count, uint32_t* __restrict__ seg_groups) {
+    const uint64_t nseg = 2ULL * networks;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t sg = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; sg < nseg; sg += stride) {
+        const uint64_t t = sg / networks, g = sg - t * networks;
+        const uint64_t f = off[t * n + net_off[g]];
+        const uint64_t e = off[t * n + net_off[g + 1]];
+        seg_first[sg] = f;
+        seg_count[sg] = e - f;
+        seg_groups[sg] = static_cast<uint32_t>((e - f + kGroup - 1) / kGroup);
+    }
+}
+
+}  // namespace
+
+// Group-walk buffers of an FP32-mixed context: the per-update lists (sized by n and the networks) and
+// the per-wave tables (sized for the waves one launch can use).
+int ensure_walk(rl_bh_ctx* c) {
+    const uint64_t n = c->n;
+    const uint64_t nseg = 2ULL * std::max<uint32_t>(c->num_networks, 1U);
+    RL_TRY(dalloc(c, &c->walk_flag, 2 * n));
+    RL_TRY(dalloc(c, &c->walk_off, 2 * n + 1));
+    RL_TRY(dalloc(c, &c->walk_list, n));
+    RL_TRY(dalloc(c, &c->walk_seg_groups, nseg));
+    RL_TRY(dalloc(c, &c->walk_seg_goff, nseg + 1));
+    RL_TRY(dalloc(c, &c->walk_seg_first, nseg));
+    RL_TRY(dalloc(c, &c->walk_seg_count, nseg));
+    const uint64_t groups_max = (n + kGroup - 1) / kGroup + nseg;
+    c->walk_blocks_cap = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(c->cus) * kWalkWavesPerCu,
+                                                                  std::max<uint64_t>(groups_max, 1)));
+    c->walk_cap = kWalkCap;
+    const size_t slots = static_cast<size_t>(c->walk_blocks_cap) * c->walk_cap;
+    RL_TRY(dalloc(c, &c->walk_cum, slots * kWave));
+    RL_TRY(dalloc(c, &c->walk_child, slots * 8));
+    return RL_OK;
+}
+
+// Group walk over the active sources: lists, the walk, the per-lane general descents it hands over,
+// then the per-source finalisation.  Sources handed to the per-lane kernel are in c->coop_overflow,
+// counted in counters[kCtrOverflow].  All counts stay on the device.
+int launch_search_walk(rl_bh_ctx* c, const SearchArgs& sa) {
+    hipStream_t s = c->stream;
+    const uint64_t n = c->n;
+    const uint32_t G = std::max<uint32_t>(c->num_networks, 1U);
+    k_walk_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->signal, c->counters,
+                                                 c->walk_flag);
+    RL_HIP(c, hipGetLastError());
+    RL_TRY(scan_counts(c, c->walk_flag, 2 * n, c->walk_off));
+    k_walk_scatter<<<grid_for(2 * n), kBlock, 0, s>>>(n, c->walk_flag, c->walk_off, c->order, c->walk_list);
+    RL_HIP(c, hipGetLastError());
+    k_walk_segments<<<grid_for(2ULL * G), kBlock, 0, s>>>(n, G, c->net_off, c->walk_off, c->walk_seg_first,
+                                                           c->walk_seg_count, c->walk_seg_groups);
+    RL_HIP(c, hipGetLastError());
+    RL_TRY(scan_counts(c, c->walk_seg_groups, 2ULL * G, c->walk_seg_goff));
+    WalkArgs a{};
+    a.s = sa;
+    a.list = c->walk_list;
+    a.seg_goff = c->walk_seg_goff;
+    a.seg_first = c->walk_seg_first;
+    a.seg_count = c->walk_seg_count;
+    a.nseg = 2 * G;
+    a.networks = G;
+    a.cursor = c->coop_cursor;
+    a.chunk = std::max<uint32_t>(c->search_chunk, 1U);
+    a.t_cum = c->walk_cum;
+    a.t_child = c->walk_child;
+    a.cap = c->walk_cap;
+    a.overflow = c->coop_overflow;
+    a.n_overflow = c->counters + kCtrOverflow;
+    a.slow = c->slow_items;
+    a.n_slow = c->counters + kCtrSlow;
+    const unsigned blocks = std::max<uint32_t>(1U, std::min<uint32_t>(c->coop_blocks, c->walk_blocks_cap));
+    const bool timed = !c->capturing;
+    if (timed) RL_HIP(c, hipEventRecord(c->ev2, s));
+    if (c->cfg.mu == 0.0) {
+        k_walk<true><<<blocks, kWave, 0, s>>>(a);
+    } else {
+        k_walk<false><<<blocks, kWave, 0, s>>>(a);
+    }
+    RL_HIP(c, hipGetLastError());
+    if (timed) RL_HIP(c, hipEventRecord(c->ev3, s));
+    RL_TRY(launch_descend_lane(c, sa, c->slow_items, c->counters + kCtrSlow));
+    return launch_finalize(c, sa);
+}
+
+}  // namespace rl
