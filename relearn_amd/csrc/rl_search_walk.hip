@@ -67,19 +67,28 @@ struct WalkArgs {
     unsigned long long* n_slow;
 };
 
-constexpr uint32_t kBlockLevels = 24;            // stack levels whose children are staged in LDS
+constexpr uint32_t kBlockLevels = 24;            // LDS blocks of staged child records (a ring over the levels)
+static_assert(kWalkDepth < 2 * kBlockLevels, "a stack level maps to its block with one subtraction");
 
-// The stack below the top entry, and per stack level the records of the level's children (copied
-// from the tree by the load-to-LDS instructions when the entry is pushed, so that the steps over a
-// node's children read them from LDS instead of waiting for memory one child at a time).
+// The stack below the top entry, and the records of the children of the stack's nodes: each push
+// copies the node's children from the tree into the block of its level (load-to-LDS instructions), so
+// that the steps over a node's children read them from LDS instead of waiting for memory one child at
+// a time.  Levels share blocks modulo kBlockLevels; a block records the level it holds, and a pop to
+// a level whose block was taken by a deeper one copies the children again.
 struct WalkShared {
     uint32_t first[kWalkDepth];
     uint32_t cur[kWalkDepth];
+    uint32_t nch[kWalkDepth];
     uint32_t slot[kWalkDepth];
     uint64_t mask[kWalkDepth];
+    uint32_t owner[kBlockLevels];
     uint4 rec[kBlockLevels][8];
     uint32_t meta[kBlockLevels][16];
 };
+
+__device__ __forceinline__ uint32_t block_of(uint32_t level) {
+    return level >= kBlockLevels ? level - kBlockLevels : level;
+}
 
 
 __device__ __forceinline__ uint32_t lane_id() {
@@ -110,21 +119,29 @@ struct Group {
 
 typedef __attribute__((address_space(3))) void LdsVoid;
 
-// Starts copying the records of the children [first, first + nc) into LDS level `lv` (lanes < nc copy
-// a 16-byte position record each, lanes < 2 nc one word of a count/link record each).
-__device__ __forceinline__ void stage_children(WalkShared& sh, const Group& G, uint32_t lv, uint32_t first,
+// s_waitcnt immediate: VM_CNT 0 (the staged records have landed in LDS), EXP_CNT and LGKM_CNT not waited for
+constexpr int kWaitVm0 = 0x0F70;
+
+// Copies the records of the children [first, first + nc) of the node at stack level `level` into its
+// LDS block (lanes < nc copy a 16-byte position record each, lanes < 2 nc one word of a count/link
+// record each) and waits for them.
+__device__ __forceinline__ void stage_children(WalkShared& sh, const Group& G, uint32_t level, uint32_t first,
                                                uint32_t nc, uint32_t ln) {
+    const uint32_t b = block_of(level);
     if (ln < nc) {
-        __builtin_amdgcn_global_load_lds((void*)(G.rec32 + first + ln), (LdsVoid*)&sh.rec[lv][0], 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((void*)(G.rec32 + first + ln), (LdsVoid*)&sh.rec[b][0], 16, 0, 0);
     }
     if (ln < 2 * nc) {
         __builtin_amdgcn_global_load_lds((void*)(reinterpret_cast<const uint32_t*>(G.meta) + 2 * first + ln),
-                                         (LdsVoid*)&sh.meta[lv][0], 4, 0, 0);
+                                         (LdsVoid*)&sh.meta[b][0], 4, 0, 0);
     }
+    if (ln == 0) {
+        sh.owner[b] = level;
+    }
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
 }
 
-// s_waitcnt immediate: VM_CNT 0 (the staged records have landed in LDS), EXP_CNT and LGKM_CNT not waited for
-constexpr int kWaitVm0 = 0x0F70;
+
 
 // Node records read through the constant address space (scalar loads for wave-uniform indices).
 typedef unsigned int U32x2 __attribute__((ext_vector_type(2)));
@@ -374,18 +391,16 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         const uint32_t root_first = mr.y & kLinkMask;
         const uint32_t root_nc = mr.x == 0 ? 0U : (mr.y >> kLinkBits);
         if (root_nc != 0) {
-            // node records through the constant address space: below the LDS-staged levels the walk's
-            // node index is wave-uniform, so they are scalar loads into scalar registers
-            KMeta* const kmeta = (KMeta*)(G.meta);
-            KRec* const krec = (KRec*)(G.rec32);
-            uint32_t top_first = root_first, top_cur = root_nc, top_slot = 0;
+            uint32_t top_first = root_first, top_cur = root_nc, top_nch = root_nc, top_slot = 0;
             uint64_t top_mask = amask;
             uint32_t nslots = 1, sp = 0;
             if (ln < 8) {
                 tchild[ln] = kNone;
             }
+            if (ln < kBlockLevels) {
+                sh.owner[ln] = kNone;
+            }
             stage_children(sh, G, 0, root_first, root_nc, ln);
-            __builtin_amdgcn_s_waitcnt(kWaitVm0);
             // a lane's level-1 tests: the root's children and the children of every node it expands
             tests = active ? root_nc : 0U;
             while (true) {
@@ -402,21 +417,19 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     const uint64_t mk = sh.mask[sp];
                     top_mask = (static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(mk >> 32U))) << 32U) |
                                uniform_u32(static_cast<uint32_t>(mk));
+                    if (uniform_u32(sh.owner[block_of(sp)]) != sp) {
+                        stage_children(sh, G, sp, top_first, uniform_u32(sh.nch[sp]), ln);
+                    }
                     continue;
                 }
                 top_cur--;
                 const uint32_t y = top_first + top_cur;
-                uint2 mt;
-                uint4 rq;
-                if (sp < kBlockLevels) {
-                    mt = make_uint2(uniform_u32(sh.meta[sp][2 * top_cur]), uniform_u32(sh.meta[sp][2 * top_cur + 1]));
-                    rq = sh.rec[sp][top_cur];
-                } else {
-                    const U32x2 m2 = kmeta[y];
-                    const U32x4 r4 = krec[y];
-                    mt = make_uint2(m2.x, m2.y);
-                    rq = make_uint4(r4.x, r4.y, r4.z, r4.w);
-                }
+                const uint32_t bl = block_of(sp);
+                const uint2 mraw = *reinterpret_cast<const uint2*>(&sh.meta[bl][2 * top_cur]);
+                const uint4 rq = sh.rec[bl][top_cur];
+                // both records are read together, before the count decides anything
+                __asm__ volatile("" ::"v"(mraw.x), "v"(mraw.y), "v"(rq.x), "v"(rq.y), "v"(rq.z), "v"(rq.w));
+                const uint2 mt = make_uint2(uniform_u32(mraw.x), uniform_u32(mraw.y));
                 if (mt.x == 0) {
                     continue;   // no vacant dendrites below: discarded by every lane
                 }
@@ -428,8 +441,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 const float dy = static_cast<float>(iy);
                 const float dz = static_cast<float>(iz);
                 const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                // lane masks straight from the comparisons; combined with scalar mask arithmetic
-                const uint64_t coinc = __builtin_amdgcn_ballot_w64((ix | iy | iz) == 0);
+                // lane masks straight from the comparisons; combined with scalar mask arithmetic.  The
+                // squared distance of integer differences is 0 exactly at a fixed-point coincidence.
+                const uint64_t coinc = __builtin_amdgcn_ballot_w64(d2 == 0.0f);
                 float x;
                 if (kMu0) {
                     x = d2 * G.k2q;
@@ -466,7 +480,8 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     exm |= __builtin_amdgcn_ballot_w64(st == kExpand);
                     handoff |= __builtin_amdgcn_ballot_w64(st == kUndecided);
                 }
-                const float wa = __builtin_amdgcn_inverse_ballot_w64(acc) ? w : 0.0f;
+                float wa = __builtin_amdgcn_inverse_ballot_w64(acc) ? w : 0.0f;
+                __asm__ volatile("" : "+v"(wa));   // select in FP32, then one conversion
                 cum += static_cast<double>(wa);
                 has |= acc;
                 if (exm != 0) {
@@ -485,21 +500,20 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     if (ln == 0) {
                         sh.first[sp] = top_first;
                         sh.cur[sp] = top_cur;
+                        sh.nch[sp] = top_nch;
                         sh.slot[sp] = top_slot;
                         sh.mask[sp] = top_mask;
                     }
                     sp++;
                     top_first = mt.y & kLinkMask;
                     top_cur = mt.y >> kLinkBits;
+                    top_nch = top_cur;
                     top_slot = ns;
                     top_mask = exm;
                     if (__builtin_amdgcn_inverse_ballot_w64(exm)) {
                         tests += top_cur;
                     }
-                    if (sp < kBlockLevels) {
-                        stage_children(sh, G, sp, top_first, top_cur, ln);
-                        __builtin_amdgcn_s_waitcnt(kWaitVm0);
-                    }
+                    stage_children(sh, G, sp, top_first, top_cur, ln);
                 }
             }
         }
