@@ -484,8 +484,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32_mixed", "fp64_plain"], default=None)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 (one lane per source) run")
-    ap.add_argument("--kernel", choices=["coop", "lane"], default="coop",
-                    help="search kernel: wavefront-cooperative (default) or one lane per source in list order")
+    ap.add_argument("--kernel", choices=["default", "wave", "lane"], default="default",
+                    help="search kernel: default (the group walk for FP32-mixed, one wavefront per source for FP64), "
+                         "wave (one wavefront per source for both), lane (one lane per source in list order)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (reference kernel for rocprofv3 byte counters)")
     ap.add_argument("--workload", choices=["single", "ensemble", "sim-ensemble", "sim-single"], default="single",
@@ -519,7 +520,7 @@ def main(argv=None):
     if args.workload.startswith("sim-"):
         return run_sim(args)
 
-    from relearn_amd import RL_FLAG_EXACT_ORDER, RL_PREC_FP32_MIXED, RL_PREC_FP64
+    from relearn_amd import RL_FLAG_EXACT_ORDER, RL_FLAG_WAVE_PER_SOURCE, RL_PREC_FP32_MIXED, RL_PREC_FP64
     from relearn_amd.workloads import uniform_cube
     import torch
 
@@ -532,7 +533,7 @@ def main(argv=None):
     d_vac = [int(a.data_ptr()) for a in d_arrays]
 
     res = {}
-    kflags = RL_FLAG_EXACT_ORDER if args.kernel == "lane" else 0
+    kflags = {"default": 0, "wave": RL_FLAG_WAVE_PER_SOURCE, "lane": RL_FLAG_EXACT_ORDER}[args.kernel]
     # fp64_plain: the FP64 path with one lane per source and the CPU path's list order (the plain
     # FP64 GPU method the FP32-mixed path is compared with)
     precs = [("fp32_mixed", RL_PREC_FP32_MIXED, kflags), ("fp64", RL_PREC_FP64, kflags)]
@@ -555,7 +556,12 @@ def main(argv=None):
     bytes_per_launch = main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO
     kernel_s = 1e-3 * main_r["dominant_kernel_ms"]
     achieved = bytes_per_launch / kernel_s / 1e9
-    kname = "k_level1" if args.kernel == "coop" and main_name != "fp64_plain" else "k_search"
+    if args.kernel == "lane" or main_name == "fp64_plain":
+        kname = "k_search"
+    elif main_name == "fp32_mixed" and args.kernel == "default":
+        kname = "k_walk"
+    else:
+        kname = "k_level1"
     traffic = read_traffic(kname, main_name, args.neurons)
     counters = read_counters(kname, main_name) if args.neurons == 16_000_000 else None
     out = {
