@@ -27,6 +27,7 @@ METRIC = "target searches/s @16M neurons, FP32-mixed vs FP64; ms per plasticity 
 HBM_PEAK_GBS = 8000.0
 B_NODE = {"fp64": 40, "fp32_mixed": 24}   # bytes read per acceptance test (record + count/link)
 B_IO = 32                                  # source position + output ids per search
+B_WALK_SLOT = 64 * 8 + 8 * 4               # group walk: per table slot the 64 running sums + 8 child slots
 # Simulation workloads: Normal(5, 2) input alone drives the Izhikevich neurons to about 0.0117 spikes
 # per step, so calcium settles near 0.117 and never reaches the reference's axon growth threshold
 # 0.4, and with no axons no synapse forms to add input; the "growing" thresholds and target let
@@ -119,6 +120,8 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
         "dominant_kernel_ms": float(np.mean(dominant_ms)),
         "search_phase_searches_per_s": searches / (1e-3 * sum(kernel_ms)),
         "node_tests_per_step": tests / steps,
+        "walk_steps_per_step": sum(r[0]["walk_steps"] for r in recs) / steps,
+        "walk_slots_per_step": sum(r[0]["walk_slots"] for r in recs) / steps,
         "node_tests_ref_per_step": tests_ref / steps,
         "phase_s": [float(np.mean([r[2][i] for r in recs])) for i in range(3)],
         "graph_ms_per_update": float(np.mean(graph_ms)),
@@ -551,9 +554,18 @@ def main(argv=None):
     main_name = "fp32_mixed" if "fp32_mixed" in res else ("fp64" if "fp64" in res else "fp64_plain")
     main_r = res[main_name]
     b_node = B_NODE["fp32_mixed" if main_name == "fp32_mixed" else "fp64"]
-    # algorithmic (logical) bytes of one launch: every acceptance test the kernel executes reads one node
-    # record, every search reads its source and writes its result
-    bytes_per_launch = main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO
+    # algorithmic bytes of one launch.  The wave-per-source and per-lane kernels read one node record for
+    # every acceptance test they execute; the group walk reads one node record per step (shared by the
+    # up to 64 sources of the wavefront) and writes one table slot per expanded node.  Every search
+    # reads its source and writes its result.
+    if main_r["walk_steps_per_step"] > 0:
+        bytes_per_launch = (main_r["walk_steps_per_step"] * b_node + main_r["walk_slots_per_step"] * B_WALK_SLOT
+                            + main_r["searches_per_step"] * B_IO)
+        bytes_model = (f"walk steps x {b_node} B (node record, read once for the wavefront) + table slots x "
+                       f"{B_WALK_SLOT} B + searches x {B_IO} B per launch")
+    else:
+        bytes_per_launch = main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO
+        bytes_model = f"executed acceptance tests x {b_node} B + searches x {B_IO} B per launch"
     kernel_s = 1e-3 * main_r["dominant_kernel_ms"]
     achieved = bytes_per_launch / kernel_s / 1e9
     if args.kernel == "lane" or main_name == "fp64_plain":
@@ -594,11 +606,14 @@ def main(argv=None):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "bytes_model": f"executed acceptance tests x {b_node} B + searches x {B_IO} B per launch (logical "
-                           "bytes: node records are re-read from L2, so this is the kernel's demand, not HBM "
-                           "traffic)",
+            "bytes_model": bytes_model,
             "kernel_ms": main_r["dominant_kernel_ms"],
             "node_tests_per_launch": main_r["node_tests_per_step"],
+            "walk_steps_per_launch": main_r["walk_steps_per_step"],
+            "walk_slots_per_launch": main_r["walk_slots_per_step"],
+            # the same executed tests priced at one node record each (the wave-per-source kernels' demand)
+            "achieved_per_test_records": (main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO)
+            / kernel_s / 1e9,
             "searches_per_launch": main_r["searches_per_step"],
             # the reference's traversal re-derives the level-1 list for every axon: its test count is
             # larger than the kernel's, which derives it once per neuron

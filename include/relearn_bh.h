@@ -83,6 +83,10 @@ typedef struct {
                                    of an FP32 sum of 0, or the n/r fallback of Kernel.h:109-126) */
     uint64_t slow_descents;     /* descents below level 1 handed to the per-lane traversal (expansion or a
                                    Gaussian sum of 0 below the picked node) */
+    uint64_t walk_steps;        /* FP32-mixed group walk: node steps (one node evaluated for the up to 64
+                                   sources of a wavefront) */
+    uint64_t walk_slots;        /* FP32-mixed group walk: expanded nodes that got a slot of the per-source
+                                   subtree sums */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a

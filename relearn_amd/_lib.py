@@ -71,6 +71,8 @@ class StepStats(C.Structure):
         ("net_regrown", C.c_uint64),
         ("lane_sources", C.c_uint64),
         ("slow_descents", C.c_uint64),
+        ("walk_steps", C.c_uint64),
+        ("walk_slots", C.c_uint64),
     ]
 
     def as_dict(self):

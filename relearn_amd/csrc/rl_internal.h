@@ -39,7 +39,9 @@ enum : int {
     kCtrNetNew = 13,     // synapses appended to the network graph by the last commit
     kCtrDeleted = 14,    // synapse units deleted by the last element commit
     kCtrDelShort = 15,   // deletions that found fewer synapses than the element commit asked for
-    kCounters = 16
+    kCtrWalkSteps = 16,  // group walk: node steps (one node for the wavefront's sources)
+    kCtrWalkSlots = 17,  // group walk: expanded nodes given a table slot
+    kCounters = 18
 };
 
 struct Status {
@@ -182,6 +184,8 @@ struct rl_bh_ctx {
     uint64_t last_tests_ref = 0;
     uint64_t last_lane_sources = 0;  // sources handled by the per-lane kernel in the last search
     uint64_t last_slow_descents = 0; // descents handed to the per-lane general traversal
+    uint64_t last_walk_steps = 0;    // group walk: node steps and table slots of the last search
+    uint64_t last_walk_slots = 0;
 
     // persistent scratch, sized at init or with slots_cap (never inside an update)
     uint64_t* wide = nullptr;        // n + 1: offsets of the active sources

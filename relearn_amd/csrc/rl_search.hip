@@ -761,6 +761,8 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     c->last_searches = h[kCtrSearches];
     c->last_slow_descents = h[kCtrSlow];
     c->last_lane_sources = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0 ? h[kCtrRedo] : h[kCtrOverflow];
+    c->last_walk_steps = h[kCtrWalkSteps];
+    c->last_walk_slots = h[kCtrWalkSlots];
     c->searched_valid = true;
     return RL_OK;
 }

@@ -324,6 +324,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
     int seg_x = static_cast<int>(blockIdx.x % kXcds);
     int tried = 0;
     unsigned long long acc_exec = 0;
+    unsigned long long acc_steps = 0, acc_slots = 0;   // wave-uniform
     const uint64_t nw = a.seg_goff[a.nseg];
     const uint32_t step = *s.step;
     const uint32_t chunk = a.chunk;
@@ -387,6 +388,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         uint32_t tests = 0;
         uint64_t has = 0;
         uint64_t handoff = 0;   // sources with an FP64 decision too close to call here
+        uint32_t nslots_used = 0;
         bool over = false;
         const uint32_t root_first = mr.y & kLinkMask;
         const uint32_t root_nc = mr.x == 0 ? 0U : (mr.y >> kLinkBits);
@@ -408,6 +410,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     // the walk leaves the subtree of this slot's node
                     tcum[static_cast<uint64_t>(top_slot) * kWave + ln] = cum;
                     if (sp == 0) {
+                        nslots_used = nslots;
                         break;
                     }
                     sp--;
@@ -423,6 +426,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     continue;
                 }
                 top_cur--;
+                acc_steps++;
                 const uint32_t y = top_first + top_cur;
                 const uint32_t bl = block_of(sp);
                 const uint2 mraw = *reinterpret_cast<const uint2*>(&sh.meta[bl][2 * top_cur]);
@@ -517,6 +521,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 }
             }
         }
+        acc_slots += root_nc != 0 ? nslots_used : 0U;
         // the table written by other lanes is read below
         __threadfence_block();
         const uint32_t na = active ? s.vac_axon[W.src] : 0U;
@@ -630,6 +635,8 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
     }
     if (ln == 0 && v != 0) {
         atomicAdd(&s.counters[kCtrTestsExec], v);
+        atomicAdd(&s.counters[kCtrWalkSteps], acc_steps);
+        atomicAdd(&s.counters[kCtrWalkSlots], acc_slots);
     }
 }
 
