@@ -39,6 +39,28 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def walk_counts(net, precision, seed, d_vac, flags, tuning):
+    """Node steps and table slots of one FP32-mixed group-walk search, from the walk's counting
+    instantiation (RL_FLAG_COUNT_WALK) on a context of its own, outside every timed region: the counts
+    depend on the tree, the positions and the vacancies only, which every step restores.  (0, 0) when
+    this precision and kernel choice do not run the walk."""
+    from relearn_amd import RL_FLAG_COUNT_WALK, RL_PREC_FP32_MIXED, BarnesHutHIP
+
+    if precision != RL_PREC_FP32_MIXED or flags & 3:
+        return 0.0, 0.0
+    g = BarnesHutHIP(net.box_min, net.box_max, theta=0.3, sigma=750.0, seed=seed, precision=precision,
+                     flags=flags | RL_FLAG_COUNT_WALK)
+    g.init(net.xyz, net.signal)
+    if tuning:
+        g.set_tuning(tuning["search_waves_per_cu"], tuning["search_chunk"])
+    g.prepare_update_connectivity_device(*d_vac)
+    g.find_target_neurons(0)
+    st = g.last_search_stats()
+    g.close()
+    log(f"group walk: {st['walk_steps']} node steps, {st['walk_slots']} table slots per search")
+    return float(st["walk_steps"]), float(st["walk_slots"])
+
+
 def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     from relearn_amd import BarnesHutHIP
 
@@ -108,6 +130,7 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     sample = np.random.default_rng(seed + 7).choice(net.n, min(1000, net.n), replace=False)
     cand = g.candidate_lists(sample, capacity=16384)
     g.close()
+    walk_steps, walk_slots = walk_counts(net, precision, seed, d_vac, flags, tuning)
     return {
         "build_s": build_s,
         "nodes": m,
@@ -120,8 +143,8 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
         "dominant_kernel_ms": float(np.mean(dominant_ms)),
         "search_phase_searches_per_s": searches / (1e-3 * sum(kernel_ms)),
         "node_tests_per_step": tests / steps,
-        "walk_steps_per_step": sum(r[0]["walk_steps"] for r in recs) / steps,
-        "walk_slots_per_step": sum(r[0]["walk_slots"] for r in recs) / steps,
+        "walk_steps_per_step": walk_steps,
+        "walk_slots_per_step": walk_slots,
         "node_tests_ref_per_step": tests_ref / steps,
         "phase_s": [float(np.mean([r[2][i] for r in recs])) for i in range(3)],
         "graph_ms_per_update": float(np.mean(graph_ms)),

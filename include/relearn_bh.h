@@ -41,6 +41,8 @@ enum {
     RL_FLAG_EXACT_ORDER = 1,      /* per-source search with sums in the CPU path's list order (slower) */
     RL_FLAG_WAVE_PER_SOURCE = 2,  /* FP32-mixed: one wavefront per source (the FP64 search) instead of the
                                      group walk (one lane per source, 64 neighbouring sources per wavefront) */
+    RL_FLAG_COUNT_WALK = 4,       /* FP32-mixed group walk: count its node steps and table slots into
+                                     rl_bh_step_stats (a separate, slower instantiation of the walk) */
 };
 
 enum {
@@ -83,10 +85,10 @@ typedef struct {
                                    of an FP32 sum of 0, or the n/r fallback of Kernel.h:109-126) */
     uint64_t slow_descents;     /* descents below level 1 handed to the per-lane traversal (expansion or a
                                    Gaussian sum of 0 below the picked node) */
-    uint64_t walk_steps;        /* FP32-mixed group walk: node steps (one node evaluated for the up to 64
-                                   sources of a wavefront) */
-    uint64_t walk_slots;        /* FP32-mixed group walk: expanded nodes that got a slot of the per-source
-                                   subtree sums */
+    uint64_t walk_steps;        /* FP32-mixed group walk with RL_FLAG_COUNT_WALK: node steps (one node evaluated
+                                   for the up to 64 sources of a wavefront); 0 without the flag */
+    uint64_t walk_slots;        /* FP32-mixed group walk with RL_FLAG_COUNT_WALK: expanded nodes that got a slot
+                                   of the per-source subtree sums; 0 without the flag */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a

@@ -18,6 +18,7 @@ RL_PREC_FP64 = 0
 RL_PREC_FP32_MIXED = 1
 RL_FLAG_EXACT_ORDER = 1
 RL_FLAG_WAVE_PER_SOURCE = 2
+RL_FLAG_COUNT_WALK = 4
 
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)

@@ -169,6 +169,10 @@ int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
         set_global_error("rl_bh_create: unknown precision");
         return RL_EINVAL;
     }
+    if ((cfg->flags & ~(RL_FLAG_EXACT_ORDER | RL_FLAG_WAVE_PER_SOURCE | RL_FLAG_COUNT_WALK)) != 0) {
+        set_global_error("rl_bh_create: unknown flag bits");
+        return RL_EINVAL;
+    }
     const double big = std::numeric_limits<double>::max() / 4.0;
     for (int d = 0; d < 3; d++) {
         if (!(cfg->box_min[d] <= cfg->box_max[d]) || !(std::fabs(cfg->box_min[d]) < big) ||

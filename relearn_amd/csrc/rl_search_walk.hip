@@ -313,9 +313,11 @@ __device__ int descend(const SearchArgs& a, const Group& G, const WalkLane& W, u
     return kDescFailed;
 }
 
-template <bool kMu0>
+template <bool kMu0, bool kCount>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) k_walk(WalkArgs a) {
     __shared__ WalkShared sh;
+    // kCount: child records the wave examined (children of the roots and of expanded nodes), table slots used
+    __shared__ unsigned long long cnt_steps, cnt_slots;
     const SearchArgs& s = a.s;
     const uint32_t ln = lane_id();
     const uint64_t cbase = static_cast<uint64_t>(blockIdx.x) * a.cap;   // first table slot of this wave
@@ -324,10 +326,13 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
     int seg_x = static_cast<int>(blockIdx.x % kXcds);
     int tried = 0;
     unsigned long long acc_exec = 0;
-    unsigned long long acc_steps = 0, acc_slots = 0;   // wave-uniform
     const uint64_t nw = a.seg_goff[a.nseg];
     const uint32_t step = *s.step;
     const uint32_t chunk = a.chunk;
+    if (kCount && ln == 0) {
+        cnt_steps = 0;
+        cnt_slots = 0;
+    }
     uint64_t w_next = 0, w_lim = 0;
     while (true) {
         if (w_next >= w_lim) {
@@ -388,7 +393,6 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         uint32_t tests = 0;
         uint64_t has = 0;
         uint64_t handoff = 0;   // sources with an FP64 decision too close to call here
-        uint32_t nslots_used = 0;
         bool over = false;
         const uint32_t root_first = mr.y & kLinkMask;
         const uint32_t root_nc = mr.x == 0 ? 0U : (mr.y >> kLinkBits);
@@ -405,12 +409,17 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
             stage_children(sh, G, 0, root_first, root_nc, ln);
             // a lane's level-1 tests: the root's children and the children of every node it expands
             tests = active ? root_nc : 0U;
+            if (kCount && ln == 0) {
+                atomicAdd(&cnt_steps, static_cast<unsigned long long>(root_nc));
+            }
             while (true) {
                 if (top_cur == 0) {
                     // the walk leaves the subtree of this slot's node
                     tcum[static_cast<uint64_t>(top_slot) * kWave + ln] = cum;
                     if (sp == 0) {
-                        nslots_used = nslots;
+                        if (kCount && ln == 0) {
+                            atomicAdd(&cnt_slots, static_cast<unsigned long long>(nslots));
+                        }
                         break;
                     }
                     sp--;
@@ -426,7 +435,6 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     continue;
                 }
                 top_cur--;
-                acc_steps++;
                 const uint32_t y = top_first + top_cur;
                 const uint32_t bl = block_of(sp);
                 const uint2 mraw = *reinterpret_cast<const uint2*>(&sh.meta[bl][2 * top_cur]);
@@ -517,11 +525,13 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     if (__builtin_amdgcn_inverse_ballot_w64(exm)) {
                         tests += top_cur;
                     }
+                    if (kCount && ln == 0) {
+                        atomicAdd(&cnt_steps, static_cast<unsigned long long>(top_cur));
+                    }
                     stage_children(sh, G, sp, top_first, top_cur, ln);
                 }
             }
         }
-        acc_slots += root_nc != 0 ? nslots_used : 0U;
         // the table written by other lanes is read below
         __threadfence_block();
         const uint32_t na = active ? s.vac_axon[W.src] : 0U;
@@ -635,8 +645,10 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
     }
     if (ln == 0 && v != 0) {
         atomicAdd(&s.counters[kCtrTestsExec], v);
-        atomicAdd(&s.counters[kCtrWalkSteps], acc_steps);
-        atomicAdd(&s.counters[kCtrWalkSlots], acc_slots);
+        if (kCount) {
+            atomicAdd(&s.counters[kCtrWalkSteps], cnt_steps);
+            atomicAdd(&s.counters[kCtrWalkSlots], cnt_slots);
+        }
     }
 }
 
@@ -739,10 +751,20 @@ int launch_search_walk(rl_bh_ctx* c, const SearchArgs& sa) {
     const unsigned blocks = std::max<uint32_t>(1U, std::min<uint32_t>(c->coop_blocks, c->walk_blocks_cap));
     const bool timed = !c->capturing;
     if (timed) RL_HIP(c, hipEventRecord(c->ev2, s));
+    // the counting instantiation is a separate kernel: the counters change the register allocation of the walk
+    const bool count = (c->cfg.flags & RL_FLAG_COUNT_WALK) != 0;
     if (c->cfg.mu == 0.0) {
-        k_walk<true><<<blocks, kWave, 0, s>>>(a);
+        if (count) {
+            k_walk<true, true><<<blocks, kWave, 0, s>>>(a);
+        } else {
+            k_walk<true, false><<<blocks, kWave, 0, s>>>(a);
+        }
     } else {
-        k_walk<false><<<blocks, kWave, 0, s>>>(a);
+        if (count) {
+            k_walk<false, true><<<blocks, kWave, 0, s>>>(a);
+        } else {
+            k_walk<false, false><<<blocks, kWave, 0, s>>>(a);
+        }
     }
     RL_HIP(c, hipGetLastError());
     if (timed) RL_HIP(c, hipEventRecord(c->ev3, s));

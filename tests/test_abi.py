@@ -81,3 +81,17 @@ def test_create_rejects_bad_config_without_gpu(lib):
     lib.rl_bh_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
     out = ctypes.c_void_p()
     assert lib.rl_bh_create(None, ctypes.byref(out)) != 0
+    lib.rl_bh_last_error.restype = ctypes.c_char_p
+    lib.rl_bh_last_error.argtypes = [ctypes.c_void_p]
+    cfg = _lib.Config()
+    cfg.box_min[:] = (0.0, 0.0, 0.0)
+    cfg.box_max[:] = (1.0, 1.0, 1.0)
+    cfg.theta, cfg.sigma, cfg.mu = 0.3, 750.0, 0.0
+    cfg.precision = _lib.RL_PREC_FP32_MIXED
+    cfg.flags = 8
+    assert lib.rl_bh_create(ctypes.byref(cfg), ctypes.byref(out)) == -1
+    assert b"flag" in lib.rl_bh_last_error(None)
+    cfg.flags = _lib.RL_FLAG_COUNT_WALK
+    cfg.theta = 0.7
+    assert lib.rl_bh_create(ctypes.byref(cfg), ctypes.byref(out)) == -1
+    assert b"theta" in lib.rl_bh_last_error(None)

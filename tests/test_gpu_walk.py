@@ -19,7 +19,7 @@ import os
 import numpy as np
 import pytest
 
-from relearn_amd import RL_FLAG_WAVE_PER_SOURCE, RL_PREC_FP32_MIXED, RL_PREC_FP64, BarnesHutHIP
+from relearn_amd import RL_FLAG_COUNT_WALK, RL_FLAG_WAVE_PER_SOURCE, RL_PREC_FP32_MIXED, RL_PREC_FP64, BarnesHutHIP
 from relearn_amd.workloads import Network, uniform_cube
 from tests.test_gpu_parity import _check_search
 
@@ -115,3 +115,23 @@ def test_walk_gaussian_against_fp64_and_wave_per_source(oracle):
     assert dw <= 1e-4
     both = tw == tgt
     assert np.array_equal(testw[both], tests[both])
+
+
+def test_walk_counting_instantiation_same_picks(oracle):
+    # RL_FLAG_COUNT_WALK runs a separate instantiation of the walk that also counts its node steps and
+    # table slots: the picks and test counts are the plain walk's, each step tests the node for at
+    # least one lane and for at most 64, and every expanded node takes one slot
+    net = uniform_cube(30_000, 10_000.0 * (0.03 ** (1 / 3)), 17)
+    _say("walk, counting instantiation, 30000 neurons")
+    ref, nreq, st, tgt, tests, cnt = _pair(oracle, net, seed=8, step=3, sigma=750.0)
+    _, nreq_c, st_c, tgt_c, tests_c, cnt_c = _pair(oracle, net, seed=8, step=3, sigma=750.0, flags=RL_FLAG_COUNT_WALK)
+    assert st["walk_steps"] == 0 and st["walk_slots"] == 0
+    assert np.array_equal(tgt, tgt_c) and np.array_equal(tests, tests_c) and np.array_equal(cnt, cnt_c)
+    assert nreq == nreq_c
+    _say("  steps", st_c["walk_steps"], "slots", st_c["walk_slots"], "lane sources", st_c["lane_sources"])
+    assert st_c["lane_sources"] == 0
+    # a lane's level-1 tests are the walk steps its mask took part in (1 to 64 lanes per step); the
+    # descents below the picked level-1 node add at most 8 tests per level
+    assert st_c["walk_steps"] > 0 and st_c["walk_slots"] > 0
+    assert st_c["walk_slots"] <= st_c["walk_steps"]
+    assert st_c["walk_steps"] <= st_c["node_tests"] <= 64 * st_c["walk_steps"] + 8 * 32 * st_c["searches"]
