@@ -342,7 +342,8 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 w0 = nw * static_cast<uint64_t>(seg_x) / kXcds +
                      atomicAdd(&a.cursor[seg_x], static_cast<unsigned long long>(chunk));
             }
-            w0 = __shfl(w0, 0, kWave);
+            w0 = (static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(w0 >> 32U))) << 32U) |
+                 uniform_u32(static_cast<uint32_t>(w0));
             if (w0 >= seg_end) {
                 if (++tried >= kXcds) {
                     break;
@@ -388,7 +389,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         const uint64_t amask = __builtin_amdgcn_ballot_w64(active);
 
         // ---- the union walk: level-1 sums of all lanes ----
-        const uint2 mr = G.meta[net];
+        // wave-uniform: the root values seed the traversal state, which must stay in scalar registers
+        const uint2 mr0 = G.meta[net];
+        const uint2 mr = make_uint2(uniform_u32(mr0.x), uniform_u32(mr0.y));
         double cum = 0.0;
         uint32_t tests = 0;
         uint64_t has = 0;
@@ -501,7 +504,8 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                         over = true;
                         break;
                     }
-                    const uint32_t ns = nslots++;
+                    const uint32_t ns = nslots;
+                    nslots = uniform_u32(ns + 1);
                     // the lane that initialised the parent's entry top_cur writes it (same lane, in order)
                     if (ln == top_cur) {
                         tchild[static_cast<uint64_t>(top_slot) * 8 + top_cur] = ns;
@@ -517,10 +521,10 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                         sh.mask[sp] = top_mask;
                     }
                     sp++;
-                    top_first = mt.y & kLinkMask;
+                    top_first = uniform_u32(mt.y & kLinkMask);
                     top_cur = mt.y >> kLinkBits;
-                    top_nch = top_cur;
-                    top_slot = ns;
+                    top_nch = uniform_u32(top_cur);
+                    top_slot = uniform_u32(ns);
                     top_mask = exm;
                     if (__builtin_amdgcn_inverse_ballot_w64(exm)) {
                         tests += top_cur;
