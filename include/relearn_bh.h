@@ -365,7 +365,7 @@ int rl_bh_graph_destroy(rl_bh_ctx* ctx);
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): the conflict resolution and
  * vacancy commit of rl_bh_commit in counter order, context-free, on caller device buffers.  It calls
- * no hipMalloc/hipFree: its grouping scratch is a static device array of the library (about 26 MiB per
+ * no hipMalloc/hipFree: its grouping scratch is a static device array of the library (about 30 MiB per
  * GPU, held until the process ends), which limits a call to n <= 262144 neurons and m <= 1048576
  * requests (RL_ELIMIT beyond).  Request i is (d_target[i], d_source[i], d_signal[i], d_axon_ordinal[i]),
  * ids 0..n-1 of one network whose draw-stream seed is network_seed (rl_bh_network_seed(seed, 0) for a

@@ -48,7 +48,7 @@ __global__ void k_commit_count(uint64_t n, const uint64_t* __restrict__ n_req, c
                                const uint32_t* __restrict__ rj, const uint32_t* __restrict__ net_of,
                                const uint64_t* __restrict__ net_off, const uint64_t* __restrict__ net_seed,
                                const uint32_t* __restrict__ step, int counter_order, uint64_t* __restrict__ prio,
-                               uint32_t* __restrict__ gcnt, uint8_t* __restrict__ ok) {
+                               uint32_t* __restrict__ gcnt, uint32_t* __restrict__ pos, uint8_t* __restrict__ ok) {
     const uint64_t m = *n_req;
     const uint32_t st = *step;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -62,23 +62,24 @@ __global__ void k_commit_count(uint64_t n, const uint64_t* __restrict__ n_req, c
         if (s == t || t >= n || s >= n) {   // self request, or an id outside the neurons: fails
             ok[i] = 0;
         } else {
-            atomicAdd(&gcnt[2 * static_cast<uint64_t>(t) + (sig[i] & 1U)], 1U);
+            // the count before this request is its position inside the group
+            pos[i] = atomicAdd(&gcnt[2 * static_cast<uint64_t>(t) + (sig[i] & 1U)], 1U);
         }
     }
 }
 
 // Step 2, after the prefix sum goff of the group sizes.  A request of a group that has no more
-// requests than vacancies is accepted here.  The others fill their group from its end; the order
-// inside a group is arbitrary (step 3 compares priorities, not positions).  Positions of accepted
-// groups keep the sentinel kUngrouped.
+// requests than vacancies is accepted here.  The others go to their group's offset plus the position
+// step 1 gave them (an arbitrary order inside the group: step 3 compares priorities, not positions).
+// Positions of accepted groups keep the sentinel kUngrouped.
 constexpr uint32_t kUngrouped = 0xFFFFFFFFU;
 
 __global__ void k_commit_scatter(uint64_t n, const uint64_t* __restrict__ n_req, const uint32_t* __restrict__ tgt,
                                  const uint32_t* __restrict__ src, const uint8_t* __restrict__ sig,
                                  const uint64_t* __restrict__ prio, const uint64_t* __restrict__ goff,
                                  const uint32_t* __restrict__ vexc, const uint32_t* __restrict__ vinh,
-                                 uint32_t* __restrict__ gcnt, uint64_t* __restrict__ sprio, uint32_t* __restrict__ sidx,
-                                 uint8_t* __restrict__ ok) {
+                                 const uint32_t* __restrict__ pos, uint64_t* __restrict__ sprio,
+                                 uint32_t* __restrict__ sidx, uint8_t* __restrict__ ok) {
     const uint64_t m = *n_req;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
     for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < m; i += stride) {
@@ -94,7 +95,7 @@ __global__ void k_commit_scatter(uint64_t n, const uint64_t* __restrict__ n_req,
             ok[i] = 1;
             continue;
         }
-        const uint64_t q = goff[g] + atomicSub(&gcnt[g], 1U) - 1U;
+        const uint64_t q = goff[g] + pos[i];
         sprio[q] = prio[i];
         sidx[q] = static_cast<uint32_t>(i);
     }
@@ -214,14 +215,16 @@ int enqueue_commit(rl_bh_ctx* c, const uint64_t* h_order) {
         RL_HIP(c, hipGetLastError());
     }
     const uint64_t* n_req = c->req_off + n;
+    // position of each request inside its group: the caller's order is no longer needed once ranked
+    uint32_t* gpos = reinterpret_cast<uint32_t*>(c->c_order);
     k_commit_count<<<kStreamBlocks, kBlock, 0, s>>>(n, n_req, c->req_target, c->req_source, c->req_signal, c->req_j,
                                                     c->net_of, c->net_off, c->net_seed, c->d_steps + 1,
-                                                    h_order == nullptr ? 1 : 0, c->c_prio, c->c_gcnt, c->req_ok);
+                                                    h_order == nullptr ? 1 : 0, c->c_prio, c->c_gcnt, gpos, c->req_ok);
     RL_HIP(c, hipGetLastError());
     RL_TRY(scan_counts(c, c->c_gcnt, groups, c->c_goff));
     RL_TRY(fill_u32(c, c->c_sidx, kUngrouped, 0, c->c_goff + groups, c->slots_cap));
     k_commit_scatter<<<kStreamBlocks, kBlock, 0, s>>>(n, n_req, c->req_target, c->req_source, c->req_signal, c->c_prio,
-                                                      c->c_goff, c->vac[1], c->vac[2], c->c_gcnt, c->c_sprio, c->c_sidx,
+                                                      c->c_goff, c->vac[1], c->vac[2], gpos, c->c_sprio, c->c_sidx,
                                                       c->req_ok);
     RL_HIP(c, hipGetLastError());
     k_commit_rank<<<kStreamBlocks, kBlock, 0, s>>>(c->c_goff + groups, c->c_goff, c->c_sprio, c->c_sidx,
@@ -291,7 +294,8 @@ constexpr size_t kNaPart = kNaGoff + align256((2 * kNoallocMaxNeurons + 1) * siz
 constexpr size_t kNaPrio = kNaPart + align256(((2 * kNoallocMaxNeurons + 4095) / 4096 + 1) * sizeof(uint64_t));
 constexpr size_t kNaSprio = kNaPrio + align256(kNoallocMaxRequests * sizeof(uint64_t));
 constexpr size_t kNaSidx = kNaSprio + align256(kNoallocMaxRequests * sizeof(uint64_t));
-constexpr size_t kNaBytes = kNaSidx + align256(kNoallocMaxRequests * sizeof(uint32_t));
+constexpr size_t kNaPos = kNaSidx + align256(kNoallocMaxRequests * sizeof(uint32_t));
+constexpr size_t kNaBytes = kNaPos + align256(kNoallocMaxRequests * sizeof(uint32_t));
 
 struct NoallocHeader {
     uint64_t m;                    // requests of the call (the kernels read the count from here)
@@ -410,19 +414,20 @@ extern "C" int rl_bh_commit_requests_noalloc(uint64_t n, uint64_t m, const uint3
     auto* prio = reinterpret_cast<uint64_t*>(base + kNaPrio);
     auto* sprio = reinterpret_cast<uint64_t*>(base + kNaSprio);
     auto* sidx = reinterpret_cast<uint32_t*>(base + kNaSidx);
+    auto* gpos = reinterpret_cast<uint32_t*>(base + kNaPos);
     const uint64_t groups = 2 * n;
     // the array keeps the last call's contents: everything read before it is written is set here
     k_noalloc_setup<<<1, 64, 0, s>>>(hdr, m, network_seed, step);
     RL_NA_HIP(hipGetLastError());
     RL_NA_HIP(fill_u32_on(s, gcnt, 0U, groups, nullptr, 0));
     k_commit_count<<<kStreamBlocks, kBlock, 0, s>>>(n, &hdr->m, d_target, d_source, d_signal, d_axon_ordinal, nullptr,
-                                                    &hdr->net_off, &hdr->seed, &hdr->step, 1, prio, gcnt, d_ok);
+                                                    &hdr->net_off, &hdr->seed, &hdr->step, 1, prio, gcnt, gpos, d_ok);
     RL_NA_HIP(hipGetLastError());
     RL_NA_HIP(scan_counts_on(s, gcnt, groups, goff, part));
     RL_NA_HIP(fill_u32_on(s, sidx, kUngrouped, 0, goff + groups, kNoallocMaxRequests));
     k_commit_scatter<<<kStreamBlocks, kBlock, 0, s>>>(n, &hdr->m, d_target, d_source, d_signal, prio, goff,
                                                       d_vacant_excitatory_dendrites, d_vacant_inhibitory_dendrites,
-                                                      gcnt, sprio, sidx, d_ok);
+                                                      gpos, sprio, sidx, d_ok);
     RL_NA_HIP(hipGetLastError());
     k_commit_rank<<<kStreamBlocks, kBlock, 0, s>>>(goff + groups, goff, sprio, sidx, d_target, d_signal,
                                                    d_vacant_excitatory_dendrites, d_vacant_inhibitory_dendrites, d_ok);

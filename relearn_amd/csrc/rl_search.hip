@@ -666,17 +666,26 @@ int enqueue_search(rl_bh_ctx* c) {
     k_search_reset<<<kStreamBlocks, kBlock, 0, s>>>(n, c->slot_off + n, c->slots_cap, c->found, c->searched,
                                                     c->slot_target, c->slot_tests, c->counters, c->coop_cursor);
     RL_HIP(c, hipGetLastError());
-    k_active_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->counters, c->coop_flag);
-    RL_HIP(c, hipGetLastError());
-    RL_TRY(scan_counts(c, c->coop_flag, n, c->wide));
-    k_scatter_active<<<grid_for(n), kBlock, 0, s>>>(n, c->coop_flag, c->wide, c->order, c->coop_work);
-    RL_HIP(c, hipGetLastError());
+    const bool exact = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0;
+    // the group walk builds its own source lists (per dendrite type); the other kernels take the
+    // active sources in Morton order
+    const bool walk = !exact && uses_walk(c);
+    const uint64_t* n_active = c->wide + n;
+    if (walk) {
+        n_active = c->walk_off + 2 * n;
+    } else {
+        k_active_flags<<<grid_for(n), kBlock, 0, s>>>(n, c->order, c->enabled, c->vac[0], c->counters,
+                                                       c->coop_flag);
+        RL_HIP(c, hipGetLastError());
+        RL_TRY(scan_counts(c, c->coop_flag, n, c->wide));
+        k_scatter_active<<<grid_for(n), kBlock, 0, s>>>(n, c->coop_flag, c->wide, c->order, c->coop_work);
+        RL_HIP(c, hipGetLastError());
+    }
 
     SearchArgs a = search_args(c);
     a.order = c->coop_work;
     a.nwork = c->wide + n;
     const size_t lds = static_cast<size_t>(std::max<uint32_t>(c->tree.depth, 1U)) * kBlock * sizeof(uint32_t);
-    const bool exact = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0;
     const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
     const bool timed = !c->capturing;
     if (exact) {
@@ -692,7 +701,7 @@ int enqueue_search(rl_bh_ctx* c) {
         a.order = c->redo;
         a.nwork = reinterpret_cast<const uint64_t*>(c->counters + kCtrRedo);
     } else {
-        if (uses_walk(c)) {
+        if (walk) {
             RL_TRY(launch_search_walk(c, a));
         } else {
             RL_TRY(launch_search_coop(c, a));
@@ -711,7 +720,7 @@ int enqueue_search(rl_bh_ctx* c) {
     k_emit_requests<<<grid_for(n), kBlock, 0, s>>>(n, c->found, c->req_off, c->slot_off, c->slot_target, c->signal,
                                                     c->req_target, c->req_source, c->req_signal, c->req_j);
     RL_HIP(c, hipGetLastError());
-    k_collect<<<1, 64, 0, s>>>(c->counters, c->slot_off + n, c->req_off + n, c->wide + n);
+    k_collect<<<1, 64, 0, s>>>(c->counters, c->slot_off + n, c->req_off + n, n_active);
     RL_HIP(c, hipGetLastError());
     return RL_OK;
 }

@@ -382,12 +382,17 @@ __device__ __forceinline__ uint32_t quant_side2(double side, double scale) {
     return __float_as_uint(static_cast<float>(sq * sq));
 }
 
+// One thread per (node, dendrite type) of a level: the type's count and FP64 centroid of an inner node
+// from its children (children loaded together, then summed in child order 0..7), clamped to the cell
+// box, and its fixed-point record.
 __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __restrict__ nbox,
                                double4* __restrict__ rec0, double4* __restrict__ rec1, uint2* __restrict__ meta0,
                                uint2* __restrict__ meta1, uint4* __restrict__ q0, uint4* __restrict__ q1,
                                const uint32_t* __restrict__ node_net, QuantArgs qa) {
 #pragma clang fp contract(off)
-    const uint64_t i = first + blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t i = first + (t >> 1U);
+    const int s = static_cast<int>(t & 1U);
     if (i >= last) {
         return;
     }
@@ -397,49 +402,56 @@ __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __re
         return;  // leaves keep the neuron position (Cell::set_neuron_position)
     }
     const uint32_t fc = packed & kLinkMask;
-    for (int s = 0; s < 2; s++) {
-        double4* rec = s == 0 ? rec0 : rec1;
-        uint2* meta = s == 0 ? meta0 : meta1;
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        uint32_t total = 0;
-        for (uint32_t k = 0; k < nc; k++) {
-            const uint32_t c = fc + k;
-            const uint32_t cn = meta[c].x;
-            total += cn;
-            if (cn != 0) {
-                const double4 cp = rec[c];
-                const double w = static_cast<double>(cn);
-                ax += cp.x * w;
-                ay += cp.y * w;
-                az += cp.z * w;
+    double4* rec = s == 0 ? rec0 : rec1;
+    uint2* meta = s == 0 ? meta0 : meta1;
+    uint32_t cn[8];
+    double4 cp[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+        if (k < nc) {
+            cn[k] = meta[fc + k].x;
+            cp[k] = rec[fc + k];
+        }
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; k++) {
+        if (k < nc) {
+            total += cn[k];
+            if (cn[k] != 0) {
+                const double w = static_cast<double>(cn[k]);
+                ax += cp[k].x * w;
+                ay += cp[k].y * w;
+                az += cp[k].z * w;
             }
         }
-        meta[i].x = total;
-        double4 r = rec[i];
-        if (total != 0) {
-            const double w = static_cast<double>(total);
-            double qx = ax / w, qy = ay / w, qz = az / w;
-            const double* b = nbox + 6 * i;
-            qx = qx < b[0] ? b[0] : (b[3] < qx ? b[3] : qx);
-            qy = qy < b[1] ? b[1] : (b[4] < qy ? b[4] : qy);
-            qz = qz < b[2] ? b[2] : (b[5] < qz ? b[5] : qz);
-            r.x = qx;
-            r.y = qy;
-            r.z = qz;
-        } else {
-            r.x = 0.0;
-            r.y = 0.0;
-            r.z = 0.0;
-        }
-        rec[i] = r;
-        if (qa.enabled) {
-            uint4* q = s == 0 ? q0 : q1;
-            const uint32_t g = node_net[i];
-            const double* lo = qa.net_box + 6 * static_cast<uint64_t>(g);
-            const double sc = qa.net_scale[g];
-            q[i] = make_uint4(quantize(r.x, lo[0], sc), quantize(r.y, lo[1], sc), quantize(r.z, lo[2], sc),
-                              quant_side2(r.w, sc));
-        }
+    }
+    meta[i].x = total;
+    double4 r = rec[i];
+    if (total != 0) {
+        const double w = static_cast<double>(total);
+        double qx = ax / w, qy = ay / w, qz = az / w;
+        const double* b = nbox + 6 * i;
+        qx = qx < b[0] ? b[0] : (b[3] < qx ? b[3] : qx);
+        qy = qy < b[1] ? b[1] : (b[4] < qy ? b[4] : qy);
+        qz = qz < b[2] ? b[2] : (b[5] < qz ? b[5] : qz);
+        r.x = qx;
+        r.y = qy;
+        r.z = qz;
+    } else {
+        r.x = 0.0;
+        r.y = 0.0;
+        r.z = 0.0;
+    }
+    rec[i] = r;
+    if (qa.enabled) {
+        uint4* q = s == 0 ? q0 : q1;
+        const uint32_t g = node_net[i];
+        const double* lo = qa.net_box + 6 * static_cast<uint64_t>(g);
+        const double sc = qa.net_scale[g];
+        q[i] = make_uint4(quantize(r.x, lo[0], sc), quantize(r.y, lo[1], sc), quantize(r.z, lo[2], sc),
+                          quant_side2(r.w, sc));
     }
 }
 
@@ -756,8 +768,9 @@ int prepare_tree(rl_bh_ctx* c) {
         if (last <= first) {
             continue;
         }
-        k_upward_level<<<grid_for(last - first), kBlock, 0, s>>>(first, last, t.box, t.rec64[0], t.rec64[1], t.meta[0],
-                                                                  t.meta[1], t.rec32[0], t.rec32[1], t.node_net, qa);
+        k_upward_level<<<grid_for(2 * (last - first)), kBlock, 0, s>>>(first, last, t.box, t.rec64[0], t.rec64[1],
+                                                                        t.meta[0], t.meta[1], t.rec32[0], t.rec32[1],
+                                                                        t.node_net, qa);
         RL_HIP(c, hipGetLastError());
     }
     return RL_OK;
