@@ -631,6 +631,10 @@ SearchArgs search_args(rl_bh_ctx* c) {
     a.net_k2q = c->net_k2q;
     a.net_invq = c->net_invq;
     a.guard = 2.0e-5f;
+    {
+        const double t2 = c->cfg.theta * c->cfg.theta;
+        a.guard_abs = static_cast<float>(12.0 * t2 / 2.0e-5 + 6.0 * t2);
+    }
     a.seed = c->cfg.seed;
     a.step = c->d_steps + 1;
     a.slot_target = c->slot_target;

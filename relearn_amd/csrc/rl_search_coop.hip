@@ -177,7 +177,7 @@ __device__ __forceinline__ int eval_node(const SearchArgs& a, const Lane& L, con
         const float s2 = __uint_as_float(d.r32.w);
         const float g = __builtin_fmaf(a.theta2f, d2, -s2);
         const bool coinc = (ix | iy | iz) == 0;
-        const bool near = !leaf && fabsf(g) <= a.guard * s2;
+        const bool near = !leaf && fabsf(g) <= __builtin_fmaf(a.guard, s2, a.guard_abs);
         int st = (leaf || g > 0.0f) ? kAccept : kExpand;
         float x;
         if (kMu0) {

@@ -30,6 +30,12 @@ struct SearchArgs {
     float k2f;           // -log2(e) / sigma^2 (micrometres)
     float muf;
     float guard;         // relative guard band of the FP32 acceptance test
+    // absolute part of the band, in squared fixed-point units.  Rounding the source and the centroid
+    // to the grid moves theta^2 d^2 by at most theta^2 (2 sqrt(3) d + 3) <= (guard/2)(|g| + s2) +
+    // guard_abs/2, with g = theta^2 d^2 - s2 and guard_abs = 12 theta^2 / guard + 6 theta^2 (from
+    // 2 sqrt(3) theta^2 d <= (guard/2) theta^2 d^2 + 6 theta^2 / guard), so outside
+    // |g| <= guard s2 + guard_abs the FP32 sign of g is the exact one, for cells of any size.
+    float guard_abs;
     // networks: network of each neuron, first neuron of each network, and per network the draw seed
     // and the fixed-point constants (-log2(e) / sigma^2 in squared units, micrometres per unit)
     const uint32_t* net_of;
@@ -151,7 +157,7 @@ __device__ __forceinline__ Eval eval_mixed_rec(const SearchArgs& a, const Lane& 
     }
     const float s2 = __uint_as_float(q.w);
     const float g = __builtin_fmaf(a.theta2f, d2, -s2);
-    if (fabsf(g) <= a.guard * s2) {
+    if (fabsf(g) <= __builtin_fmaf(a.guard, s2, a.guard_abs)) {
         return eval_exact(a, L, node, 0, leaf);
     }
     e.status = g > 0.0f ? kAccept : kExpand;

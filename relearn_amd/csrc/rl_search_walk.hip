@@ -206,7 +206,7 @@ __device__ __forceinline__ NodeEval eval_walk(const SearchArgs& a, const Group& 
     const float s2 = __uint_as_float(rq.w);
     const float g = __builtin_fmaf(a.theta2f, d2, -s2);
     const bool coinc = (ix | iy | iz) == 0;
-    const bool near = !leaf && fabsf(g) <= a.guard * s2;
+    const bool near = !leaf && fabsf(g) <= __builtin_fmaf(a.guard, s2, a.guard_abs);
     float x;
     if (kMu0) {
         x = d2 * G.k2q;
@@ -479,7 +479,8 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     // accepted when theta^2 d^2 > side^2 clear of the guard band, expanded when below it
                     const float s2 = __uint_as_float(rq.w);
                     const float g = __builtin_fmaf(s.theta2f, d2, -s2);
-                    const uint64_t near = __builtin_amdgcn_ballot_w64(fabsf(g) <= s.guard * s2);
+                    const uint64_t near =
+                        __builtin_amdgcn_ballot_w64(fabsf(g) <= __builtin_fmaf(s.guard, s2, s.guard_abs));
                     const uint64_t pos = __builtin_amdgcn_ballot_w64(g > 0.0f);
                     need = top_mask & (coinc | near);
                     const uint64_t sure = top_mask & ~(coinc | near);

@@ -135,3 +135,24 @@ def test_walk_counting_instantiation_same_picks(oracle):
     assert st_c["walk_steps"] > 0 and st_c["walk_slots"] > 0
     assert st_c["walk_slots"] <= st_c["walk_steps"]
     assert st_c["walk_steps"] <= st_c["node_tests"] <= 64 * st_c["walk_steps"] + 8 * 32 * st_c["searches"]
+
+
+@pytest.mark.parametrize("flags", [0, RL_FLAG_WAVE_PER_SOURCE])
+def test_fp32_guard_tiny_cells_flat_kernel(oracle, flags):
+    # 1,500 neurons in a 2 um cube inside a 25,000 um box: the cluster's cells are a few hundredths of
+    # a micrometre, a few thousand fixed-point units, where rounding the source and the centroid to
+    # the grid moves theta^2 d^2 by more than the relative guard band of the FP32 test.  The band's
+    # absolute part sends those decisions to FP64, so the candidate lists, and with the flat kernel
+    # the picks and test counts, are the FP64 oracle's in both FP32-mixed kernels.
+    rng = np.random.default_rng(11)
+    xyz = np.concatenate([rng.random((500, 3)) * 25_000.0, 12_000.0 + rng.random((1500, 3)) * 2.0])
+    n = len(xyz)
+    net = Network(xyz, (rng.random(n) < 0.2).astype(np.uint8), rng.poisson(2, n).astype(np.uint32),
+                  rng.poisson(1.5, n).astype(np.uint32), rng.poisson(1.5, n).astype(np.uint32),
+                  (0.0, 0.0, 0.0), (25_000.0, 25_000.0, 25_000.0))
+    _say("FP32 guard band, 2 um cluster, flags", flags)
+    ref, nreq, st, tgt, tests, cnt = _pair(oracle, net, seed=13, step=2, sigma=FLAT, flags=flags)
+    bad, near = _check_search(ref, tgt, tests, cnt, net)
+    _say("  near-boundary", near, "mismatches", len(bad), bad[:5], "lane sources", st["lane_sources"])
+    assert not bad
+    assert nreq == int(cnt.sum())
