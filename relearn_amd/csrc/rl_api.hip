@@ -345,12 +345,13 @@ int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const doubl
         }
         c->coop_blocks = c->cus * c->waves_per_cu;
         c->coop_cap = 2048;
-        const size_t slots = static_cast<size_t>(c->cus) * kMaxWavesPerCu * c->coop_cap;
-        RL_TRY(dalloc(c, &c->coop_node, slots));
-        RL_TRY(dalloc(c, &c->coop_fexp, slots));
-        RL_TRY(dalloc(c, &c->coop_mass, slots));
         if (uses_walk(c)) {
-            RL_TRY(ensure_walk(c));
+            RL_TRY(ensure_walk(c));   // the wave-per-source kernel and its tables are not used
+        } else {
+            const size_t slots = static_cast<size_t>(c->cus) * kMaxWavesPerCu * c->coop_cap;
+            RL_TRY(dalloc(c, &c->coop_node, slots));
+            RL_TRY(dalloc(c, &c->coop_fexp, slots));
+            RL_TRY(dalloc(c, &c->coop_mass, slots));
         }
     }
     hipStream_t s = c->stream;

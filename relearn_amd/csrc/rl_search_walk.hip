@@ -709,9 +709,15 @@ int ensure_walk(rl_bh_ctx* c) {
     RL_TRY(dalloc(c, &c->walk_seg_goff, nseg + 1));
     RL_TRY(dalloc(c, &c->walk_seg_first, nseg));
     RL_TRY(dalloc(c, &c->walk_seg_count, nseg));
+    // tables for at most one wave per group, and at most about 8 KiB of table per neuron (at least
+    // 256 MiB): a context of a small network does not take the tables of the whole GPU, so that many
+    // such contexts fit on one device (its waves then take several groups each)
     const uint64_t groups_max = (n + kGroup - 1) / kGroup + nseg;
-    c->walk_blocks_cap = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(c->cus) * kWalkWavesPerCu,
-                                                                  std::max<uint64_t>(groups_max, 1)));
+    const uint64_t per_wave = static_cast<uint64_t>(kWalkCap) * (kWave * sizeof(double) + 8 * sizeof(uint32_t));
+    const uint64_t by_memory = std::max<uint64_t>(256ULL << 20, n * 8192ULL) / per_wave;
+    c->walk_blocks_cap = static_cast<uint32_t>(
+        std::min<uint64_t>(std::min<uint64_t>(static_cast<uint64_t>(c->cus) * kWalkWavesPerCu, by_memory),
+                           std::max<uint64_t>(groups_max, 1)));
     c->walk_cap = kWalkCap;
     const size_t slots = static_cast<size_t>(c->walk_blocks_cap) * c->walk_cap;
     RL_TRY(dalloc(c, &c->walk_cum, slots * kWave));

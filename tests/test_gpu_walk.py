@@ -139,18 +139,18 @@ def test_walk_counting_instantiation_same_picks(oracle):
 
 @pytest.mark.parametrize("flags", [0, RL_FLAG_WAVE_PER_SOURCE])
 def test_fp32_guard_tiny_cells_flat_kernel(oracle, flags):
-    # 1,500 neurons in a 2 um cube inside a 25,000 um box: the cluster's cells are a few hundredths of
-    # a micrometre, a few thousand fixed-point units, where rounding the source and the centroid to
-    # the grid moves theta^2 d^2 by more than the relative guard band of the FP32 test.  The band's
-    # absolute part sends those decisions to FP64, so the candidate lists, and with the flat kernel
-    # the picks and test counts, are the FP64 oracle's in both FP32-mixed kernels.
+    # 1,500 neurons in a 0.05 um cube inside a 25,000 um box: the cluster's cells are a few hundred
+    # fixed-point units wide, where rounding the source and the centroid to the grid moves
+    # theta^2 d^2 by far more than the relative guard band of the FP32 test.  The band's absolute part
+    # sends those decisions to FP64, so the candidate lists, and with the flat kernel the picks and
+    # test counts, are the FP64 oracle's in both FP32-mixed kernels.
     rng = np.random.default_rng(11)
-    xyz = np.concatenate([rng.random((500, 3)) * 25_000.0, 12_000.0 + rng.random((1500, 3)) * 2.0])
+    xyz = np.concatenate([rng.random((500, 3)) * 25_000.0, 12_000.0 + rng.random((1500, 3)) * 0.05])
     n = len(xyz)
     net = Network(xyz, (rng.random(n) < 0.2).astype(np.uint8), rng.poisson(2, n).astype(np.uint32),
                   rng.poisson(1.5, n).astype(np.uint32), rng.poisson(1.5, n).astype(np.uint32),
                   (0.0, 0.0, 0.0), (25_000.0, 25_000.0, 25_000.0))
-    _say("FP32 guard band, 2 um cluster, flags", flags)
+    _say("FP32 guard band, 0.05 um cluster, flags", flags)
     ref, nreq, st, tgt, tests, cnt = _pair(oracle, net, seed=13, step=2, sigma=FLAT, flags=flags)
     bad, near = _check_search(ref, tgt, tests, cnt, net)
     _say("  near-boundary", near, "mismatches", len(bad), bad[:5], "lane sources", st["lane_sources"])
