@@ -39,6 +39,13 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def kernel_flags(kernel):
+    from relearn_amd import RL_FLAG_EXACT_ORDER, RL_FLAG_WALK_ALWAYS, RL_FLAG_WAVE_PER_SOURCE
+
+    return {"default": 0, "wave": RL_FLAG_WAVE_PER_SOURCE, "walk": RL_FLAG_WALK_ALWAYS,
+            "lane": RL_FLAG_EXACT_ORDER}[kernel]
+
+
 def walk_counts(net, precision, seed, d_vac, flags, tuning):
     """Node steps and table slots of one FP32-mixed group-walk search, from the walk's counting
     instantiation (RL_FLAG_COUNT_WALK) on a context of its own, outside every timed region: the counts
@@ -184,7 +191,8 @@ def run_ensemble(args):
                                   "1000 (n/1000)^(1/3) um, theta 0.3, sigma 750 um, one plasticity update per step",
                       "networks": G, "neurons": n, "seed": args.seed}}
     for name, prec in (("fp32_mixed", RL_PREC_FP32_MIXED), ("fp64", RL_PREC_FP64)):
-        g = BarnesHutHIP(lo, hi, theta=0.3, sigma=750.0, seed=args.seed + 1, precision=prec)
+        g = BarnesHutHIP(lo, hi, theta=0.3, sigma=750.0, seed=args.seed + 1, precision=prec,
+                         flags=kernel_flags(args.kernel))
         t0 = time.perf_counter()
         g.init_networks(ens.offsets, ens.boxes, ens.xyz, ens.signal)
         build_s = time.perf_counter() - t0
@@ -292,7 +300,8 @@ def run_sim(args):
                       "neurons": n, "networks": G, "periods": periods, "period_steps": period, "seed": args.seed}}
     traj = {}
     for name, prec in (("fp32_mixed", RL_PREC_FP32_MIXED), ("fp64", RL_PREC_FP64)):
-        g = BarnesHutHIP(lo, hi, theta=0.3, sigma=750.0, seed=args.seed + 1, precision=prec)
+        g = BarnesHutHIP(lo, hi, theta=0.3, sigma=750.0, seed=args.seed + 1, precision=prec,
+                         flags=kernel_flags(args.kernel))
         if args.workload == "sim-ensemble":
             g.init_networks(ens.offsets, ens.boxes, ens.xyz, ens.signal)
         else:
@@ -511,9 +520,11 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32_mixed", "fp64_plain"], default=None)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 (one lane per source) run")
-    ap.add_argument("--kernel", choices=["default", "wave", "lane"], default="default",
-                    help="search kernel: default (the group walk for FP32-mixed, one wavefront per source for FP64), "
-                         "wave (one wavefront per source for both), lane (one lane per source in list order)")
+    ap.add_argument("--kernel", choices=["default", "wave", "walk", "lane"], default="default",
+                    help="search kernel: default (FP32-mixed: the group walk from 192 x 64 active sources per CU, "
+                         "one wavefront per source below; FP64: one wavefront per source), wave (one wavefront per "
+                         "source always), walk (FP32-mixed: the group walk always), lane (one lane per source in "
+                         "list order)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (reference kernel for rocprofv3 byte counters)")
     ap.add_argument("--workload", choices=["single", "ensemble", "sim-ensemble", "sim-single"], default="single",
@@ -560,7 +571,7 @@ def main(argv=None):
     d_vac = [int(a.data_ptr()) for a in d_arrays]
 
     res = {}
-    kflags = {"default": 0, "wave": RL_FLAG_WAVE_PER_SOURCE, "lane": RL_FLAG_EXACT_ORDER}[args.kernel]
+    kflags = kernel_flags(args.kernel)
     # fp64_plain: the FP64 path with one lane per source and the CPU path's list order (the plain
     # FP64 GPU method the FP32-mixed path is compared with)
     precs = [("fp32_mixed", RL_PREC_FP32_MIXED, kflags), ("fp64", RL_PREC_FP64, kflags)]
@@ -594,8 +605,8 @@ def main(argv=None):
     achieved = bytes_per_launch / kernel_s / 1e9
     if args.kernel == "lane" or main_name == "fp64_plain":
         kname = "k_search"
-    elif main_name == "fp32_mixed" and args.kernel == "default":
-        kname = "k_walk"
+    elif main_name == "fp32_mixed" and main_r["walk_steps_per_step"] > 0:
+        kname = "k_walk"   # the group walk ran (its counting run counted steps)
     else:
         kname = "k_level1"
     traffic = read_traffic(kname, main_name, args.neurons)

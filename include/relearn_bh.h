@@ -39,10 +39,14 @@ enum {
 
 enum {
     RL_FLAG_EXACT_ORDER = 1,      /* per-source search with sums in the CPU path's list order (slower) */
-    RL_FLAG_WAVE_PER_SOURCE = 2,  /* FP32-mixed: one wavefront per source (the FP64 search) instead of the
-                                     group walk (one lane per source, 64 neighbouring sources per wavefront) */
+    RL_FLAG_WAVE_PER_SOURCE = 2,  /* FP32-mixed: one wavefront per source (the FP64 search) for every
+                                     update.  By default an FP32-mixed update with at least 192 x 64 active
+                                     sources per CU (3,145,728 on 256 CUs) takes the group walk (one lane
+                                     per source, 64 neighbouring sources per wavefront), a smaller one a
+                                     wavefront per source; the device chooses from its own count. */
     RL_FLAG_COUNT_WALK = 4,       /* FP32-mixed group walk: count its node steps and table slots into
                                      rl_bh_step_stats (a separate, slower instantiation of the walk) */
+    RL_FLAG_WALK_ALWAYS = 8,      /* FP32-mixed: the group walk for every update, whatever its size */
 };
 
 enum {

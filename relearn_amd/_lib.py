@@ -19,6 +19,7 @@ RL_PREC_FP32_MIXED = 1
 RL_FLAG_EXACT_ORDER = 1
 RL_FLAG_WAVE_PER_SOURCE = 2
 RL_FLAG_COUNT_WALK = 4
+RL_FLAG_WALK_ALWAYS = 8
 
 _u8p = C.POINTER(C.c_uint8)
 _u32p = C.POINTER(C.c_uint32)

@@ -169,7 +169,8 @@ int rl_bh_create(const rl_bh_config* cfg, rl_bh_ctx** out) {
         set_global_error("rl_bh_create: unknown precision");
         return RL_EINVAL;
     }
-    if ((cfg->flags & ~(RL_FLAG_EXACT_ORDER | RL_FLAG_WAVE_PER_SOURCE | RL_FLAG_COUNT_WALK)) != 0) {
+    if ((cfg->flags & ~(RL_FLAG_EXACT_ORDER | RL_FLAG_WAVE_PER_SOURCE | RL_FLAG_COUNT_WALK | RL_FLAG_WALK_ALWAYS)) !=
+        0) {
         set_global_error("rl_bh_create: unknown flag bits");
         return RL_EINVAL;
     }
@@ -345,13 +346,16 @@ int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const doubl
         }
         c->coop_blocks = c->cus * c->waves_per_cu;
         c->coop_cap = 2048;
-        if (uses_walk(c)) {
-            RL_TRY(ensure_walk(c));   // the wave-per-source kernel and its tables are not used
-        } else {
+        if (!uses_walk(c) || walk_min_sources(c) > 0) {
+            // the wave-per-source kernel (an FP32-mixed context with the group walk uses it for
+            // updates with few active sources)
             const size_t slots = static_cast<size_t>(c->cus) * kMaxWavesPerCu * c->coop_cap;
             RL_TRY(dalloc(c, &c->coop_node, slots));
             RL_TRY(dalloc(c, &c->coop_fexp, slots));
             RL_TRY(dalloc(c, &c->coop_mass, slots));
+        }
+        if (uses_walk(c)) {
+            RL_TRY(ensure_walk(c));
         }
     }
     hipStream_t s = c->stream;

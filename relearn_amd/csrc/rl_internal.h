@@ -305,6 +305,8 @@ int candidate_lists(rl_bh_ctx* c, uint64_t num_sources, const uint32_t* sources,
                     double* probs, uint32_t* counts);
 struct SearchArgs;
 int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa);
+int launch_level1(rl_bh_ctx* c, const SearchArgs& sa, const uint32_t* work, const uint64_t* nwork,
+                  uint64_t run_below, uint32_t blocks, uint32_t chunk);
 int launch_finalize(rl_bh_ctx* c, const SearchArgs& sa);
 int launch_search_walk(rl_bh_ctx* c, const SearchArgs& sa);
 int ensure_walk(rl_bh_ctx* c);   // group-walk buffers (FP32-mixed contexts), sized at init
@@ -312,6 +314,16 @@ int ensure_walk(rl_bh_ctx* c);   // group-walk buffers (FP32-mixed contexts), si
 // source, or one lane per source in the CPU path's order
 inline bool uses_walk(const rl_bh_ctx* c) {
     return c->cfg.precision == RL_PREC_FP32_MIXED && (c->cfg.flags & (RL_FLAG_EXACT_ORDER | RL_FLAG_WAVE_PER_SOURCE)) == 0;
+}
+// Active sources from which an FP32-mixed update takes the group walk; below, one wavefront per source
+// is faster.  A group's walk is a chain of about 10^4 dependent steps (7-9 ms alone on a wave), which
+// pays off only when every wave has several groups.  Measured: one uniform network at the density of
+// 16M neurons in a 25,000 um cube crosses between 0.87M and 1.7M active sources, while the updates of
+// a 256-network simulation with 1.2M-2.3M searches still run 10 ms faster with a wavefront per
+// source; from 3.5M on the walk is ahead in both.  Hence 192 groups of 64 per CU, 3,145,728 on 256
+// CUs.  RL_FLAG_WALK_ALWAYS: from 0.
+inline uint64_t walk_min_sources(const rl_bh_ctx* c) {
+    return (c->cfg.flags & RL_FLAG_WALK_ALWAYS) != 0 ? 0 : 64ULL * 192ULL * c->cus;
 }
 int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, const unsigned long long* count);
 

@@ -57,6 +57,7 @@ struct CoopArgs {
     unsigned long long* n_overflow;
     uint4* slow;                  // (slot, source, node, level) of descents for the per-lane kernel
     unsigned long long* n_slow;
+    uint64_t run_below;           // the kernel returns at once when *nwork >= run_below
 };
 
 // Slot records are built in LDS during the traversal and copied to the wave's global slot table in
@@ -633,6 +634,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(5, 8
     int tried = 0;
     unsigned long long acc_exec = 0;
     const uint64_t nw = *a.nwork;
+    if (nw >= a.run_below) {
+        return;   // the group walk takes this update
+    }
     const uint32_t step = *s.step;
     const uint32_t chunk = a.chunk;
     uint64_t w_next = 0, w_lim = 0;   // the wave's current chunk of its segment
@@ -802,13 +806,26 @@ __global__ void __launch_bounds__(256) k_finalize(SearchArgs s) {
 // overflow the slot table (or whose root list sums to 0) go to c->coop_overflow, counted in
 // counters[kCtrOverflow], for the per-lane kernel.  All counts stay on the device.
 int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa) {
+    const bool timed = !c->capturing;
+    if (timed) RL_HIP(c, hipEventRecord(c->ev2, c->stream));
+    RL_TRY(launch_level1(c, sa, c->coop_work, sa.nwork, UINT64_MAX, c->coop_blocks, c->search_chunk));
+    if (timed) RL_HIP(c, hipEventRecord(c->ev3, c->stream));
+    RL_TRY(launch_descend_lane(c, sa, c->slow_items, c->counters + kCtrSlow));
+    return launch_finalize(c, sa);
+}
+
+// The wave-per-source kernel over work[0, *nwork) on `blocks` persistent waves taking `chunk` sources
+// per cursor increment, skipped on the device when *nwork >= run_below.
+int launch_level1(rl_bh_ctx* c, const SearchArgs& sa, const uint32_t* work, const uint64_t* nwork,
+                  uint64_t run_below, uint32_t blocks, uint32_t chunk) {
     hipStream_t s = c->stream;
     CoopArgs a{};
     a.s = sa;
-    a.work = c->coop_work;
-    a.nwork = sa.nwork;
+    a.work = work;
+    a.nwork = nwork;
+    a.run_below = run_below;
     a.cursor = c->coop_cursor;
-    a.chunk = std::max<uint32_t>(c->search_chunk, 1U);
+    a.chunk = std::max<uint32_t>(chunk, 1U);
     a.t_my = c->coop_node;
     a.t_fexp = c->coop_fexp;
     a.t_mass = c->coop_mass;
@@ -819,19 +836,15 @@ int launch_search_coop(rl_bh_ctx* c, const SearchArgs& sa) {
     a.n_slow = c->counters + kCtrSlow;
     const bool mixed = c->cfg.precision == RL_PREC_FP32_MIXED;
     const bool mu0 = c->cfg.mu == 0.0;
-    const bool timed = !c->capturing;
-    if (timed) RL_HIP(c, hipEventRecord(c->ev2, s));
     if (mixed && mu0) {
-        k_level1<true, true><<<c->coop_blocks, kWave, 0, s>>>(a);
+        k_level1<true, true><<<blocks, kWave, 0, s>>>(a);
     } else if (mixed) {
-        k_level1<true, false><<<c->coop_blocks, kWave, 0, s>>>(a);
+        k_level1<true, false><<<blocks, kWave, 0, s>>>(a);
     } else {
-        k_level1<false, false><<<c->coop_blocks, kWave, 0, s>>>(a);
+        k_level1<false, false><<<blocks, kWave, 0, s>>>(a);
     }
     RL_HIP(c, hipGetLastError());
-    if (timed) RL_HIP(c, hipEventRecord(c->ev3, s));
-    RL_TRY(launch_descend_lane(c, sa, c->slow_items, c->counters + kCtrSlow));
-    return launch_finalize(c, sa);
+    return RL_OK;
 }
 
 int launch_finalize(rl_bh_ctx* c, const SearchArgs& sa) {

@@ -65,6 +65,8 @@ struct WalkArgs {
     unsigned long long* n_overflow;
     uint4* slow;                 // (slot, source, node, level) of descents for the per-lane traversal
     unsigned long long* n_slow;
+    const uint64_t* n_active;    // active sources of the update
+    uint64_t run_from;           // the kernel returns at once when *n_active < run_from
 };
 
 constexpr uint32_t kBlockLevels = 24;            // LDS blocks of staged child records (a ring over the levels)
@@ -318,6 +320,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
     __shared__ WalkShared sh;
     // kCount: child records the wave examined (children of the roots and of expanded nodes), table slots used
     __shared__ unsigned long long cnt_steps, cnt_slots;
+    if (*a.n_active < a.run_from) {
+        return;   // the wave-per-source kernel takes this update
+    }
     const SearchArgs& s = a.s;
     const uint32_t ln = lane_id();
     const uint64_t cbase = static_cast<uint64_t>(blockIdx.x) * a.cap;   // first table slot of this wave
@@ -759,9 +764,18 @@ int launch_search_walk(rl_bh_ctx* c, const SearchArgs& sa) {
     a.n_overflow = c->counters + kCtrOverflow;
     a.slow = c->slow_items;
     a.n_slow = c->counters + kCtrSlow;
+    // both kernels are launched; the active-source count on the device decides which one searches
+    const uint64_t* n_active = c->walk_off + 2 * n;
+    a.n_active = n_active;
+    a.run_from = walk_min_sources(c);
     const unsigned blocks = std::max<uint32_t>(1U, std::min<uint32_t>(c->coop_blocks, c->walk_blocks_cap));
     const bool timed = !c->capturing;
     if (timed) RL_HIP(c, hipEventRecord(c->ev2, s));
+    if (a.run_from > 0) {
+        // the wave-per-source kernel over the same lists (sources of each type in Morton order), at the
+        // setting its tuning picks for updates below the threshold (20 waves per CU, 8 sources per grab)
+        RL_TRY(launch_level1(c, sa, c->walk_list, n_active, a.run_from, c->cus * 20U, 8U));
+    }
     // the counting instantiation is a separate kernel: the counters change the register allocation of the walk
     const bool count = (c->cfg.flags & RL_FLAG_COUNT_WALK) != 0;
     if (c->cfg.mu == 0.0) {

@@ -88,7 +88,7 @@ def test_create_rejects_bad_config_without_gpu(lib):
     cfg.box_max[:] = (1.0, 1.0, 1.0)
     cfg.theta, cfg.sigma, cfg.mu = 0.3, 750.0, 0.0
     cfg.precision = _lib.RL_PREC_FP32_MIXED
-    cfg.flags = 8
+    cfg.flags = 16
     assert lib.rl_bh_create(ctypes.byref(cfg), ctypes.byref(out)) == -1
     assert b"flag" in lib.rl_bh_last_error(None)
     cfg.flags = _lib.RL_FLAG_COUNT_WALK

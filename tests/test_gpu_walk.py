@@ -7,7 +7,8 @@ vacant count in both precisions and every running sum is an exact integer.  The 
 then be the oracle's FP64 picks exactly, with the same acceptance-test counts: this pins the walk's
 traversal, its table, the top-down resolution of the draws and the descents, for group sizes from 1
 to many groups, disabled neurons, empty lists, clustered positions and long lists (theta 0.1, where
-groups overflow the table and go to the per-lane kernel).  With the Gaussian at sigma = 750 um the
+groups overflow the table and go to the per-lane kernel).  These tests force the walk
+(RL_FLAG_WALK_ALWAYS); by default the device takes it only for updates with many active sources.  With the Gaussian at sigma = 750 um the
 FP32 probabilities differ from FP64 in the last bits: the picks may differ only in a small fraction of
 searches, the test counts must agree wherever the picks do, and the walk must agree with the
 wave-per-source FP32 search (RL_FLAG_WAVE_PER_SOURCE) just as closely.
@@ -19,7 +20,8 @@ import os
 import numpy as np
 import pytest
 
-from relearn_amd import RL_FLAG_COUNT_WALK, RL_FLAG_WAVE_PER_SOURCE, RL_PREC_FP32_MIXED, RL_PREC_FP64, BarnesHutHIP
+from relearn_amd import (RL_FLAG_COUNT_WALK, RL_FLAG_WALK_ALWAYS, RL_FLAG_WAVE_PER_SOURCE, RL_PREC_FP32_MIXED,
+                         RL_PREC_FP64, BarnesHutHIP)
 from relearn_amd.workloads import Network, uniform_cube
 from tests.test_gpu_parity import _check_search
 
@@ -33,7 +35,7 @@ def _say(*a):
     print("[gpu-case]", *a, flush=True)
 
 
-def _pair(oracle, net, seed, step, sigma, theta=0.3, enabled=None, flags=0, precision=RL_PREC_FP32_MIXED):
+def _pair(oracle, net, seed, step, sigma, theta=0.3, enabled=None, flags=RL_FLAG_WALK_ALWAYS, precision=RL_PREC_FP32_MIXED):
     t = oracle.Tree(net.xyz, net.box_min, net.box_max)
     t.update(net.vac_exc, net.vac_inh, enabled)
     ref = t.find_targets(net.signal, net.vac_axon, enabled, theta=theta, sigma=sigma, seed=seed, step=step, threads=0)
@@ -124,7 +126,8 @@ def test_walk_counting_instantiation_same_picks(oracle):
     net = uniform_cube(30_000, 10_000.0 * (0.03 ** (1 / 3)), 17)
     _say("walk, counting instantiation, 30000 neurons")
     ref, nreq, st, tgt, tests, cnt = _pair(oracle, net, seed=8, step=3, sigma=750.0)
-    _, nreq_c, st_c, tgt_c, tests_c, cnt_c = _pair(oracle, net, seed=8, step=3, sigma=750.0, flags=RL_FLAG_COUNT_WALK)
+    _, nreq_c, st_c, tgt_c, tests_c, cnt_c = _pair(oracle, net, seed=8, step=3, sigma=750.0,
+                                                   flags=RL_FLAG_COUNT_WALK | RL_FLAG_WALK_ALWAYS)
     assert st["walk_steps"] == 0 and st["walk_slots"] == 0
     assert np.array_equal(tgt, tgt_c) and np.array_equal(tests, tests_c) and np.array_equal(cnt, cnt_c)
     assert nreq == nreq_c
@@ -137,7 +140,7 @@ def test_walk_counting_instantiation_same_picks(oracle):
     assert st_c["walk_steps"] <= st_c["node_tests"] <= 64 * st_c["walk_steps"] + 8 * 32 * st_c["searches"]
 
 
-@pytest.mark.parametrize("flags", [0, RL_FLAG_WAVE_PER_SOURCE])
+@pytest.mark.parametrize("flags", [RL_FLAG_WALK_ALWAYS, RL_FLAG_WAVE_PER_SOURCE])
 def test_fp32_guard_tiny_cells_flat_kernel(oracle, flags):
     # 1,500 neurons in a 0.05 um cube inside a 25,000 um box: the cluster's cells are a few hundred
     # fixed-point units wide, where rounding the source and the centroid to the grid moves
@@ -156,3 +159,30 @@ def test_fp32_guard_tiny_cells_flat_kernel(oracle, flags):
     _say("  near-boundary", near, "mismatches", len(bad), bad[:5], "lane sources", st["lane_sources"])
     assert not bad
     assert nreq == int(cnt.sum())
+
+
+def _gpu_only(net, seed, step, flags):
+    g = BarnesHutHIP(net.box_min, net.box_max, theta=0.3, sigma=750.0, seed=seed, precision=RL_PREC_FP32_MIXED,
+                     flags=flags)
+    g.init(net.xyz, net.signal)
+    g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+    g.find_target_neurons(step)
+    st = g.last_search_stats()
+    out = g.get_search_slots(int(net.vac_axon.astype(np.uint64).sum()))
+    g.close()
+    return st, out
+
+
+@pytest.mark.parametrize("n", [20_000, 4_000_000])
+def test_default_kernel_choice_by_active_sources(n):
+    # by default the device takes the group walk from 192 x 64 active sources per CU on (3,145,728 on
+    # 256 CUs) and a wavefront per source below; the counting flag shows whether the walk ran, and the
+    # results are those of the forced choice, bit for bit
+    net = uniform_cube(n, 10_000.0 * (n / 1e6) ** (1 / 3), 23)
+    _say("default kernel choice,", n, "neurons")
+    st, (tgt, tests, cnt) = _gpu_only(net, 4, 1, RL_FLAG_COUNT_WALK)
+    walked = st["walk_steps"] > 0
+    assert walked == (n > 1_000_000)   # 4M neurons: 3.46M active sources
+    forced = RL_FLAG_WALK_ALWAYS if walked else RL_FLAG_WAVE_PER_SOURCE
+    st_f, (tgt_f, tests_f, cnt_f) = _gpu_only(net, 4, 1, forced)
+    assert np.array_equal(tgt, tgt_f) and np.array_equal(tests, tests_f) and np.array_equal(cnt, cnt_f)
