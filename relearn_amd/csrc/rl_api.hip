@@ -971,10 +971,9 @@ int rl_bh_autotune(rl_bh_ctx* c, uint32_t step, rl_bh_tuning* best) {
     }
     RL_TRY(set_device(c));
     RL_TRY(find_targets(c, step));  // warm-up at the current setting
+    // the result is one of the candidates below (the fastest; the first of equals)
     rl_bh_tuning bt{};
-    bt.search_waves_per_cu = c->waves_per_cu;
-    bt.search_chunk = c->search_chunk;
-    double best_ms = c->last_search_ms;
+    double best_ms = std::numeric_limits<double>::infinity();
     // each setting is timed twice and its faster run counts: single runs vary by 1-2%, as much as
     // neighbouring settings differ
     auto trial = [&](uint32_t waves, uint32_t chunk) -> int {
@@ -1060,6 +1059,7 @@ int rl_bh_graph_create(rl_bh_ctx* c, uint32_t steps, uint64_t max_vacant_axons) 
     }
     c->graph_steps = steps;
     c->graph_max_vacant = max_vacant_axons;
+    c->graph_appends = c->net.ready;
     c->prepared = false;
     c->searched_valid = false;
     return RL_OK;
@@ -1079,6 +1079,17 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     if (synaptic) {
         RL_TRY(syn_prepare(c));
     }
+    if (stats == nullptr && c->graph_appends) {
+        // a replay without stats cannot grow the network graph after an append that does not fit (the
+        // period would keep its committed vacancies without the synapses): it runs only while the graph
+        // holds the most synapses the period can create (one per vacant axon of its capacity)
+        const uint64_t bound = c->net.entries_bound + c->graph_max_vacant;
+        if (bound > c->net.cap) {
+            return fail(c, RL_ELIMIT, "rl_bh_graph_launch: without stats the network graph might not hold this "
+                                      "period's synapses; launch with stats, which grows it");
+        }
+        c->net.entries_bound = bound;
+    }
     RL_TRY(set_steps(c, first_step, update));
     if (stats != nullptr) {
         RL_HIP(c, hipEventRecord(c->ev0, s));
@@ -1096,6 +1107,12 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     RL_HIP(c, hipEventRecord(c->ev1, s));
     unsigned long long h[kCounters];
     RL_TRY(read_counters(c, h));
+    if (c->graph_appends && c->n > 0) {
+        uint64_t entries = 0;   // the graph's entries now (the stream is idle after the counters)
+        RL_HIP(c, hipMemcpyAsync(&entries, c->net.off + c->n, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        RL_HIP(c, hipStreamSynchronize(s));
+        c->net.entries_bound = entries;
+    }
     float ms = 0.0f;
     RL_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *stats = rl_bh_step_stats{};

@@ -93,6 +93,9 @@ struct NetGraph {
     int32_t* out_w = nullptr;      // cap
     int32_t* acc[2] = { nullptr, nullptr };  // n each
     uint64_t cap = 0;
+    // host: an upper bound of the graph's entries, exact after a call that synchronised with the device
+    // and raised by each replay without stats by the most synapses a period can append
+    uint64_t entries_bound = 0;
     bool ready = false;
     // acc[parity] holds the pushes of fired_buf[parity] over the current pairs, acc[parity ^ 1] is
     // zero, and the out-edge lists match the graph
@@ -236,6 +239,7 @@ struct rl_bh_ctx {
     hipGraphExec_t graph_exec = nullptr;
     uint32_t graph_steps = 0;
     uint64_t graph_max_vacant = 0;
+    bool graph_appends = false;   // the captured period appends its synapses to the network graph
     bool capturing = false;
     rl::Neurons neurons;
     rl::NetGraph net;

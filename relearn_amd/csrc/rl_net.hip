@@ -579,6 +579,7 @@ int net_init(rl_bh_ctx* c, uint64_t capacity, const uint64_t* in_offsets, const 
     }
     RL_HIP(c, hipMemsetAsync(g.ocnt, 0, n * sizeof(uint32_t), s));
     RL_HIP(c, hipStreamSynchronize(s));
+    g.entries_bound = src.size();
     g.ready = true;
     g.syn_valid = false;
     return RL_OK;
@@ -706,6 +707,7 @@ static int net_merge_grow(rl_bh_ctx* c) {
     if (have + add > g.cap) {
         RL_TRY(net_grow(c, have, have + add));
     }
+    g.entries_bound = have + add;   // the merge keeps at most the old entries and the new synapses
     return enqueue_net_merge(c);
 }
 

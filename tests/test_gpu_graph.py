@@ -257,3 +257,31 @@ def test_search_tuning_does_not_change_results(precision):
     with pytest.raises(BHError):
         g.set_tuning(16, 65)
     g.close()
+
+
+def test_graph_without_stats_refuses_when_the_network_graph_may_overflow():
+    # a replay without stats cannot grow the network graph, so it runs only while the graph holds the
+    # most synapses the period can create; a replay with stats grows it and completes the period
+    _say("graph without stats, network graph capacity")
+    net = uniform_cube(3000, 1400.0, 41)
+    g = BarnesHutHIP(net.box_min, net.box_max, seed=2)
+    g.init(net.xyz, net.signal)
+    g.net_init(16)
+    g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+    g.graph_create(0, int(net.vac_axon.sum()))
+    with pytest.raises(BHError) as e:
+        g.graph_launch(0, 1, stats=False)
+    assert e.value.code == RL_ELIMIT
+    va, _, _ = g.get_vacancies()
+    assert np.array_equal(va, net.vac_axon)   # nothing ran
+    st = g.graph_launch(0, 1)
+    assert st["created"] > 0 and st["net_regrown"] == 1
+    off, src, w = g.net_get_edges()[:3]
+    assert int(np.abs(w).sum()) == st["created"]
+    # with room for a whole period, a replay without stats runs
+    g.net_init(int(net.vac_axon.sum()) + 64)
+    g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+    g.graph_create(0, int(net.vac_axon.sum()))
+    g.graph_launch(0, 2, stats=False)
+    st = g.graph_launch(0, 3)
+    g.close()
