@@ -284,3 +284,19 @@ def test_reference_text_files_drive_a_simulation(oracle, tmp_path):
                                  100, 33, rng, ca_init=ca_init, target_ca=ca_tgt, min_ca=se["min_calcium"],
                                  retract_per_neuron=se["retract_ratio"])
     assert created > 0
+
+
+def test_reference_configuration_with_growing_thresholds_matches_oracle(oracle):
+    # the shape of the CPU reference configuration (1,000 neurons, 200 plasticity periods of 100 steps,
+    # 20,000 steps, elements and calcium from 0, synaptic input at conductance 0.03) with thresholds at
+    # which calcium lets elements grow (eta 0.02 / 0.01 / 0.0, target calcium 0.2, growth rate 1e-3):
+    # synapses form through the whole run, so every connectivity update (search, commit, network
+    # append, synaptic input over the grown network) is checked against the oracle, not only the step
+    n, periods, steps, seed = 1000, 200, 100, 0
+    _say("1000-neuron, 20,000-step simulation parity with growing thresholds")
+    net = uniform_cube(n, 1000.0, 0)
+    created, deleted = _simulate(oracle, net.xyz, net.signal, net.box_min, net.box_max, None, periods, steps, seed,
+                                 np.random.default_rng(23), cond=0.03, eta=(0.02, 0.01, 0.0), target=0.2, nu=1e-3,
+                                 ca0=None, start_elements=False, check_every=20)
+    _say("  created", created, "deleted", deleted)
+    assert created > 1000
