@@ -54,9 +54,10 @@ public:
     // BarnesHut(bounding_box, space_filling_curve, acceptance_criterion) (BarnesHut.h:55-60) with the
     // Gaussian kernel's sigma and mu (Gaussian.h:47-52) and the seed of the draw stream.  The default
     // precision is the CPU path's FP64, since the adapter stands in under the CPU name `barnes-hut`;
-    // FP32-mixed is an explicit choice.
+    // FP32-mixed is an explicit choice.  flags: RL_FLAG_* of relearn_bh.h (search kernel choice).
     BarnesHutHIP(const BoundingBox& box, double acceptance_criterion, double sigma = 750.0, double mu = 0.0,
-                 std::uint64_t seed = 0, Precision precision = Precision::FP64, int device = 0) {
+                 std::uint64_t seed = 0, Precision precision = Precision::FP64, int device = 0,
+                 std::int32_t flags = 0) {
         rl_bh_config cfg{};
         for (int d = 0; d < 3; d++) {
             cfg.box_min[d] = box.min[d];
@@ -68,6 +69,7 @@ public:
         cfg.seed = seed;
         cfg.precision = static_cast<std::int32_t>(precision);
         cfg.device = device;
+        cfg.flags = flags;
         const int rc = rl_bh_create(&cfg, &ctx_);
         if (rc != RL_OK) {
             throw Error(std::string("BarnesHutHIP: ") + rl_bh_last_error(nullptr));
