@@ -26,8 +26,14 @@
 // summed in list order, drawn from and picked; a level that needs expansion, or whose sum is 0 (the
 // reference's fallbacks), is handed to the per-lane general traversal.
 //
-// Sources whose list has candidates but a sum of 0 (FP32 underflow; the FP64 and n/r fallbacks), and
-// groups whose table or stack would overflow, go to the per-lane kernel whole.
+// Sources whose list has candidates but a sum of 0 (FP32 underflow; the FP64 and n/r fallbacks),
+// sources with an FP64 acceptance comparison too close to call on squares, and groups whose table or
+// stack would overflow go to the per-lane kernel whole.
+//
+// The walk pays off only when every wave has several groups (a group's walk is a chain of about 10^4
+// dependent steps): an update with fewer active sources than walk_min_sources() takes the
+// wave-per-source kernel, which is launched beside the walk; each returns at once unless the device's
+// active-source count is on its side.
 #include <algorithm>
 
 #include "rl_search_eval.h"
@@ -92,7 +98,6 @@ __device__ __forceinline__ uint32_t block_of(uint32_t level) {
     return level >= kBlockLevels ? level - kBlockLevels : level;
 }
 
-
 __device__ __forceinline__ uint32_t lane_id() {
     return __lane_id();
 }
@@ -142,14 +147,6 @@ __device__ __forceinline__ void stage_children(WalkShared& sh, const Group& G, u
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
 }
-
-
-
-// Node records read through the constant address space (scalar loads for wave-uniform indices).
-typedef unsigned int U32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int U32x4 __attribute__((ext_vector_type(4)));
-typedef const __attribute__((address_space(4))) U32x2 KMeta;
-typedef const __attribute__((address_space(4))) U32x4 KRec;
 
 struct NodeEval {
     int st;
