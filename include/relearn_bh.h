@@ -332,8 +332,11 @@ int rl_bh_set_tuning(rl_bh_ctx* ctx, const rl_bh_tuning* tuning);
 int rl_bh_get_tuning(rl_bh_ctx* ctx, rl_bh_tuning* tuning);
 
 /* Times the search of the prepared update `step` (after rl_bh_prepare_*) at candidate tunings (waves
- * per CU 8..24, then work-queue chunks 1..8 at the best wave count) and keeps the fastest.  Only the search runs; the vacancies do not change, and afterwards the search
- * result of `step` is readable as after rl_bh_find_target_neurons.  best may be NULL. */
+ * per CU 8..24, then work-queue chunks 1..8 at the best wave count) and keeps the fastest.  In an
+ * FP32-mixed context the tuning applies to the group walk: when the prepared update is below the walk's
+ * threshold (see RL_FLAG_WAVE_PER_SOURCE) nothing is timed and the setting stays.  Only the search
+ * runs; the vacancies do not change, and afterwards the search result of `step` is readable as after
+ * rl_bh_find_target_neurons.  best may be NULL. */
 int rl_bh_autotune(rl_bh_ctx* ctx, uint32_t step, rl_bh_tuning* best);
 
 /* ------------------------------------------------------------------ captured plasticity periods */

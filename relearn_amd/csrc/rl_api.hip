@@ -971,6 +971,15 @@ int rl_bh_autotune(rl_bh_ctx* c, uint32_t step, rl_bh_tuning* best) {
     }
     RL_TRY(set_device(c));
     RL_TRY(find_targets(c, step));  // warm-up at the current setting
+    if (uses_walk(c) && c->last_active < walk_min_sources(c)) {
+        // the prepared update is below the group walk's threshold: the wave-per-source kernel runs at
+        // its own fixed setting, and the walk's setting, which the tuning sets, is kept as it is
+        if (best != nullptr) {
+            best->search_waves_per_cu = c->waves_per_cu;
+            best->search_chunk = c->search_chunk;
+        }
+        return RL_OK;
+    }
     // the result is one of the candidates below (the fastest; the first of equals)
     rl_bh_tuning bt{};
     double best_ms = std::numeric_limits<double>::infinity();

@@ -183,6 +183,7 @@ struct rl_bh_ctx {
     bool committed = false;          // the requests of the last search were committed (a second commit
                                      // would apply them again)
     uint64_t last_searches = 0;
+    uint64_t last_active = 0;        // active sources of the last search
     uint64_t last_tests = 0;
     uint64_t last_tests_ref = 0;
     uint64_t last_lane_sources = 0;  // sources handled by the per-lane kernel in the last search

@@ -772,6 +772,7 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     c->last_tests = h[kCtrTestsExec];
     c->last_tests_ref = h[kCtrTestsRef];
     c->last_searches = h[kCtrSearches];
+    c->last_active = h[kCtrActive];
     c->last_slow_descents = h[kCtrSlow];
     c->last_lane_sources = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0 ? h[kCtrRedo] : h[kCtrOverflow];
     c->last_walk_steps = h[kCtrWalkSteps];

@@ -249,7 +249,11 @@ def test_search_tuning_does_not_change_results(precision):
         for a, b in zip(ref, cur):
             assert np.array_equal(a, b), (w, k)
     best = g.autotune(2)
-    assert 1 <= best["search_waves_per_cu"] <= 24 and 1 <= best["search_chunk"] <= 8
+    if precision == RL_PREC_FP32_MIXED:
+        # 30,000 neurons are below the group walk's threshold: the tuning (the walk's) stays as set
+        assert best == {"search_waves_per_cu": 3, "search_chunk": 64}
+    else:
+        assert 1 <= best["search_waves_per_cu"] <= 24 and 1 <= best["search_chunk"] <= 8
     for a, b in zip(ref, g.get_search_slots(slots)):
         assert np.array_equal(a, b)
     with pytest.raises(BHError):
