@@ -59,7 +59,7 @@ def walk_counts(net, precision, seed, d_vac, flags, tuning):
                      flags=flags | RL_FLAG_COUNT_WALK)
     g.init(net.xyz, net.signal)
     if tuning:
-        g.set_tuning(tuning["search_waves_per_cu"], tuning["search_chunk"])
+        g.set_tuning(tuning["search_waves_per_cu"], tuning["search_chunk"], tuning.get("walk_min_sources", 0))
     g.prepare_update_connectivity_device(*d_vac)
     g.find_target_neurons(0)
     st = g.last_search_stats()

@@ -320,10 +320,16 @@ int rl_net_get_edges(rl_bh_ctx* ctx, uint64_t capacity, uint64_t* in_offsets, ui
 /* ------------------------------------------------------------------ launch tuning */
 
 typedef struct {
-    uint32_t search_waves_per_cu;  /* persistent waves of the cooperative search per compute unit, 1..24 */
-    uint32_t search_chunk;         /* sources a search wave takes from the work queue at a time, 1..64
-                                      (0: 1) */
-    uint32_t reserved[6];
+    uint32_t search_waves_per_cu;  /* persistent waves of the search per compute unit, 1..24 (the group
+                                      walk's in an FP32-mixed context) */
+    uint32_t search_chunk;         /* sources (groups for the walk) a search wave takes from the work
+                                      queue at a time, 1..64 (0: 1) */
+    uint64_t walk_min_sources;     /* FP32-mixed context without RL_FLAG_WALK_ALWAYS or
+                                      RL_FLAG_WAVE_PER_SOURCE: active sources from which an update takes
+                                      the group walk instead of a wavefront per source (default 192 x 64
+                                      per CU; UINT64_MAX: never).  rl_bh_set_tuning: 0 keeps the current
+                                      value. */
+    uint32_t reserved[4];
 } rl_bh_tuning;
 
 /* Launch tuning of the search (no counterpart in the reference, whose CPU loop has none).  Results
@@ -333,10 +339,11 @@ int rl_bh_get_tuning(rl_bh_ctx* ctx, rl_bh_tuning* tuning);
 
 /* Times the search of the prepared update `step` (after rl_bh_prepare_*) at candidate tunings (waves
  * per CU 8..24, then work-queue chunks 1..8 at the best wave count) and keeps the fastest.  In an
- * FP32-mixed context the tuning applies to the group walk: when the prepared update is below the walk's
- * threshold (see RL_FLAG_WAVE_PER_SOURCE) nothing is timed and the setting stays.  Only the search
- * runs; the vacancies do not change, and afterwards the search result of `step` is readable as after
- * rl_bh_find_target_neurons.  best may be NULL. */
+ * FP32-mixed context (without RL_FLAG_WALK_ALWAYS or RL_FLAG_WAVE_PER_SOURCE) it first times the
+ * group walk and a wavefront per source on the update and moves walk_min_sources so that updates with
+ * this many active sources take the faster one; the walk's waves and chunk are then tuned when the walk
+ * is the faster, and kept otherwise.  Only the search runs; the vacancies do not change, and afterwards
+ * the search result of `step` is readable as after rl_bh_find_target_neurons.  best may be NULL. */
 int rl_bh_autotune(rl_bh_ctx* ctx, uint32_t step, rl_bh_tuning* best);
 
 /* ------------------------------------------------------------------ captured plasticity periods */

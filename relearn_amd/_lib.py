@@ -85,7 +85,8 @@ class Tuning(C.Structure):
     _fields_ = [
         ("search_waves_per_cu", C.c_uint32),
         ("search_chunk", C.c_uint32),
-        ("reserved", C.c_uint32 * 6),
+        ("walk_min_sources", C.c_uint64),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 
@@ -379,22 +380,30 @@ class BarnesHutHIP:
                         int(counts[i])))
         return out
 
-    def set_tuning(self, search_waves_per_cu, search_chunk=1):
+    def set_tuning(self, search_waves_per_cu, search_chunk=1, walk_min_sources=0):
+        """walk_min_sources: FP32-mixed, active sources from which an update takes the group walk
+        (0 keeps the current value)."""
         t = Tuning()
         t.search_waves_per_cu = int(search_waves_per_cu)
         t.search_chunk = int(search_chunk)
+        t.walk_min_sources = int(walk_min_sources)
         self._check(self.L.rl_bh_set_tuning(self.h, C.byref(t)))
+
+    @staticmethod
+    def _tuning_dict(t):
+        return {"search_waves_per_cu": t.search_waves_per_cu, "search_chunk": t.search_chunk,
+                "walk_min_sources": t.walk_min_sources}
 
     def get_tuning(self):
         t = Tuning()
         self._check(self.L.rl_bh_get_tuning(self.h, C.byref(t)))
-        return {"search_waves_per_cu": t.search_waves_per_cu, "search_chunk": t.search_chunk}
+        return self._tuning_dict(t)
 
     def autotune(self, step=0):
         """Keeps the fastest search tuning for the prepared update; returns it."""
         t = Tuning()
         self._check(self.L.rl_bh_autotune(self.h, int(step), C.byref(t)))
-        return {"search_waves_per_cu": t.search_waves_per_cu, "search_chunk": t.search_chunk}
+        return self._tuning_dict(t)
 
     def graph_create(self, steps=0, max_vacant_axons=None):
         """Captures a plasticity period (steps neuron steps, element commit, connectivity update)."""

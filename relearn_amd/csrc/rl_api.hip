@@ -341,6 +341,7 @@ int init_networks(rl_bh_ctx* c, uint32_t G, const uint64_t* offsets, const doubl
         int cus = 0;
         RL_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->cfg.device));
         c->cus = static_cast<uint32_t>(std::max(cus, 1));
+        c->walk_min = walk_min_default(c);
         if (uses_walk(c)) {
             c->waves_per_cu = 24;   // the group walk's tables and LDS are sized for 24 waves per CU
         }
@@ -940,12 +941,14 @@ int rl_bh_set_tuning(rl_bh_ctx* c, const rl_bh_tuning* t) {
     if (chunk > 64) {
         return fail(c, RL_EINVAL, "rl_bh_set_tuning: search_chunk must be in [0, 64]");
     }
-    if (t->search_waves_per_cu != c->waves_per_cu || chunk != c->search_chunk) {
-        destroy_graph(c);  // the captured launch has the old grid and chunk
+    const uint64_t walk_min = t->walk_min_sources == 0 ? c->walk_min : t->walk_min_sources;
+    if (t->search_waves_per_cu != c->waves_per_cu || chunk != c->search_chunk || walk_min != c->walk_min) {
+        destroy_graph(c);  // the captured launch has the old grid, chunk and kernel choice
     }
     c->search_chunk = chunk;
     c->waves_per_cu = t->search_waves_per_cu;
     c->coop_blocks = c->cus * c->waves_per_cu;
+    c->walk_min = walk_min;
     return RL_OK;
 }
 
@@ -956,6 +959,7 @@ int rl_bh_get_tuning(rl_bh_ctx* c, rl_bh_tuning* t) {
     *t = rl_bh_tuning{};
     t->search_waves_per_cu = c->waves_per_cu;
     t->search_chunk = c->search_chunk;
+    t->walk_min_sources = c->walk_min;
     return RL_OK;
 }
 
@@ -971,14 +975,42 @@ int rl_bh_autotune(rl_bh_ctx* c, uint32_t step, rl_bh_tuning* best) {
     }
     RL_TRY(set_device(c));
     RL_TRY(find_targets(c, step));  // warm-up at the current setting
-    if (uses_walk(c) && c->last_active < walk_min_sources(c)) {
-        // the prepared update is below the group walk's threshold: the wave-per-source kernel runs at
-        // its own fixed setting, and the walk's setting, which the tuning sets, is kept as it is
+    if (c->last_active == 0) {
+        // nothing searches in this update: nothing to time, the setting stays
         if (best != nullptr) {
-            best->search_waves_per_cu = c->waves_per_cu;
-            best->search_chunk = c->search_chunk;
+            RL_TRY(rl_bh_get_tuning(c, best));
         }
         return RL_OK;
+    }
+    if (uses_walk(c) && (c->cfg.flags & RL_FLAG_WALK_ALWAYS) == 0) {
+        // traversal width: the group walk (64 sources per wavefront) against a wavefront per source on
+        // this update, each forced, twice, the faster run counting; the threshold moves so that updates
+        // of this size take the faster kernel
+        const uint64_t active = c->last_active;
+        const uint64_t before = c->walk_min;
+        double t_walk = 0.0, t_wave = 0.0;
+        for (int k = 0; k < 2; k++) {
+            c->walk_min = k == 0 ? 1ULL : UINT64_MAX;
+            double ms = 0.0;
+            for (int rep = 0; rep < 2; rep++) {
+                RL_TRY(find_targets(c, step));
+                ms = rep == 0 ? c->last_search_ms : std::min(ms, c->last_search_ms);
+            }
+            (k == 0 ? t_walk : t_wave) = ms;
+        }
+        const bool walk = t_walk < t_wave;
+        c->walk_min = walk ? std::min(before, active) : std::max(before, active + 1);
+        if (c->walk_min != before) {
+            destroy_graph(c);
+        }
+        if (!walk) {
+            // the wave-per-source kernel runs at its own fixed setting; the walk's setting stays
+            RL_TRY(find_targets(c, step));
+            if (best != nullptr) {
+                RL_TRY(rl_bh_get_tuning(c, best));
+            }
+            return RL_OK;
+        }
     }
     // the result is one of the candidates below (the fastest; the first of equals)
     rl_bh_tuning bt{};
@@ -1010,10 +1042,11 @@ int rl_bh_autotune(rl_bh_ctx* c, uint32_t step, rl_bh_tuning* best) {
         RL_TRY(trial(waves, k));
     }
     rl_bh_tuning t = bt;
+    t.walk_min_sources = 0;   // keeps the width decision above
     RL_TRY(rl_bh_set_tuning(c, &t));
     RL_TRY(find_targets(c, step));  // leaves the search result of `step` readable, as find_target_neurons does
     if (best != nullptr) {
-        *best = t;
+        RL_TRY(rl_bh_get_tuning(c, best));
     }
     return RL_OK;
 }

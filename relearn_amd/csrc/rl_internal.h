@@ -184,6 +184,7 @@ struct rl_bh_ctx {
                                      // would apply them again)
     uint64_t last_searches = 0;
     uint64_t last_active = 0;        // active sources of the last search
+    uint64_t walk_min = 0;           // FP32-mixed: active sources from which an update takes the group walk
     uint64_t last_tests = 0;
     uint64_t last_tests_ref = 0;
     uint64_t last_lane_sources = 0;  // sources handled by the per-lane kernel in the last search
@@ -326,9 +327,12 @@ inline bool uses_walk(const rl_bh_ctx* c) {
 // 16M neurons in a 25,000 um cube crosses between 0.87M and 1.7M active sources, while the updates of
 // a 256-network simulation with 1.2M-2.3M searches still run 10 ms faster with a wavefront per
 // source; from 3.5M on the walk is ahead in both.  Hence 192 groups of 64 per CU, 3,145,728 on 256
-// CUs.  RL_FLAG_WALK_ALWAYS: from 0.
+// CUs: the default of walk_min, which the autotuner moves (rl_bh_autotune).  RL_FLAG_WALK_ALWAYS: from 0.
+inline uint64_t walk_min_default(const rl_bh_ctx* c) {
+    return 64ULL * 192ULL * c->cus;
+}
 inline uint64_t walk_min_sources(const rl_bh_ctx* c) {
-    return (c->cfg.flags & RL_FLAG_WALK_ALWAYS) != 0 ? 0 : 64ULL * 192ULL * c->cus;
+    return (c->cfg.flags & RL_FLAG_WALK_ALWAYS) != 0 ? 0 : c->walk_min;
 }
 int launch_descend_lane(rl_bh_ctx* c, const SearchArgs& sa, const uint4* items, const unsigned long long* count);
 

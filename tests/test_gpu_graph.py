@@ -240,7 +240,8 @@ def test_search_tuning_does_not_change_results(precision):
     ref = None
     for w, k in ((1, 1), (5, 1), (16, 1), (24, 1), (16, 4), (3, 64)):
         g.set_tuning(w, k)
-        assert g.get_tuning() == {"search_waves_per_cu": w, "search_chunk": k}
+        tu = g.get_tuning()
+        assert (tu["search_waves_per_cu"], tu["search_chunk"]) == (w, k)
         g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
         g.find_target_neurons(2)
         cur = g.get_search_slots(slots)
@@ -250,8 +251,10 @@ def test_search_tuning_does_not_change_results(precision):
             assert np.array_equal(a, b), (w, k)
     best = g.autotune(2)
     if precision == RL_PREC_FP32_MIXED:
-        # 30,000 neurons are below the group walk's threshold: the tuning (the walk's) stays as set
-        assert best == {"search_waves_per_cu": 3, "search_chunk": 64}
+        # 30,000 neurons: a wavefront per source is faster than the group walk, so the walk's setting stays
+        # as set and the walk's threshold stays above this update
+        assert (best["search_waves_per_cu"], best["search_chunk"]) == (3, 64)
+        assert best["walk_min_sources"] > 30_000
     else:
         assert 1 <= best["search_waves_per_cu"] <= 24 and 1 <= best["search_chunk"] <= 8
     for a, b in zip(ref, g.get_search_slots(slots)):
@@ -288,4 +291,24 @@ def test_graph_without_stats_refuses_when_the_network_graph_may_overflow():
     g.graph_create(0, int(net.vac_axon.sum()))
     g.graph_launch(0, 2, stats=False)
     st = g.graph_launch(0, 3)
+    g.close()
+
+
+@pytest.mark.parametrize("n", [30_000, 4_000_000])
+def test_autotune_moves_the_walk_threshold_to_the_faster_kernel(n):
+    # FP32-mixed: the autotuner times the group walk and a wavefront per source on the prepared update
+    # and moves the threshold so that an update of this size takes the faster one: at 30,000 neurons the
+    # wavefront per source (the threshold rises above the update), at 4M the walk (it falls to the
+    # update's active sources)
+    _say("autotuned traversal width,", n, "neurons")
+    net = uniform_cube(n, 10_000.0 * (n / 1e6) ** (1 / 3), 47)
+    active = int((net.vac_axon > 0).sum())
+    g = BarnesHutHIP(net.box_min, net.box_max, seed=6, precision=RL_PREC_FP32_MIXED)
+    g.init(net.xyz, net.signal)
+    g.set_tuning(24, 1, 1 if n < 1_000_000 else 10 ** 12)
+    g.prepare_update_connectivity(net.vac_axon, net.vac_exc, net.vac_inh)
+    best = g.autotune(1)
+    _say("  tuning", best, "active sources", active)
+    assert best["walk_min_sources"] == (active + 1 if n < 1_000_000 else active)
+    assert g.get_tuning() == best
     g.close()
