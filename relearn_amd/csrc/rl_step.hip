@@ -1,6 +1,7 @@
 // Fused per-step neuron update: background input, Izhikevich membrane update, calcium integrator and
-// the growth-curve deltas of the three synaptic element types, one lane per neuron, FP64 throughout
-// and in the CPU path's operation order (files compiled with -ffp-contract=off):
+// the growth-curve deltas of the three synaptic element types, one lane per neuron, in FP64 (the
+// counter-based background draw, normal_input below, is FP32) and in the CPU path's operation order
+// (files compiled with -ffp-contract=off):
 //   IzhikevichModelCPU::update_activity   (source/neurons/models/izhikevich/IzhikevichModelCPU.cpp:75-125)
 //   CalciumCalculatorCPU::update_current_calcium (source/neurons/calcium/CalciumCalculatorCPU.cpp:40-103)
 //   update_number_elements_openmp + gaussian_growth_curve
