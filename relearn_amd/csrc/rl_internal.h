@@ -53,9 +53,12 @@ struct Status {
 // node are contiguous.  Per dendrite type s (0 excitatory, 1 inhibitory) the search reads
 //   rec64[s][i] = (centroid x, y, z, max edge)          FP64 path, 32 B
 //   rec32[s][i] = (centroid x, y, z as 31-bit fixed point, squared max edge in squared fixed-point
-//                  units as float bits)                 FP32 path, 16 B
+//                  units as float bits, -infinity for a leaf)   FP32 path, 16 B
 //   meta[s][i]  = (vacant count, link | nchild << 28)  8 B
-// link is the first child for inner nodes and the neuron id for leaves.
+// link is the first child for inner nodes and the neuron id for leaves.  meta and rec32 hold
+// kNodePad more entries than nodes: the group walk copies the 8 records from a node's first child on,
+// whatever the node's child count, and the last node's first child may be among the last nodes.
+constexpr uint64_t kNodePad = 8;
 struct Tree {
     uint64_t m = 0;
     uint32_t depth = 0;

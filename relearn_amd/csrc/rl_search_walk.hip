@@ -15,7 +15,7 @@
 // one FP64 addition.
 //
 // Every node that some lane expands gets a slot in the wave's table: the slots of its expanded
-// children and, per lane, the running sum when the walk leaves the node's subtree.  A draw r of lane
+// children and, per lane that expanded it, the running sum when the walk leaves the node's subtree.  A draw r of lane
 // l is then resolved by lane l alone, top-down (ProbabilityPicker::pick_target, source/util/
 // ProbabilityPicker.h:46-112): from the root it scans the children of its current node in list order,
 // adds the probability of an accepted child, or jumps to the stored sum after an expanded child, and
@@ -65,7 +65,7 @@ struct WalkArgs {
     unsigned long long* cursor;  // kXcds work cursors
     uint32_t chunk;              // groups per cursor increment
     double* t_cum;               // per wave: cap x 64 running sums at the exit of each slot's subtree
-    uint32_t* t_child;           // per wave: cap x 8 slots of the expanded children (kNone otherwise)
+    uint32_t* t_child;           // per wave: cap x 8 slots of the expanded children (written for those only)
     uint32_t cap;
     uint32_t* overflow;          // sources for the per-lane kernel
     unsigned long long* n_overflow;
@@ -84,11 +84,7 @@ static_assert(kWalkDepth < 2 * kBlockLevels, "a stack level maps to its block wi
 // a time.  Levels share blocks modulo kBlockLevels; a block records the level it holds, and a pop to
 // a level whose block was taken by a deeper one copies the children again.
 struct WalkShared {
-    uint32_t first[kWalkDepth];
-    uint32_t cur[kWalkDepth];
-    uint32_t nch[kWalkDepth];
-    uint32_t slot[kWalkDepth];
-    uint64_t mask[kWalkDepth];
+    uint4 ent[kWalkDepth];   // (first child | next child to visit << 28, slot, lane mask low, high)
     uint32_t owner[kBlockLevels];
     uint4 rec[kBlockLevels][8];
     uint32_t meta[kBlockLevels][16];
@@ -129,20 +125,21 @@ typedef __attribute__((address_space(3))) void LdsVoid;
 // s_waitcnt immediate: VM_CNT 0 (the staged records have landed in LDS), EXP_CNT and LGKM_CNT not waited for
 constexpr int kWaitVm0 = 0x0F70;
 
-// Copies the records of the children [first, first + nc) of the node at stack level `level` into its
-// LDS block (lanes < nc copy a 16-byte position record each, lanes < 2 nc one word of a count/link
-// record each) and waits for them.
+// Copies the records of the 8 nodes from the first child of the node at stack level `level` on into
+// its LDS block (lanes 0-7 a 16-byte position record each, lanes 0-15 one word of a count/link record
+// each) and waits for them.  Records past the node's last child are copied too and never read: the
+// node arrays end with kNodePad spare entries, so the copy stays inside them for every node.
 __device__ __forceinline__ void stage_children(WalkShared& sh, const Group& G, uint32_t level, uint32_t first,
-                                               uint32_t nc, uint32_t ln) {
+                                               uint32_t ln) {
     const uint32_t b = block_of(level);
-    if (ln < nc) {
+    if (__builtin_amdgcn_inverse_ballot_w64(0xFFULL)) {
         __builtin_amdgcn_global_load_lds((void*)(G.rec32 + first + ln), (LdsVoid*)&sh.rec[b][0], 16, 0, 0);
     }
-    if (ln < 2 * nc) {
+    if (__builtin_amdgcn_inverse_ballot_w64(0xFFFFULL)) {
         __builtin_amdgcn_global_load_lds((void*)(reinterpret_cast<const uint32_t*>(G.meta) + 2 * first + ln),
                                          (LdsVoid*)&sh.meta[b][0], 4, 0, 0);
     }
-    if (ln == 0) {
+    if (__builtin_amdgcn_inverse_ballot_w64(1ULL)) {
         sh.owner[b] = level;
     }
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
@@ -402,132 +399,109 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         const uint32_t root_first = mr.y & kLinkMask;
         const uint32_t root_nc = mr.x == 0 ? 0U : (mr.y >> kLinkBits);
         if (root_nc != 0) {
-            uint32_t top_first = root_first, top_cur = root_nc, top_nch = root_nc, top_slot = 0;
+            uint32_t top_first = root_first, top_cur = root_nc, top_slot = 0;
             uint64_t top_mask = amask;
             uint32_t nslots = 1, sp = 0;
-            if (ln < 8) {
-                tchild[ln] = kNone;
-            }
             if (ln < kBlockLevels) {
                 sh.owner[ln] = kNone;
             }
-            stage_children(sh, G, 0, root_first, root_nc, ln);
+            stage_children(sh, G, 0, root_first, ln);
             // a lane's level-1 tests: the root's children and the children of every node it expands
             tests = active ? root_nc : 0U;
             if (kCount && ln == 0) {
                 atomicAdd(&cnt_steps, static_cast<unsigned long long>(root_nc));
             }
             while (true) {
-                if (top_cur == 0) {
-                    // the walk leaves the subtree of this slot's node
-                    tcum[static_cast<uint64_t>(top_slot) * kWave + ln] = cum;
-                    if (sp == 0) {
-                        if (kCount && ln == 0) {
-                            atomicAdd(&cnt_slots, static_cast<unsigned long long>(nslots));
+                // steps over the remaining children of the top entry, until some lane expands one or
+                // none is left; only the child counter and the sums change here
+                uint64_t exm = 0;
+                {
+                    const uint32_t bl = block_of(sp);
+                    while (exm == 0 && top_cur != 0) {
+                        top_cur--;
+                        const uint2 mraw = *reinterpret_cast<const uint2*>(&sh.meta[bl][2 * top_cur]);
+                        const uint4 rq = sh.rec[bl][top_cur];
+                        // both records are read together, before the count decides anything
+                        __asm__ volatile("" ::"v"(mraw.x), "v"(mraw.y), "v"(rq.x), "v"(rq.y), "v"(rq.z), "v"(rq.w));
+                        const uint32_t cnt = uniform_u32(mraw.x);
+                        if (cnt == 0) {
+                            continue;   // no vacant dendrites below: discarded by every lane
                         }
-                        break;
+                        const int32_t ix = static_cast<int32_t>(rq.x) - W.qx;
+                        const int32_t iy = static_cast<int32_t>(rq.y) - W.qy;
+                        const int32_t iz = static_cast<int32_t>(rq.z) - W.qz;
+                        const float dx = static_cast<float>(ix);
+                        const float dy = static_cast<float>(iy);
+                        const float dz = static_cast<float>(iz);
+                        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        // accepted when theta^2 d^2 > side^2 clear of the guard band, expanded when below
+                        // it.  A leaf's squared side is -infinity, so a leaf is accepted, never near the
+                        // band.  The squared distance of integer differences is 0 exactly at a
+                        // fixed-point coincidence, which is decided in FP64 (the source's own leaf, a
+                        // coincidence since a leaf keeps its neuron's position, is dropped there).  Lane
+                        // masks come straight from the comparisons and are combined with scalar mask
+                        // arithmetic.
+                        const float s2 = __uint_as_float(rq.w);
+                        const float g = __builtin_fmaf(s.theta2f, d2, -s2);
+                        const uint64_t coinc = __builtin_amdgcn_ballot_w64(d2 == 0.0f);
+                        const uint64_t near =
+                            __builtin_amdgcn_ballot_w64(fabsf(g) <= __builtin_fmaf(s.guard, s2, s.guard_abs));
+                        const uint64_t pos = __builtin_amdgcn_ballot_w64(g > 0.0f);
+                        float x;
+                        if (kMu0) {
+                            x = d2 * G.k2q;
+                        } else {
+                            const float dd = sqrtf(d2) * G.inv_q - s.muf;
+                            x = dd * dd * s.k2f;
+                        }
+                        const float w = __builtin_amdgcn_exp2f(x) * static_cast<float>(cnt);
+                        const uint64_t unsure = coinc | near;
+                        const uint64_t sure = top_mask & ~unsure;
+                        const uint64_t need = top_mask & unsure;
+                        uint64_t acc = sure & pos;
+                        uint64_t ex = sure & ~pos;
+                        if (need != 0) {
+                            const uint32_t mt_y = uniform_u32(mraw.y);
+                            const bool leaf = (mt_y >> kLinkBits) == 0;
+                            int st = kDiscard;
+                            if (__builtin_amdgcn_inverse_ballot_w64(need)) {
+                                if (!leaf || (mt_y & kLinkMask) != W.src) {
+                                    st = decide_exact(s, G, W.src, top_first + top_cur, leaf);
+                                }
+                            }
+                            acc |= __builtin_amdgcn_ballot_w64(st == kAccept);
+                            ex |= __builtin_amdgcn_ballot_w64(st == kExpand);
+                            handoff |= __builtin_amdgcn_ballot_w64(st == kUndecided);
+                        }
+                        float wa = __builtin_amdgcn_inverse_ballot_w64(acc) ? w : 0.0f;
+                        __asm__ volatile("" : "+v"(wa));   // select in FP32, then one conversion
+                        cum += static_cast<double>(wa);
+                        has |= acc;
+                        exm = ex;
                     }
-                    sp--;
-                    top_first = uniform_u32(sh.first[sp]);
-                    top_cur = uniform_u32(sh.cur[sp]);
-                    top_slot = uniform_u32(sh.slot[sp]);
-                    const uint64_t mk = sh.mask[sp];
-                    top_mask = (static_cast<uint64_t>(uniform_u32(static_cast<uint32_t>(mk >> 32U))) << 32U) |
-                               uniform_u32(static_cast<uint32_t>(mk));
-                    if (uniform_u32(sh.owner[block_of(sp)]) != sp) {
-                        stage_children(sh, G, sp, top_first, uniform_u32(sh.nch[sp]), ln);
-                    }
-                    continue;
                 }
-                top_cur--;
-                const uint32_t y = top_first + top_cur;
-                const uint32_t bl = block_of(sp);
-                const uint2 mraw = *reinterpret_cast<const uint2*>(&sh.meta[bl][2 * top_cur]);
-                const uint4 rq = sh.rec[bl][top_cur];
-                // both records are read together, before the count decides anything
-                __asm__ volatile("" ::"v"(mraw.x), "v"(mraw.y), "v"(rq.x), "v"(rq.y), "v"(rq.z), "v"(rq.w));
-                const uint2 mt = make_uint2(uniform_u32(mraw.x), uniform_u32(mraw.y));
-                if (mt.x == 0) {
-                    continue;   // no vacant dendrites below: discarded by every lane
-                }
-                const bool leaf = (mt.y >> kLinkBits) == 0;
-                const int32_t ix = static_cast<int32_t>(rq.x) - W.qx;
-                const int32_t iy = static_cast<int32_t>(rq.y) - W.qy;
-                const int32_t iz = static_cast<int32_t>(rq.z) - W.qz;
-                const float dx = static_cast<float>(ix);
-                const float dy = static_cast<float>(iy);
-                const float dz = static_cast<float>(iz);
-                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                // lane masks straight from the comparisons; combined with scalar mask arithmetic.  The
-                // squared distance of integer differences is 0 exactly at a fixed-point coincidence.
-                const uint64_t coinc = __builtin_amdgcn_ballot_w64(d2 == 0.0f);
-                float x;
-                if (kMu0) {
-                    x = d2 * G.k2q;
-                } else {
-                    const float dd = sqrtf(d2) * G.inv_q - s.muf;
-                    x = dd * dd * s.k2f;
-                }
-                const float w = __builtin_amdgcn_exp2f(x) * static_cast<float>(mt.x);
-                uint64_t acc, exm, need;
-                if (leaf) {
-                    // accepted unless at distance 0: the source's own leaf is dropped without a test in
-                    // FP64, another fixed-point coincidence is decided in FP64
-                    const uint64_t self = __builtin_amdgcn_ballot_w64((mt.y & kLinkMask) == W.src);
-                    need = top_mask & ~self & coinc;
-                    acc = top_mask & ~self & ~coinc;
-                    exm = 0;
-                } else {
-                    // accepted when theta^2 d^2 > side^2 clear of the guard band, expanded when below it
-                    const float s2 = __uint_as_float(rq.w);
-                    const float g = __builtin_fmaf(s.theta2f, d2, -s2);
-                    const uint64_t near =
-                        __builtin_amdgcn_ballot_w64(fabsf(g) <= __builtin_fmaf(s.guard, s2, s.guard_abs));
-                    const uint64_t pos = __builtin_amdgcn_ballot_w64(g > 0.0f);
-                    need = top_mask & (coinc | near);
-                    const uint64_t sure = top_mask & ~(coinc | near);
-                    acc = sure & pos;
-                    exm = sure & ~pos;
-                }
-                if (need != 0) {
-                    int st = kDiscard;
-                    if (__builtin_amdgcn_inverse_ballot_w64(need)) {
-                        st = decide_exact(s, G, W.src, y, leaf);
-                    }
-                    acc |= __builtin_amdgcn_ballot_w64(st == kAccept);
-                    exm |= __builtin_amdgcn_ballot_w64(st == kExpand);
-                    handoff |= __builtin_amdgcn_ballot_w64(st == kUndecided);
-                }
-                float wa = __builtin_amdgcn_inverse_ballot_w64(acc) ? w : 0.0f;
-                __asm__ volatile("" : "+v"(wa));   // select in FP32, then one conversion
-                cum += static_cast<double>(wa);
-                has |= acc;
                 if (exm != 0) {
+                    // some lanes expand child top_cur: its slot and children become the top entry
                     if (nslots >= a.cap || sp + 1 >= kWalkDepth) {
                         over = true;
                         break;
                     }
                     const uint32_t ns = nslots;
                     nslots = uniform_u32(ns + 1);
-                    // the lane that initialised the parent's entry top_cur writes it (same lane, in order)
+                    // the child's slot in the parent's table entry; entries are written for expanded
+                    // children only, and a lane reads only those of the children it expanded
                     if (ln == top_cur) {
                         tchild[static_cast<uint64_t>(top_slot) * 8 + top_cur] = ns;
                     }
-                    if (ln < 8) {
-                        tchild[static_cast<uint64_t>(ns) * 8 + ln] = kNone;
-                    }
-                    if (ln == 0) {
-                        sh.first[sp] = top_first;
-                        sh.cur[sp] = top_cur;
-                        sh.nch[sp] = top_nch;
-                        sh.slot[sp] = top_slot;
-                        sh.mask[sp] = top_mask;
+                    const uint32_t link = uniform_u32(sh.meta[block_of(sp)][2 * top_cur + 1]);
+                    if (__builtin_amdgcn_inverse_ballot_w64(1ULL)) {
+                        sh.ent[sp] = make_uint4(top_first | (top_cur << kLinkBits), top_slot,
+                                                static_cast<uint32_t>(top_mask), static_cast<uint32_t>(top_mask >> 32U));
                     }
                     sp++;
-                    top_first = uniform_u32(mt.y & kLinkMask);
-                    top_cur = mt.y >> kLinkBits;
-                    top_nch = uniform_u32(top_cur);
-                    top_slot = uniform_u32(ns);
+                    top_first = link & kLinkMask;
+                    top_cur = link >> kLinkBits;
+                    top_slot = ns;
                     top_mask = exm;
                     if (__builtin_amdgcn_inverse_ballot_w64(exm)) {
                         tests += top_cur;
@@ -535,7 +509,29 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     if (kCount && ln == 0) {
                         atomicAdd(&cnt_steps, static_cast<unsigned long long>(top_cur));
                     }
-                    stage_children(sh, G, sp, top_first, top_cur, ln);
+                    stage_children(sh, G, sp, top_first, ln);
+                    continue;
+                }
+                // the walk leaves the subtree of this slot's node: the running sums of the lanes that
+                // expanded the node, the only lanes that read them back
+                if (__builtin_amdgcn_inverse_ballot_w64(top_mask)) {
+                    tcum[static_cast<uint64_t>(top_slot) * kWave + ln] = cum;
+                }
+                if (sp == 0) {
+                    if (kCount && ln == 0) {
+                        atomicAdd(&cnt_slots, static_cast<unsigned long long>(nslots));
+                    }
+                    break;
+                }
+                sp--;
+                const uint4 e = sh.ent[sp];
+                const uint32_t e0 = uniform_u32(e.x);
+                top_first = e0 & kLinkMask;
+                top_cur = e0 >> kLinkBits;
+                top_slot = uniform_u32(e.y);
+                top_mask = (static_cast<uint64_t>(uniform_u32(e.w)) << 32U) | uniform_u32(e.z);
+                if (uniform_u32(sh.owner[block_of(sp)]) != sp) {
+                    stage_children(sh, G, sp, top_first, ln);
                 }
             }
         }

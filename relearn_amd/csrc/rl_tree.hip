@@ -465,8 +465,11 @@ __global__ void k_quant_leaves(uint64_t m, const double4* __restrict__ rec, cons
     const uint32_t g = node_net[i];
     const double* lo = qa.net_box + 6 * static_cast<uint64_t>(g);
     const double sc = qa.net_scale[g];
+    // a leaf's squared side is -infinity: theta^2 d^2 - side^2 is then +infinity, so the FP32 test accepts
+    // the leaf, outside any guard band, as the acceptance criterion does for every leaf.  The upward
+    // pass rewrites the records of inner nodes.
     const uint4 v = make_uint4(quantize(r.x, lo[0], sc), quantize(r.y, lo[1], sc), quantize(r.z, lo[2], sc),
-                               quant_side2(r.w, sc));
+                               0xFF800000U);
     q0[i] = v;
     q1[i] = v;
 }
@@ -710,13 +713,13 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         t.level_start.assign(hls.begin(), hls.begin() + depth + 2);
         if ((status = dalloc(c, &t.level, m)) || (status = dalloc(c, &t.octant, m)) || (status = dalloc(c, &t.box, 6 * m)) ||
             (status = dalloc(c, &t.rec64[0], m)) || (status = dalloc(c, &t.rec64[1], m)) ||
-            (status = dalloc(c, &t.meta[0], m)) || (status = dalloc(c, &t.meta[1], m)) ||
+            (status = dalloc(c, &t.meta[0], m + kNodePad)) || (status = dalloc(c, &t.meta[1], m + kNodePad)) ||
             (status = dalloc(c, &parent, m)) || (status = dalloc(c, &link, m)) || (status = dalloc(c, &nchild, m)) ||
             (status = dalloc(c, &isleaf, m)) || (status = dalloc(c, &c->leaf_of, n)) || (status = dalloc(c, &t.node_net, m))) {
             break;
         }
         if (c->cfg.precision == RL_PREC_FP32_MIXED) {
-            if ((status = dalloc(c, &t.rec32[0], m)) || (status = dalloc(c, &t.rec32[1], m)) ||
+            if ((status = dalloc(c, &t.rec32[0], m + kNodePad)) || (status = dalloc(c, &t.rec32[1], m + kNodePad)) ||
                 (status = dalloc(c, &c->qpos, 3 * n))) {
                 break;
             }

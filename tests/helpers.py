@@ -57,3 +57,24 @@ def subtree_leaves(export):
                 acc.extend(leaves[c])
             leaves[i] = acc
     return leaves
+
+
+def deep_cluster(seed):
+    """3,000 neurons uniform in a 1,000 um box, and 8 sequences converging on one point of the root's
+    octant 2 (x and z low, y high) at 1000 * 2^-k um from it for k = 3..28 (down to 8 units of the
+    31-bit fixed point): the octree is about 30 levels deep there, and the level-1 lists of the
+    cluster's sources hold nodes that deep."""
+    from relearn_amd.workloads import Network
+
+    rng = np.random.default_rng(seed)
+    box = 1000.0
+    c = np.array([437.1, 611.3, 389.7])
+    dirs = np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], np.float64)
+    dirs = dirs * (0.5 + 0.1 * rng.random((8, 3)))
+    pts = [c + box * 2.0 ** -k * d for k in range(3, 29) for d in dirs]
+    xyz = np.concatenate([rng.random((3000, 3)) * box, np.array(pts)])
+    n = len(xyz)
+    vac_axon = rng.poisson(2, n).astype(np.uint32)
+    vac_axon[3000:] = 1 + rng.poisson(1, n - 3000).astype(np.uint32)
+    return Network(xyz, (rng.random(n) < 0.2).astype(np.uint8), vac_axon, rng.poisson(1.5, n).astype(np.uint32),
+                   rng.poisson(1.5, n).astype(np.uint32), (0.0, 0.0, 0.0), (box, box, box))

@@ -23,6 +23,7 @@ import pytest
 from relearn_amd import (RL_FLAG_COUNT_WALK, RL_FLAG_WALK_ALWAYS, RL_FLAG_WAVE_PER_SOURCE, RL_PREC_FP32_MIXED,
                          RL_PREC_FP64, BarnesHutHIP)
 from relearn_amd.workloads import Network, uniform_cube
+from tests.helpers import deep_cluster
 from tests.test_gpu_parity import _check_search
 
 pytestmark = pytest.mark.gpu
@@ -186,3 +187,19 @@ def test_default_kernel_choice_by_active_sources(n):
     forced = RL_FLAG_WALK_ALWAYS if walked else RL_FLAG_WAVE_PER_SOURCE
     st_f, (tgt_f, tests_f, cnt_f) = _gpu_only(net, 4, 1, forced)
     assert np.array_equal(tgt, tgt_f) and np.array_equal(tests, tests_f) and np.array_equal(cnt, cnt_f)
+
+
+@pytest.mark.parametrize("flags,precision", [(RL_FLAG_WALK_ALWAYS, RL_PREC_FP32_MIXED),
+                                             (RL_FLAG_WAVE_PER_SOURCE, RL_PREC_FP32_MIXED),
+                                             (0, RL_PREC_FP64)])
+def test_deep_cluster_flat_kernel(oracle, flags, precision):
+    # a union walk deeper than the LDS ring of staged children (the lists reach level 27 or deeper:
+    # test_deep_cluster_lists_reach_past_the_staging_ring): when the walk returns to a level whose block
+    # a deeper level took, it copies that level's children again, with the level's own child count
+    net = deep_cluster(19)
+    _say("deep cluster, flat kernel, flags", flags, "precision", precision)
+    ref, nreq, st, tgt, tests, cnt = _pair(oracle, net, seed=17, step=3, sigma=FLAT, flags=flags, precision=precision)
+    bad, near = _check_search(ref, tgt, tests, cnt, net)
+    _say("  near-boundary", near, "mismatches", len(bad), bad[:5], "lane sources", st["lane_sources"])
+    assert not bad
+    assert nreq == int(cnt.sum())

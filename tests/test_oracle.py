@@ -10,7 +10,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from tests.helpers import brute_octree, subtree_leaves
+from tests.helpers import brute_octree, deep_cluster, subtree_leaves
 
 
 # ------------------------------------------------------------------ random number streams
@@ -659,3 +659,22 @@ def test_deletion_is_order_independent_and_uniform(oracle):
     for key, u in units.items():
         expect = trials * k * u / m
         assert abs(hits.get(key, 0) - expect) < 5 * math.sqrt(expect) + 5, (key, hits.get(key, 0), expect)
+
+
+def test_deep_cluster_lists_reach_past_the_staging_ring(oracle):
+    # the deep-tree workload of the GPU walk tests: level-1 lists of the cluster's sources hold nodes 27
+    # or more levels below the root, deeper than the group walk's 24 LDS blocks of staged children
+    # (the children of a node at stack level l, level l + 1 below the root, are staged into block l mod 24)
+    net = deep_cluster(19)
+    t = oracle.Tree(net.xyz, net.box_min, net.box_max)
+    t.update(net.vac_exc, net.vac_inh)
+    lev = t.export()["level"]
+    depth = t.depth
+    deepest = 0
+    for i in range(3000, len(net.xyz), 5):
+        nodes, prob, total, fb, tests = t.candidate_list(0, net.xyz[i], i, int(net.signal[i]), 0.3, 1.0e12)
+        if len(nodes):
+            deepest = max(deepest, int(lev[nodes.astype(np.int64)].max()))
+    t.close()
+    assert depth >= 28
+    assert deepest >= 27
