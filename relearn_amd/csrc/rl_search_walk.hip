@@ -539,7 +539,6 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         __threadfence_block();
         const uint32_t na = active ? s.vac_axon[W.src] : 0U;
         const uint64_t slot0 = active ? s.slot_off[W.src] : 0U;
-        const uint32_t local = W.src - net_first;
         const double total = cum;
         const bool mine = ((has >> ln) & 1ULL) != 0;
         acc_exec += tests;
@@ -562,16 +561,41 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         }
         const bool draws = active && !to_lane && mine;
         const double next1 = draws ? nextafter(total, total + 0.00001) : 0.0;
-        uint32_t maxna = draws ? na : 0U;
+        // the draws of all lanes, numbered lane by lane (a lane's draws j = 0 .. na - 1 in order), are
+        // shared out over the wave 64 at a time: item i goes to lane i mod 64, which resolves it for
+        // the source lane holding it, with that lane's position, sums and table column
+        const uint32_t nd = draws ? na : 0U;
+        uint32_t incl = nd;   // inclusive prefix sum of the draws over the lanes
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            maxna = max(maxna, static_cast<uint32_t>(__shfl_xor(static_cast<int>(maxna), o, kWave)));
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), o, kWave));
+            incl += ln >= static_cast<uint32_t>(o) ? t : 0U;
         }
-        maxna = uniform_u32(maxna);
+        const uint32_t ndraws = uniform_u32(static_cast<uint32_t>(__shfl(static_cast<int>(incl), kWave - 1, kWave)));
         uint32_t exec = 0;
-        for (uint32_t j = 0; j < maxna; j++) {
-            bool nav = draws && j < na;
-            const double r = nav ? draw_to_uniform(pick_draw(seed, local, step, j, 1U), next1) : 0.0;
+        for (uint32_t base = 0; base < ndraws; base += kWave) {
+            const uint32_t item = base + ln;
+            // the source lane of the item: the first lane whose inclusive prefix exceeds it
+            uint32_t sl = 0;
+#pragma unroll
+            for (uint32_t half = kWave / 2; half > 0; half >>= 1) {
+                const uint32_t probe = sl + half - 1U;
+                const uint32_t at = static_cast<uint32_t>(__shfl(static_cast<int>(incl), static_cast<int>(probe), kWave));
+                sl = at <= item ? sl + half : sl;
+            }
+            bool nav = item < ndraws;
+            sl = nav ? sl : ln;
+            const uint32_t j = item - (static_cast<uint32_t>(__shfl(static_cast<int>(incl), static_cast<int>(sl), kWave)) -
+                                       static_cast<uint32_t>(__shfl(static_cast<int>(nd), static_cast<int>(sl), kWave)));
+            WalkLane V;
+            V.qx = __shfl(W.qx, static_cast<int>(sl), kWave);
+            V.qy = __shfl(W.qy, static_cast<int>(sl), kWave);
+            V.qz = __shfl(W.qz, static_cast<int>(sl), kWave);
+            V.src = static_cast<uint32_t>(__shfl(static_cast<int>(W.src), static_cast<int>(sl), kWave));
+            const double vnext1 = __shfl(next1, static_cast<int>(sl), kWave);
+            const uint32_t vtests = static_cast<uint32_t>(__shfl(static_cast<int>(tests), static_cast<int>(sl), kWave));
+            const uint32_t vlocal = V.src - net_first;
+            const double r = nav ? draw_to_uniform(pick_draw(seed, vlocal, step, j, 1U), vnext1) : 0.0;
             uint32_t nfirst = root_first, ncur = root_nc, nslot = 0;
             double ncum = 0.0;
             uint32_t picked = kNone;
@@ -589,7 +613,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                         const uint2 mt = G.meta[y];
                         const uint4 rq = G.rec32[y];
                         exec++;
-                        const NodeEval e = eval_walk<kMu0>(s, G, W, y, mt, rq, true);
+                        const NodeEval e = eval_walk<kMu0>(s, G, V, y, mt, rq, true);
                         if (e.st == kAccept) {
                             const double cn = ncum + e.p;
                             if (reaches(cn, r)) {
@@ -598,13 +622,13 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                             }
                             ncum = cn;
                         } else if (e.st == kExpand) {
-                            const uint32_t sl = tchild[static_cast<uint64_t>(nslot) * 8 + ncur];
-                            if (sl >= a.cap) {
+                            const uint32_t cs = tchild[static_cast<uint64_t>(nslot) * 8 + ncur];
+                            if (cs >= a.cap) {
                                 nav = false;   // unreachable for a consistent table
                             } else {
-                                const double ce = tcum[static_cast<uint64_t>(sl) * kWave + ln];
+                                const double ce = tcum[static_cast<uint64_t>(cs) * kWave + sl];
                                 if (reaches(ce, r)) {
-                                    nslot = sl;
+                                    nslot = cs;
                                     nfirst = mt.y & kLinkMask;
                                     ncur = mt.y >> kLinkBits;
                                 } else {
@@ -616,11 +640,11 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 }
             }
             // descent below the picked node
-            const bool drew = draws && j < na;
-            uint32_t node = picked, level = 2, dtests = tests;
+            const bool drew = item < ndraws;
+            uint32_t node = picked, level = 2, dtests = vtests;
             int state = kDescFailed;
             if (drew && picked != kNone) {
-                state = descend<kMu0>(s, G, W, seed, local, step, j, node, level, dtests, exec);
+                state = descend<kMu0>(s, G, V, seed, vlocal, step, j, node, level, dtests, exec);
             }
             const uint64_t sm = __builtin_amdgcn_ballot_w64(drew && state == kDescSlow);
             unsigned long long at0 = 0;
@@ -629,10 +653,10 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
             }
             at0 = __shfl(at0, 0, kWave);
             if (drew) {
-                const uint64_t slot = slot0 + j;
+                const uint64_t slot = s.slot_off[V.src] + j;
                 s.slot_tests[slot] = dtests;
                 if (state == kDescSlow) {
-                    a.slow[at0 + rank_below(sm)] = make_uint4(static_cast<uint32_t>(slot), W.src, node, level);
+                    a.slow[at0 + rank_below(sm)] = make_uint4(static_cast<uint32_t>(slot), V.src, node, level);
                 } else {
                     s.slot_target[slot] = state == kDescDone ? node : kNone;
                 }
