@@ -15,13 +15,16 @@
 // one FP64 addition.
 //
 // Every node that some lane expands gets a slot in the wave's table: the slots of its expanded
-// children and, per lane that expanded it, the running sum when the walk leaves the node's subtree.  A draw r of lane
-// l is then resolved by lane l alone, top-down (ProbabilityPicker::pick_target, source/util/
-// ProbabilityPicker.h:46-112): from the root it scans the children of its current node in list order,
-// adds the probability of an accepted child, or jumps to the stored sum after an expanded child, and
-// stops at the first accepted child whose running sum reaches r or enters the first expanded child
-// whose stored sum does.  The running sums are the same FP64 additions in the same order as the
-// walk's, so the pick is the one of the sequential list.  Below a picked inner node the lane descends
+// children and, per lane that expanded it, the running sum when the walk leaves the node's subtree.
+// A node none of whose children is expanded (most of them: their children are accepted or
+// discarded by every lane) keeps no sums; its entry in the parent's slot says so, and a draw that
+// meets it sums its children itself, in the walk's order.  A draw r of lane l is then resolved by
+// lane l alone, top-down (ProbabilityPicker::pick_target, source/util/ProbabilityPicker.h:46-112):
+// from the root it scans the children of its current node in list order, adds the probability of an
+// accepted child, or jumps to the stored sum after an expanded child, and stops at the first accepted
+// child whose running sum reaches r or enters the first expanded child whose stored sum does (or
+// whose children, summed on, do).  The running sums are the same FP64 additions in the same order as
+// the walk's, so the pick is the one of the sequential list.  Below a picked inner node the lane descends
 // (BarnesHutBase::find_target_neuron, :223-255): the node's children, normally all accepted, are
 // summed in list order, drawn from and picked; a level that needs expansion, or whose sum is 0 (the
 // reference's fallbacks), is handed to the per-lane general traversal.
@@ -48,6 +51,7 @@ constexpr uint32_t kWalkDepth = kMaxLevels + 4;  // stack entries (one per tree 
 constexpr int kXcds = kCursorSlots;
 constexpr uint32_t kWalkCap = 3584;              // table slots per wave (expanded nodes of one group)
 constexpr uint32_t kWalkWavesPerCu = 24;         // waves per CU the tables are sized for
+constexpr uint32_t kNoGrandSlots = 0xFFFFFFFEU;  // child-slot entry of an expanded child none of whose children is expanded
 
 inline unsigned grid_for(uint64_t n, int block = kBlock) {
     const uint64_t g = (n + block - 1) / block;
@@ -488,11 +492,6 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     }
                     const uint32_t ns = nslots;
                     nslots = uniform_u32(ns + 1);
-                    // the child's slot in the parent's table entry; entries are written for expanded
-                    // children only, and a lane reads only those of the children it expanded
-                    if (ln == top_cur) {
-                        tchild[static_cast<uint64_t>(top_slot) * 8 + top_cur] = ns;
-                    }
                     const uint32_t link = uniform_u32(sh.meta[block_of(sp)][2 * top_cur + 1]);
                     if (__builtin_amdgcn_inverse_ballot_w64(1ULL)) {
                         sh.ent[sp] = make_uint4(top_first | (top_cur << kLinkBits), top_slot,
@@ -512,16 +511,21 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     stage_children(sh, G, sp, top_first, ln);
                     continue;
                 }
-                // the walk leaves the subtree of this slot's node: the running sums of the lanes that
-                // expanded the node, the only lanes that read them back
-                if (__builtin_amdgcn_inverse_ballot_w64(top_mask)) {
-                    tcum[static_cast<uint64_t>(top_slot) * kWave + ln] = cum;
-                }
+                // the walk leaves the subtree of this slot's node
                 if (sp == 0) {
+                    // the root's sums are the lanes' totals, which the draws take from registers
                     if (kCount && ln == 0) {
                         atomicAdd(&cnt_slots, static_cast<unsigned long long>(nslots));
                     }
                     break;
+                }
+                // the running sums of the lanes that expanded the node, the only lanes that read them
+                // back, unless no child of the node was expanded: a draw then sums the node's children
+                // itself (as many as a table entry would save it), in the walk's order
+                const uint32_t cslot = top_slot;
+                const bool grand = nslots != cslot + 1U;
+                if (grand && __builtin_amdgcn_inverse_ballot_w64(top_mask)) {
+                    tcum[static_cast<uint64_t>(cslot) * kWave + ln] = cum;
                 }
                 sp--;
                 const uint4 e = sh.ent[sp];
@@ -530,6 +534,11 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 top_cur = e0 >> kLinkBits;
                 top_slot = uniform_u32(e.y);
                 top_mask = (static_cast<uint64_t>(uniform_u32(e.w)) << 32U) | uniform_u32(e.z);
+                // the child's entry in the parent's slot: written for expanded children only, and a
+                // lane reads only those of the children it expanded
+                if (ln == top_cur) {
+                    tchild[static_cast<uint64_t>(top_slot) * 8 + top_cur] = grand ? cslot : kNoGrandSlots;
+                }
                 if (uniform_u32(sh.owner[block_of(sp)]) != sp) {
                     stage_children(sh, G, sp, top_first, ln);
                 }
@@ -597,16 +606,26 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
             const uint32_t vlocal = V.src - net_first;
             const double r = nav ? draw_to_uniform(pick_draw(seed, vlocal, step, j, 1U), vnext1) : 0.0;
             uint32_t nfirst = root_first, ncur = root_nc, nslot = 0;
+            // the level left for the children of an expanded child without a table slot of its own
+            uint32_t rfirst = 0, rcur = 0;
+            bool below = false;
             double ncum = 0.0;
             uint32_t picked = kNone;
             // level-1 pick: top-down through the table
-            for (uint32_t guard = 0; guard < 64U * (kMaxLevels + 2); guard++) {
+            for (uint32_t guard = 0; guard < 80U * (kMaxLevels + 2); guard++) {
                 if (__builtin_amdgcn_ballot_w64(nav) == 0) {
                     break;
                 }
                 if (nav) {
                     if (ncur == 0) {
-                        nav = false;   // unreachable for a consistent table
+                        if (below) {
+                            // the children summed without reaching the draw: back to the level above
+                            nfirst = rfirst;
+                            ncur = rcur;
+                            below = false;
+                        } else {
+                            nav = false;   // unreachable for a consistent table
+                        }
                     } else {
                         ncur--;
                         const uint32_t y = nfirst + ncur;
@@ -622,8 +641,15 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                             }
                             ncum = cn;
                         } else if (e.st == kExpand) {
-                            const uint32_t cs = tchild[static_cast<uint64_t>(nslot) * 8 + ncur];
-                            if (cs >= a.cap) {
+                            const uint32_t cs = below ? kNone : tchild[static_cast<uint64_t>(nslot) * 8 + ncur];
+                            if (cs == kNoGrandSlots) {
+                                // no child of this child was expanded: its children, in list order
+                                rfirst = nfirst;
+                                rcur = ncur;
+                                below = true;
+                                nfirst = mt.y & kLinkMask;
+                                ncur = mt.y >> kLinkBits;
+                            } else if (cs >= a.cap) {
                                 nav = false;   // unreachable for a consistent table
                             } else {
                                 const double ce = tcum[static_cast<uint64_t>(cs) * kWave + sl];
