@@ -50,6 +50,7 @@ constexpr uint32_t kGroup = 64;                  // sources per wavefront
 constexpr uint32_t kWalkDepth = kMaxLevels + 4;  // stack entries (one per tree level)
 constexpr int kXcds = kCursorSlots;
 constexpr uint32_t kWalkCap = 3584;              // table slots per wave (expanded nodes of one group)
+static_assert(kWalkCap < 0xFFFFU, "a slot and the no-slot mark fit the 16-bit child entries kept in LDS");
 constexpr uint32_t kWalkWavesPerCu = 24;         // waves per CU the tables are sized for
 constexpr uint32_t kNoGrandSlots = 0xFFFFFFFEU;  // child-slot entry of an expanded child none of whose children is expanded
 
@@ -89,6 +90,7 @@ static_assert(kWalkDepth < 2 * kBlockLevels, "a stack level maps to its block wi
 // a level whose block was taken by a deeper one copies the children again.
 struct WalkShared {
     uint4 ent[kWalkDepth];   // (first child | next child to visit << 28, slot, lane mask low, high)
+    uint16_t kids[kWalkDepth][8];   // entries of the children of each level's node, until it is left
     uint32_t owner[kBlockLevels];
     uint4 rec[kBlockLevels][8];
     uint32_t meta[kBlockLevels][16];
@@ -513,7 +515,12 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 }
                 // the walk leaves the subtree of this slot's node
                 if (sp == 0) {
-                    // the root's sums are the lanes' totals, which the draws take from registers
+                    // the root's sums are the lanes' totals, which the draws take from registers; its
+                    // children's entries are stored
+                    if (__builtin_amdgcn_inverse_ballot_w64(0xFFULL)) {
+                        const uint32_t v = sh.kids[0][ln];
+                        tchild[ln] = v == 0xFFFFU ? kNoGrandSlots : v;
+                    }
                     if (kCount && ln == 0) {
                         atomicAdd(&cnt_slots, static_cast<unsigned long long>(nslots));
                     }
@@ -524,8 +531,16 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 // itself (as many as a table entry would save it), in the walk's order
                 const uint32_t cslot = top_slot;
                 const bool grand = nslots != cslot + 1U;
-                if (grand && __builtin_amdgcn_inverse_ballot_w64(top_mask)) {
-                    tcum[static_cast<uint64_t>(cslot) * kWave + ln] = cum;
+                if (grand) {
+                    if (__builtin_amdgcn_inverse_ballot_w64(top_mask)) {
+                        tcum[static_cast<uint64_t>(cslot) * kWave + ln] = cum;
+                    }
+                    // the entries of the node's children, in one store: written for expanded children
+                    // only, and a lane reads only those of the children it expanded
+                    if (__builtin_amdgcn_inverse_ballot_w64(0xFFULL)) {
+                        const uint32_t v = sh.kids[sp][ln];
+                        tchild[static_cast<uint64_t>(cslot) * 8 + ln] = v == 0xFFFFU ? kNoGrandSlots : v;
+                    }
                 }
                 sp--;
                 const uint4 e = sh.ent[sp];
@@ -534,10 +549,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 top_cur = e0 >> kLinkBits;
                 top_slot = uniform_u32(e.y);
                 top_mask = (static_cast<uint64_t>(uniform_u32(e.w)) << 32U) | uniform_u32(e.z);
-                // the child's entry in the parent's slot: written for expanded children only, and a
-                // lane reads only those of the children it expanded
-                if (ln == top_cur) {
-                    tchild[static_cast<uint64_t>(top_slot) * 8 + top_cur] = grand ? cslot : kNoGrandSlots;
+                // the child's entry in the parent's slot
+                if (__builtin_amdgcn_inverse_ballot_w64(1ULL)) {
+                    sh.kids[sp][top_cur] = static_cast<uint16_t>(grand ? cslot : 0xFFFFU);
                 }
                 if (uniform_u32(sh.owner[block_of(sp)]) != sp) {
                     stage_children(sh, G, sp, top_first, ln);
