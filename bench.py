@@ -27,7 +27,6 @@ METRIC = "target searches/s @16M neurons, FP32-mixed vs FP64; ms per plasticity 
 HBM_PEAK_GBS = 8000.0
 B_NODE = {"fp64": 40, "fp32_mixed": 24}   # bytes read per acceptance test (record + count/link)
 B_IO = 32                                  # source position + output ids per search
-B_WALK_SLOT = 64 * 8 + 8 * 4               # group walk: per table slot the 64 running sums + 8 child slots
 # Simulation workloads: Normal(5, 2) input alone drives the Izhikevich neurons to about 0.0117 spikes
 # per step, so calcium settles near 0.117 and never reaches the reference's axon growth threshold
 # 0.4, and with no axons no synapse forms to add input; the "growing" thresholds and target let
@@ -54,7 +53,7 @@ def walk_counts(net, precision, seed, d_vac, flags, tuning):
     from relearn_amd import RL_FLAG_COUNT_WALK, RL_PREC_FP32_MIXED, BarnesHutHIP
 
     if precision != RL_PREC_FP32_MIXED or flags & 3:
-        return 0.0, 0.0
+        return 0.0, 0.0, 0.0
     g = BarnesHutHIP(net.box_min, net.box_max, theta=0.3, sigma=750.0, seed=seed, precision=precision,
                      flags=flags | RL_FLAG_COUNT_WALK)
     g.init(net.xyz, net.signal)
@@ -64,8 +63,9 @@ def walk_counts(net, precision, seed, d_vac, flags, tuning):
     g.find_target_neurons(0)
     st = g.last_search_stats()
     g.close()
-    log(f"group walk: {st['walk_steps']} node steps, {st['walk_slots']} table slots per search")
-    return float(st["walk_steps"]), float(st["walk_slots"])
+    log(f"group walk: {st['walk_steps']} node steps, {st['walk_slots']} table slots, "
+        f"{st['walk_table_bytes']} table bytes stored per search")
+    return float(st["walk_steps"]), float(st["walk_slots"]), float(st["walk_table_bytes"])
 
 
 def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
@@ -137,7 +137,7 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
     sample = np.random.default_rng(seed + 7).choice(net.n, min(1000, net.n), replace=False)
     cand = g.candidate_lists(sample, capacity=16384)
     g.close()
-    walk_steps, walk_slots = walk_counts(net, precision, seed, d_vac, flags, tuning)
+    walk_steps, walk_slots, walk_bytes = walk_counts(net, precision, seed, d_vac, flags, tuning)
     return {
         "build_s": build_s,
         "nodes": m,
@@ -152,6 +152,7 @@ def run_precision(net, precision, steps, warmup, seed, d_vac, flags=0):
         "node_tests_per_step": tests / steps,
         "walk_steps_per_step": walk_steps,
         "walk_slots_per_step": walk_slots,
+        "walk_table_bytes_per_step": walk_bytes,
         "node_tests_ref_per_step": tests_ref / steps,
         "phase_s": [float(np.mean([r[2][i] for r in recs])) for i in range(3)],
         "graph_ms_per_update": float(np.mean(graph_ms)),
@@ -591,13 +592,14 @@ def main(argv=None):
     b_node = B_NODE["fp32_mixed" if main_name == "fp32_mixed" else "fp64"]
     # algorithmic bytes of one launch.  The wave-per-source and per-lane kernels read one node record for
     # every acceptance test they execute; the group walk reads one node record per step (shared by the
-    # up to 64 sources of the wavefront) and writes one table slot per expanded node.  Every search
-    # reads its source and writes its result.
+    # up to 64 sources of the wavefront) and stores, for each expanded node with an expanded child, the
+    # running sums of the sources that expanded it and its child entries (counted by the walk's counting
+    # instantiation).  Every search reads its source and writes its result.
     if main_r["walk_steps_per_step"] > 0:
-        bytes_per_launch = (main_r["walk_steps_per_step"] * b_node + main_r["walk_slots_per_step"] * B_WALK_SLOT
+        bytes_per_launch = (main_r["walk_steps_per_step"] * b_node + main_r["walk_table_bytes_per_step"]
                             + main_r["searches_per_step"] * B_IO)
-        bytes_model = (f"walk steps x {b_node} B (node record, read once for the wavefront) + table slots x "
-                       f"{B_WALK_SLOT} B + searches x {B_IO} B per launch")
+        bytes_model = (f"walk steps x {b_node} B (node record, read once for the wavefront) + table bytes stored "
+                       f"(8 B per running sum kept, 32 B per child-entry row) + searches x {B_IO} B per launch")
     else:
         bytes_per_launch = main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO
         bytes_model = f"executed acceptance tests x {b_node} B + searches x {B_IO} B per launch"
@@ -646,6 +648,7 @@ def main(argv=None):
             "node_tests_per_launch": main_r["node_tests_per_step"],
             "walk_steps_per_launch": main_r["walk_steps_per_step"],
             "walk_slots_per_launch": main_r["walk_slots_per_step"],
+            "walk_table_bytes_per_launch": main_r["walk_table_bytes_per_step"],
             # the same executed tests priced at one node record each (the wave-per-source kernels' demand)
             "achieved_per_test_records": (main_r["node_tests_per_step"] * b_node + main_r["searches_per_step"] * B_IO)
             / kernel_s / 1e9,

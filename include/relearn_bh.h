@@ -93,6 +93,9 @@ typedef struct {
                                    for the up to 64 sources of a wavefront); 0 without the flag */
     uint64_t walk_slots;        /* FP32-mixed group walk with RL_FLAG_COUNT_WALK: expanded nodes that got a slot
                                    of the per-source subtree sums; 0 without the flag */
+    uint64_t walk_table_bytes;  /* FP32-mixed group walk with RL_FLAG_COUNT_WALK: bytes stored in the slot tables
+                                   (8 per running sum of a source that expanded a node with an expanded
+                                   child, 32 per child-entry row of such a node); 0 without the flag */
 } rl_bh_step_stats;
 
 /* Algorithm's constructor: BarnesHut(bounding_box, sfc, theta) (BarnesHut.h:54-60) with a

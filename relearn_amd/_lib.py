@@ -75,6 +75,7 @@ class StepStats(C.Structure):
         ("slow_descents", C.c_uint64),
         ("walk_steps", C.c_uint64),
         ("walk_slots", C.c_uint64),
+        ("walk_table_bytes", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -785,6 +785,7 @@ int rl_bh_update_connectivity(rl_bh_ctx* c, uint32_t step, rl_bh_step_stats* sta
             stats->slow_descents = c->last_slow_descents;
             stats->walk_steps = c->last_walk_steps;
             stats->walk_slots = c->last_walk_slots;
+            stats->walk_table_bytes = c->last_walk_bytes;
         }
     } while (false);
     for (auto& ev : e) {
@@ -854,6 +855,7 @@ int rl_bh_last_search_stats(rl_bh_ctx* c, rl_bh_step_stats* st) {
     st->slow_descents = c->last_slow_descents;
     st->walk_steps = c->last_walk_steps;
     st->walk_slots = c->last_walk_slots;
+    st->walk_table_bytes = c->last_walk_bytes;
     return RL_OK;
 }
 
@@ -1174,10 +1176,12 @@ int rl_bh_graph_launch(rl_bh_ctx* c, uint32_t first_step, uint32_t update, rl_bh
     c->last_lane_sources = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0 ? h[kCtrRedo] : h[kCtrOverflow];
     c->last_walk_steps = h[kCtrWalkSteps];
     c->last_walk_slots = h[kCtrWalkSlots];
+    c->last_walk_bytes = h[kCtrWalkBytes];
     stats->lane_sources = c->last_lane_sources;
     stats->slow_descents = c->last_slow_descents;
     stats->walk_steps = c->last_walk_steps;
     stats->walk_slots = c->last_walk_slots;
+    stats->walk_table_bytes = c->last_walk_bytes;
     c->searched_valid = true;
     c->committed = h[kCtrOverCap] == 0;   // the replay committed its requests (none when over capacity)
     if (h[kCtrOverCap] != 0) {

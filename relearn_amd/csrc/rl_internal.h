@@ -41,7 +41,8 @@ enum : int {
     kCtrDelShort = 15,   // deletions that found fewer synapses than the element commit asked for
     kCtrWalkSteps = 16,  // group walk: node steps (one node for the wavefront's sources)
     kCtrWalkSlots = 17,  // group walk: expanded nodes given a table slot
-    kCounters = 18
+    kCtrWalkBytes = 18,  // group walk: bytes of running sums and child entries stored in the tables
+    kCounters = 19
 };
 
 struct Status {
@@ -194,6 +195,7 @@ struct rl_bh_ctx {
     uint64_t last_slow_descents = 0; // descents handed to the per-lane general traversal
     uint64_t last_walk_steps = 0;    // group walk: node steps and table slots of the last search
     uint64_t last_walk_slots = 0;
+    uint64_t last_walk_bytes = 0;
 
     // persistent scratch, sized at init or with slots_cap (never inside an update)
     uint64_t* wide = nullptr;        // n + 1: offsets of the active sources

@@ -777,6 +777,7 @@ int find_targets(rl_bh_ctx* c, uint32_t step) {
     c->last_lane_sources = (c->cfg.flags & RL_FLAG_EXACT_ORDER) != 0 || c->tree.depth == 0 ? h[kCtrRedo] : h[kCtrOverflow];
     c->last_walk_steps = h[kCtrWalkSteps];
     c->last_walk_slots = h[kCtrWalkSlots];
+    c->last_walk_bytes = h[kCtrWalkBytes];
     c->searched_valid = true;
     return RL_OK;
 }

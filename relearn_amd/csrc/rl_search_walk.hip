@@ -319,7 +319,7 @@ template <bool kMu0, bool kCount>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) k_walk(WalkArgs a) {
     __shared__ WalkShared sh;
     // kCount: child records the wave examined (children of the roots and of expanded nodes), table slots used
-    __shared__ unsigned long long cnt_steps, cnt_slots;
+    __shared__ unsigned long long cnt_steps, cnt_slots, cnt_bytes;
     if (*a.n_active < a.run_from) {
         return;   // the wave-per-source kernel takes this update
     }
@@ -337,6 +337,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
     if (kCount && ln == 0) {
         cnt_steps = 0;
         cnt_slots = 0;
+        cnt_bytes = 0;
     }
     uint64_t w_next = 0, w_lim = 0;
     while (true) {
@@ -522,6 +523,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                         tchild[ln] = v == 0xFFFFU ? kNoGrandSlots : v;
                     }
                     if (kCount && ln == 0) {
+                        atomicAdd(&cnt_bytes, 32ULL);
+                    }
+                    if (kCount && ln == 0) {
                         atomicAdd(&cnt_slots, static_cast<unsigned long long>(nslots));
                     }
                     break;
@@ -540,6 +544,9 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     if (__builtin_amdgcn_inverse_ballot_w64(0xFFULL)) {
                         const uint32_t v = sh.kids[sp][ln];
                         tchild[static_cast<uint64_t>(cslot) * 8 + ln] = v == 0xFFFFU ? kNoGrandSlots : v;
+                    }
+                    if (kCount && ln == 0) {
+                        atomicAdd(&cnt_bytes, static_cast<unsigned long long>(8 * __popcll(top_mask) + 32));
                     }
                 }
                 sp--;
@@ -715,6 +722,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         if (kCount) {
             atomicAdd(&s.counters[kCtrWalkSteps], cnt_steps);
             atomicAdd(&s.counters[kCtrWalkSlots], cnt_slots);
+            atomicAdd(&s.counters[kCtrWalkBytes], cnt_bytes);
         }
     }
 }

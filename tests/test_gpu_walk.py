@@ -129,7 +129,7 @@ def test_walk_counting_instantiation_same_picks(oracle):
     ref, nreq, st, tgt, tests, cnt = _pair(oracle, net, seed=8, step=3, sigma=750.0)
     _, nreq_c, st_c, tgt_c, tests_c, cnt_c = _pair(oracle, net, seed=8, step=3, sigma=750.0,
                                                    flags=RL_FLAG_COUNT_WALK | RL_FLAG_WALK_ALWAYS)
-    assert st["walk_steps"] == 0 and st["walk_slots"] == 0
+    assert st["walk_steps"] == 0 and st["walk_slots"] == 0 and st["walk_table_bytes"] == 0
     assert np.array_equal(tgt, tgt_c) and np.array_equal(tests, tests_c) and np.array_equal(cnt, cnt_c)
     assert nreq == nreq_c
     _say("  steps", st_c["walk_steps"], "slots", st_c["walk_slots"], "lane sources", st_c["lane_sources"])
@@ -138,6 +138,9 @@ def test_walk_counting_instantiation_same_picks(oracle):
     # descents below the picked level-1 node add at most 8 tests per level
     assert st_c["walk_steps"] > 0 and st_c["walk_slots"] > 0
     assert st_c["walk_slots"] <= st_c["walk_steps"]
+    # stored: at most the 64 running sums and one 32-byte child-entry row per slot, at least the root's row
+    assert 32 <= st_c["walk_table_bytes"] <= (64 * 8 + 32) * st_c["walk_slots"]
+    assert st_c["walk_table_bytes"] % 8 == 0
     assert st_c["walk_steps"] <= st_c["node_tests"] <= 64 * st_c["walk_steps"] + 8 * 32 * st_c["searches"]
 
 
