@@ -450,8 +450,9 @@ def read_counters(kernel, precision_label):
     path = os.path.join(ROOT, "profiles", f"{kernel}_{precision_label}_16M_counters.md")
     if not os.path.exists(path):
         return None
-    out = {"resource": "instruction issue (VALU and SALU)", "profile": os.path.relpath(path, ROOT)}
-    keys = {"VALU busy": "valu_busy", "SALU busy": "salu_busy", "L2 hit rate": "l2_hit_rate",
+    out = {"profile": os.path.relpath(path, ROOT)}
+    keys = {"limiter": "resource", "VALU busy": "valu_busy", "SALU busy": "salu_busy",
+            "texture data busy": "td_busy", "L2 hit rate": "l2_hit_rate",
             "VALU instructions per active source":
             "valu_insts_per_source", "SALU instructions per active source": "salu_insts_per_source",
             "commit": "profiled_commit"}
