@@ -652,6 +652,11 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                         const uint32_t y = nfirst + ncur;
                         const uint2 mt = G.meta[y];
                         const uint4 rq = G.rec32[y];
+                        // the child's table entry, read with its records (it is used if the lane
+                        // expands the child), so that an expanded child costs one more memory round
+                        // trip (its stored sum) rather than two
+                        const uint32_t cs = below ? kNone : tchild[static_cast<uint64_t>(nslot) * 8 + ncur];
+                        __asm__ volatile("" ::"v"(mt.x), "v"(mt.y), "v"(rq.x), "v"(rq.y), "v"(rq.z), "v"(rq.w), "v"(cs));
                         exec++;
                         const NodeEval e = eval_walk<kMu0>(s, G, V, y, mt, rq, true);
                         if (e.st == kAccept) {
@@ -662,7 +667,6 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                             }
                             ncum = cn;
                         } else if (e.st == kExpand) {
-                            const uint32_t cs = below ? kNone : tchild[static_cast<uint64_t>(nslot) * 8 + ncur];
                             if (cs == kNoGrandSlots) {
                                 // no child of this child was expanded: its children, in list order
                                 rfirst = nfirst;
