@@ -69,6 +69,10 @@ struct Tree {
     double* box = nullptr;              // 6 per node: lo xyz, hi xyz
     double4* rec64[2] = { nullptr, nullptr };
     uint4* rec32[2] = { nullptr, nullptr };
+    // FP32 path, group walk: per node rec32 and meta side by side (2 x 16 B, the meta pair in the low half
+    // of the second), written with them (links and leaf records at the build, counts and inner records by
+    // every leaf fill and upward pass), so that one load brings both; kNodePad spare nodes
+    uint4* wrec[2] = { nullptr, nullptr };
     uint2* meta[2] = { nullptr, nullptr };
     uint32_t* node_net = nullptr;       // network of each node (the roots are nodes 0..networks-1)
 };

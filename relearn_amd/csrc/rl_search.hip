@@ -617,6 +617,7 @@ SearchArgs search_args(rl_bh_ctx* c) {
     for (int k = 0; k < 2; k++) {
         a.rec64[k] = c->tree.rec64[k];
         a.rec32[k] = c->tree.rec32[k];
+        a.wrec[k] = c->tree.wrec[k];
         a.meta[k] = c->tree.meta[k];
     }
     a.theta = c->cfg.theta;

@@ -22,6 +22,7 @@ struct SearchArgs {
     const uint64_t* slot_off;
     const double4* rec64[2];
     const uint4* rec32[2];
+    const uint4* wrec[2];   // FP32 path: rec32 and meta side by side, two uint4 per node
     const uint2* meta[2];
     double theta;
     double scale64;      // -1 / sigma^2

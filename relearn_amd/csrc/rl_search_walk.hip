@@ -80,8 +80,7 @@ struct WalkArgs {
     uint64_t run_from;           // the kernel returns at once when *n_active < run_from
 };
 
-constexpr uint32_t kBlockLevels = 24;            // LDS blocks of staged child records (a ring over the levels)
-static_assert(kWalkDepth < 2 * kBlockLevels, "a stack level maps to its block with one subtraction");
+constexpr uint32_t kBlockLevels = 18;            // LDS blocks of staged child records (a ring over the levels)
 
 // The stack below the top entry, and the records of the children of the stack's nodes: each push
 // copies the node's children from the tree into the block of its level (load-to-LDS instructions), so
@@ -92,12 +91,11 @@ struct WalkShared {
     uint4 ent[kWalkDepth];   // (first child | next child to visit << 28, slot, lane mask low, high)
     uint16_t kids[kWalkDepth][8];   // entries of the children of each level's node, until it is left
     uint32_t owner[kBlockLevels];
-    uint4 rec[kBlockLevels][8];
-    uint32_t meta[kBlockLevels][16];
+    uint4 rec[kBlockLevels][16];   // per child: fixed-point record, then the count/link pair (low half)
 };
 
 __device__ __forceinline__ uint32_t block_of(uint32_t level) {
-    return level >= kBlockLevels ? level - kBlockLevels : level;
+    return level % kBlockLevels;
 }
 
 __device__ __forceinline__ uint32_t lane_id() {
@@ -121,10 +119,17 @@ struct WalkLane {
 
 struct Group {
     float k2q, inv_q;
-    const uint2* meta;
-    const uint4* rec32;
+    const uint4* wrec;   // per node: fixed-point record, then the count/link pair (low half)
     const double4* rec64;
 };
+
+__device__ __forceinline__ uint4 node_rec(const Group& G, uint32_t y) {
+    return G.wrec[2 * static_cast<uint64_t>(y)];
+}
+
+__device__ __forceinline__ uint2 node_meta(const Group& G, uint32_t y) {
+    return *reinterpret_cast<const uint2*>(G.wrec + 2 * static_cast<uint64_t>(y) + 1);
+}
 
 typedef __attribute__((address_space(3))) void LdsVoid;
 
@@ -132,18 +137,15 @@ typedef __attribute__((address_space(3))) void LdsVoid;
 constexpr int kWaitVm0 = 0x0F70;
 
 // Copies the records of the 8 nodes from the first child of the node at stack level `level` on into
-// its LDS block (lanes 0-7 a 16-byte position record each, lanes 0-15 one word of a count/link record
-// each) and waits for them.  Records past the node's last child are copied too and never read: the
-// node arrays end with kNodePad spare entries, so the copy stays inside them for every node.
+// its LDS block (lanes 0-15 a 16-byte half of a node's 32-byte record each) and waits for them.
+// Records past the node's last child are copied too and never read: the node arrays end with
+// kNodePad spare entries, so the copy stays inside them for every node.
 __device__ __forceinline__ void stage_children(WalkShared& sh, const Group& G, uint32_t level, uint32_t first,
                                                uint32_t ln) {
     const uint32_t b = block_of(level);
-    if (__builtin_amdgcn_inverse_ballot_w64(0xFFULL)) {
-        __builtin_amdgcn_global_load_lds((void*)(G.rec32 + first + ln), (LdsVoid*)&sh.rec[b][0], 16, 0, 0);
-    }
     if (__builtin_amdgcn_inverse_ballot_w64(0xFFFFULL)) {
-        __builtin_amdgcn_global_load_lds((void*)(reinterpret_cast<const uint32_t*>(G.meta) + 2 * first + ln),
-                                         (LdsVoid*)&sh.meta[b][0], 4, 0, 0);
+        __builtin_amdgcn_global_load_lds((void*)(G.wrec + 2 * static_cast<uint64_t>(first) + ln),
+                                         (LdsVoid*)&sh.rec[b][0], 16, 0, 0);
     }
     if (__builtin_amdgcn_inverse_ballot_w64(1ULL)) {
         sh.owner[b] = level;
@@ -255,7 +257,7 @@ __device__ int descend(const SearchArgs& a, const Group& G, const WalkLane& W, u
         if (node >= a.m) {
             return kDescFailed;
         }
-        const uint2 mn = G.meta[node];
+        const uint2 mn = node_meta(G, node);
         const uint32_t nc = mn.y >> kLinkBits;
         if (nc == 0) {
             node = mn.y & kLinkMask;  // leaf: the target neuron
@@ -271,7 +273,7 @@ __device__ int descend(const SearchArgs& a, const Group& G, const WalkLane& W, u
         uint32_t accepted = 0;
         for (uint32_t k = nc; k-- > 0;) {
             const uint32_t y = first + k;
-            const NodeEval e = eval_walk<kMu0>(a, G, W, y, G.meta[y], G.rec32[y], true);
+            const NodeEval e = eval_walk<kMu0>(a, G, W, y, node_meta(G, y), node_rec(G, y), true);
             expand = expand || e.st == kExpand || e.st == kUndecided;
             if (e.st == kAccept) {
                 tot += e.p;
@@ -295,7 +297,7 @@ __device__ int descend(const SearchArgs& a, const Group& G, const WalkLane& W, u
         int sel = -1;
         for (uint32_t k = nc; k-- > 0;) {
             const uint32_t y = first + k;
-            const NodeEval e = eval_walk<kMu0>(a, G, W, y, G.meta[y], G.rec32[y], true);
+            const NodeEval e = eval_walk<kMu0>(a, G, W, y, node_meta(G, y), node_rec(G, y), true);
             exec++;
             if (e.st == kAccept) {
                 const double cn = cum + e.p;
@@ -379,8 +381,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
         const uint64_t left = a.seg_count[seg] - kGroup * gi;
         const uint32_t gsize = static_cast<uint32_t>(std::min<uint64_t>(left, kGroup));
         Group G;
-        G.meta = s.meta[type];
-        G.rec32 = s.rec32[type];
+        G.wrec = s.wrec[type];
         G.rec64 = s.rec64[type];
         G.k2q = s.net_k2q[net];
         G.inv_q = s.net_invq[net];
@@ -396,7 +397,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
 
         // ---- the union walk: level-1 sums of all lanes ----
         // wave-uniform: the root values seed the traversal state, which must stay in scalar registers
-        const uint2 mr0 = G.meta[net];
+        const uint2 mr0 = node_meta(G, net);
         const uint2 mr = make_uint2(uniform_u32(mr0.x), uniform_u32(mr0.y));
         double cum = 0.0;
         uint32_t tests = 0;
@@ -426,8 +427,8 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     const uint32_t bl = block_of(sp);
                     while (exm == 0 && top_cur != 0) {
                         top_cur--;
-                        const uint2 mraw = *reinterpret_cast<const uint2*>(&sh.meta[bl][2 * top_cur]);
-                        const uint4 rq = sh.rec[bl][top_cur];
+                        const uint2 mraw = *reinterpret_cast<const uint2*>(&sh.rec[bl][2 * top_cur + 1]);
+                        const uint4 rq = sh.rec[bl][2 * top_cur];
                         // both records are read together, before the count decides anything
                         __asm__ volatile("" ::"v"(mraw.x), "v"(mraw.y), "v"(rq.x), "v"(rq.y), "v"(rq.z), "v"(rq.w));
                         const uint32_t cnt = uniform_u32(mraw.x);
@@ -495,7 +496,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     }
                     const uint32_t ns = nslots;
                     nslots = uniform_u32(ns + 1);
-                    const uint32_t link = uniform_u32(sh.meta[block_of(sp)][2 * top_cur + 1]);
+                    const uint32_t link = uniform_u32(sh.rec[block_of(sp)][2 * top_cur + 1].y);
                     if (__builtin_amdgcn_inverse_ballot_w64(1ULL)) {
                         sh.ent[sp] = make_uint4(top_first | (top_cur << kLinkBits), top_slot,
                                                 static_cast<uint32_t>(top_mask), static_cast<uint32_t>(top_mask >> 32U));
@@ -650,8 +651,8 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     } else {
                         ncur--;
                         const uint32_t y = nfirst + ncur;
-                        const uint2 mt = G.meta[y];
-                        const uint4 rq = G.rec32[y];
+                        const uint2 mt = node_meta(G, y);
+                        const uint4 rq = node_rec(G, y);
                         // the child's table entry, read with its records (it is used if the lane
                         // expands the child), so that an expanded child costs one more memory round
                         // trip (its stored sum) rather than two
