@@ -57,8 +57,9 @@ struct Status {
 //                  units as float bits, -infinity for a leaf)   FP32 path, 16 B
 //   meta[s][i]  = (vacant count, link | nchild << 28)  8 B
 // link is the first child for inner nodes and the neuron id for leaves.  meta and rec32 hold
-// kNodePad more entries than nodes: the group walk copies the 8 records from a node's first child on,
-// whatever the node's child count, and the last node's first child may be among the last nodes.
+// kNodePad more entries than nodes: when the group walk copies a node's children again (after a deeper
+// level took their LDS block) it copies the 8 records from the first child on, whatever the node's
+// child count, and the last node's first child may be among the last nodes.
 constexpr uint64_t kNodePad = 8;
 struct Tree {
     uint64_t m = 0;

@@ -136,14 +136,15 @@ typedef __attribute__((address_space(3))) void LdsVoid;
 // s_waitcnt immediate: VM_CNT 0 (the staged records have landed in LDS), EXP_CNT and LGKM_CNT not waited for
 constexpr int kWaitVm0 = 0x0F70;
 
-// Copies the records of the 8 nodes from the first child of the node at stack level `level` on into
-// its LDS block (lanes 0-15 a 16-byte half of a node's 32-byte record each) and waits for them.
-// Records past the node's last child are copied too and never read: the node arrays end with
+// Copies the records of the children [first, first + nc) of the node at stack level `level` into its
+// LDS block (lanes < 2 nc a 16-byte half of a node's 32-byte record each) and waits for them.  A copy
+// again after a deeper level took the block passes nc = 8 (the stack keeps no child count): records
+// past the node's last child are then copied too and never read, and the node arrays end with
 // kNodePad spare entries, so the copy stays inside them for every node.
 __device__ __forceinline__ void stage_children(WalkShared& sh, const Group& G, uint32_t level, uint32_t first,
-                                               uint32_t ln) {
+                                               uint32_t nc, uint32_t ln) {
     const uint32_t b = block_of(level);
-    if (__builtin_amdgcn_inverse_ballot_w64(0xFFFFULL)) {
+    if (ln < 2U * nc) {
         __builtin_amdgcn_global_load_lds((void*)(G.wrec + 2 * static_cast<uint64_t>(first) + ln),
                                          (LdsVoid*)&sh.rec[b][0], 16, 0, 0);
     }
@@ -413,7 +414,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
             if (ln < kBlockLevels) {
                 sh.owner[ln] = kNone;
             }
-            stage_children(sh, G, 0, root_first, ln);
+            stage_children(sh, G, 0, root_first, root_nc, ln);
             // a lane's level-1 tests: the root's children and the children of every node it expands
             tests = active ? root_nc : 0U;
             if (kCount && ln == 0) {
@@ -512,7 +513,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     if (kCount && ln == 0) {
                         atomicAdd(&cnt_steps, static_cast<unsigned long long>(top_cur));
                     }
-                    stage_children(sh, G, sp, top_first, ln);
+                    stage_children(sh, G, sp, top_first, top_cur, ln);
                     continue;
                 }
                 // the walk leaves the subtree of this slot's node
@@ -562,7 +563,7 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                     sh.kids[sp][top_cur] = static_cast<uint16_t>(grand ? cslot : 0xFFFFU);
                 }
                 if (uniform_u32(sh.owner[block_of(sp)]) != sp) {
-                    stage_children(sh, G, sp, top_first, ln);
+                    stage_children(sh, G, sp, top_first, 8U, ln);
                 }
             }
         }
