@@ -538,14 +538,16 @@ __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6
                 const uint32_t cslot = top_slot;
                 const bool grand = nslots != cslot + 1U;
                 if (grand) {
+                    // the table is written once and read back at most once, by the draws: its stores
+                    // bypass the caches' retention (non-temporal), which keeps L2 for the node records
                     if (__builtin_amdgcn_inverse_ballot_w64(top_mask)) {
-                        tcum[static_cast<uint64_t>(cslot) * kWave + ln] = cum;
+                        __builtin_nontemporal_store(cum, &tcum[static_cast<uint64_t>(cslot) * kWave + ln]);
                     }
                     // the entries of the node's children, in one store: written for expanded children
                     // only, and a lane reads only those of the children it expanded
                     if (__builtin_amdgcn_inverse_ballot_w64(0xFFULL)) {
                         const uint32_t v = sh.kids[sp][ln];
-                        tchild[static_cast<uint64_t>(cslot) * 8 + ln] = v == 0xFFFFU ? kNoGrandSlots : v;
+                        __builtin_nontemporal_store(v == 0xFFFFU ? kNoGrandSlots : v, &tchild[static_cast<uint64_t>(cslot) * 8 + ln]);
                     }
                     if (kCount && ln == 0) {
                         atomicAdd(&cnt_bytes, static_cast<unsigned long long>(8 * __popcll(top_mask) + 32));
