@@ -150,7 +150,6 @@ struct rl_bh_ctx {
     uint8_t* enabled = nullptr;
     uint32_t* vac[3] = { nullptr, nullptr, nullptr };  // axon, exc dendrite, inh dendrite
     uint32_t* conn[3] = { nullptr, nullptr, nullptr };
-    uint32_t* leaf_of = nullptr;    // neuron -> leaf node
     uint32_t* order = nullptr;      // neurons in Morton order (search order)
     rl::Tree tree;
     bool prepared = false;

@@ -231,7 +231,7 @@ __global__ void k_link(uint64_t m, uint64_t n, const uint64_t* __restrict__ keys
                        const double* __restrict__ pos, const double* __restrict__ net_box,
                        uint8_t* __restrict__ level, uint8_t* __restrict__ octant, uint32_t* __restrict__ parent,
                        double* __restrict__ nbox, uint32_t* __restrict__ link, uint8_t* __restrict__ isleaf,
-                       uint32_t* __restrict__ leaf_of, uint32_t* __restrict__ node_net, double4* __restrict__ rec0,
+                       uint32_t* __restrict__ node_net, double4* __restrict__ rec0,
                        double4* __restrict__ rec1) {
 #pragma clang fp contract(off)
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
@@ -282,7 +282,6 @@ __global__ void k_link(uint64_t m, uint64_t n, const uint64_t* __restrict__ keys
     if (leaf) {
         const uint32_t neuron = sorted[p];
         link[i] = neuron;
-        leaf_of[neuron] = static_cast<uint32_t>(i);
         const double4 r = make_double4(pos[neuron], pos[n + neuron], pos[2 * n + neuron], side);
         rec0[i] = r;
         rec1[i] = r;
@@ -349,24 +348,28 @@ __global__ void k_pack_meta(uint64_t m, uint64_t n, const uint32_t* __restrict__
 // ------------------------------------------------------------------ upward pass
 
 // Vacant dendrite counts of the leaves, into the count/link records and, on the FP32 path, into the
-// group walk's node records (w0/w1, or null).
-__global__ void k_leaf_fill(uint64_t n, const uint32_t* __restrict__ leaf_of, const uint8_t* __restrict__ enabled,
-                            const uint32_t* __restrict__ vexc, const uint32_t* __restrict__ vinh,
-                            uint2* __restrict__ meta0, uint2* __restrict__ meta1, uint4* __restrict__ w0,
-                            uint4* __restrict__ w1) {
+// group walk's node records (w0/w1, or null).  One thread per node, so that the writes of neighbouring
+// leaves are neighbouring (the reads of the neurons' flags and counts are the scattered side).
+__global__ void k_leaf_fill(uint64_t m, const uint8_t* __restrict__ enabled, const uint32_t* __restrict__ vexc,
+                            const uint32_t* __restrict__ vinh, uint2* __restrict__ meta0, uint2* __restrict__ meta1,
+                            uint4* __restrict__ w0, uint4* __restrict__ w1) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i >= n) {
+    if (i >= m) {
         return;
     }
-    const uint32_t node = leaf_of[i];
-    const bool on = enabled[i] != 0;
-    const uint32_t ce = on ? vexc[i] : 0U;
-    const uint32_t ci = on ? vinh[i] : 0U;
-    meta0[node].x = ce;
-    meta1[node].x = ci;
+    const uint32_t packed = meta0[i].y;
+    if ((packed >> kLinkBits) != 0) {
+        return;   // inner node: the upward pass
+    }
+    const uint32_t nrn = packed & kLinkMask;   // a leaf links its neuron
+    const bool on = enabled[nrn] != 0;
+    const uint32_t ce = on ? vexc[nrn] : 0U;
+    const uint32_t ci = on ? vinh[nrn] : 0U;
+    meta0[i].x = ce;
+    meta1[i].x = ci;
     if (w0 != nullptr) {
-        w0[2 * static_cast<uint64_t>(node) + 1].x = ce;
-        w1[2 * static_cast<uint64_t>(node) + 1].x = ci;
+        w0[2 * i + 1].x = ce;
+        w1[2 * i + 1].x = ci;
     }
 }
 
@@ -741,7 +744,7 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
             (status = dalloc(c, &t.rec64[0], m)) || (status = dalloc(c, &t.rec64[1], m)) ||
             (status = dalloc(c, &t.meta[0], m + kNodePad)) || (status = dalloc(c, &t.meta[1], m + kNodePad)) ||
             (status = dalloc(c, &parent, m)) || (status = dalloc(c, &link, m)) || (status = dalloc(c, &nchild, m)) ||
-            (status = dalloc(c, &isleaf, m)) || (status = dalloc(c, &c->leaf_of, n)) || (status = dalloc(c, &t.node_net, m))) {
+            (status = dalloc(c, &isleaf, m)) || (status = dalloc(c, &t.node_net, m))) {
             break;
         }
         if (c->cfg.precision == RL_PREC_FP32_MIXED) {
@@ -753,7 +756,7 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         }
         RL_STEP(hipMemsetAsync(nchild, 0, m * sizeof(uint32_t), s));
         k_link<<<grid_for(m), kBlock, 0, s>>>(m, n, nkeys2, ls, lcp, khi2, klo2, idx2, netp, c->pos, c->net_box, t.level,
-                                               t.octant, parent, t.box, link, isleaf, c->leaf_of, t.node_net, t.rec64[0],
+                                               t.octant, parent, t.box, link, isleaf, t.node_net, t.rec64[0],
                                                t.rec64[1]);
         RL_STEP(hipGetLastError());
         k_children<<<grid_for(m), kBlock, 0, s>>>(m, ls, t.level, parent, link, nchild);
@@ -790,8 +793,8 @@ int prepare_tree(rl_bh_ctx* c) {
     }
     Tree& t = c->tree;
     hipStream_t s = c->stream;
-    k_leaf_fill<<<grid_for(c->n), kBlock, 0, s>>>(c->n, c->leaf_of, c->enabled, c->vac[1], c->vac[2], t.meta[0],
-                                                   t.meta[1], t.wrec[0], t.wrec[1]);
+    k_leaf_fill<<<grid_for(t.m), kBlock, 0, s>>>(t.m, c->enabled, c->vac[1], c->vac[2], t.meta[0], t.meta[1],
+                                                  t.wrec[0], t.wrec[1]);
     RL_HIP(c, hipGetLastError());
     const QuantArgs qa = quant_args(c);
     for (int l = static_cast<int>(t.depth) - 1; l >= 0; l--) {
