@@ -75,6 +75,7 @@ void free_all(rl_bh_ctx* c) {
         dfree(&c->vac[t]);
         dfree(&c->conn[t]);
     }
+    dfree(&c->leaf_of);
     dfree(&c->order);
     dfree(&c->slot_off);
     dfree(&c->slot_target);

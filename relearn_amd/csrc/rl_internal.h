@@ -56,10 +56,10 @@ struct Status {
 //   rec32[s][i] = (centroid x, y, z as 31-bit fixed point, squared max edge in squared fixed-point
 //                  units as float bits, -infinity for a leaf)   FP32 path, 16 B
 //   meta[s][i]  = (vacant count, link | nchild << 28)  8 B
-// link is the first child for inner nodes and the neuron id for leaves.  meta and rec32 hold
-// kNodePad more entries than nodes: when the group walk copies a node's children again (after a deeper
-// level took their LDS block) it copies the 8 records from the first child on, whatever the node's
-// child count, and the last node's first child may be among the last nodes.
+// link is the first child for inner nodes and the neuron id for leaves.  The group walk's records
+// (wrec) hold kNodePad more nodes than the tree: when the walk copies a node's children again (after a
+// deeper level took their LDS block) it copies the 8 records from the first child on, whatever the
+// node's child count, and the last node's first child may be among the last nodes.
 constexpr uint64_t kNodePad = 8;
 struct Tree {
     uint64_t m = 0;
@@ -71,8 +71,8 @@ struct Tree {
     double4* rec64[2] = { nullptr, nullptr };
     uint4* rec32[2] = { nullptr, nullptr };
     // FP32 path, group walk: per node rec32 and meta side by side (2 x 16 B, the meta pair in the low half
-    // of the second), written with them (links and leaf records at the build, counts and inner records by
-    // every leaf fill and upward pass), so that one load brings both; kNodePad spare nodes
+    // of the second), packed from them for every update the walk searches, so that one load brings
+    // both; kNodePad spare nodes
     uint4* wrec[2] = { nullptr, nullptr };
     uint2* meta[2] = { nullptr, nullptr };
     uint32_t* node_net = nullptr;       // network of each node (the roots are nodes 0..networks-1)
@@ -150,6 +150,7 @@ struct rl_bh_ctx {
     uint8_t* enabled = nullptr;
     uint32_t* vac[3] = { nullptr, nullptr, nullptr };  // axon, exc dendrite, inh dendrite
     uint32_t* conn[3] = { nullptr, nullptr, nullptr };
+    uint32_t* leaf_of = nullptr;    // neuron -> leaf node
     uint32_t* order = nullptr;      // neurons in Morton order (search order)
     rl::Tree tree;
     bool prepared = false;

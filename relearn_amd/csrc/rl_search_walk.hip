@@ -318,6 +318,29 @@ __device__ int descend(const SearchArgs& a, const Group& G, const WalkLane& W, u
     return kDescFailed;
 }
 
+// The walk's node records, packed for an update that the walk searches: per node and dendrite type the
+// fixed-point record and the count/link pair side by side (32 B), so that one load brings both (a push
+// copies a node's children in one instruction, a draw's read of a node touches one cache line).  One
+// thread per node, both types; returns at once when the wave-per-source kernel takes the update.
+__global__ void k_pack_walk(uint64_t m, const uint64_t* __restrict__ n_active, uint64_t run_from,
+                            const uint2* __restrict__ meta0, const uint2* __restrict__ meta1,
+                            const uint4* __restrict__ q0, const uint4* __restrict__ q1, uint4* __restrict__ w0,
+                            uint4* __restrict__ w1) {
+    if (*n_active < run_from) {
+        return;
+    }
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= m) {
+        return;
+    }
+    const uint2 a = meta0[i];
+    const uint2 b = meta1[i];
+    w0[2 * i] = q0[i];
+    w0[2 * i + 1] = make_uint4(a.x, a.y, 0U, 0U);
+    w1[2 * i] = q1[i];
+    w1[2 * i + 1] = make_uint4(b.x, b.y, 0U, 0U);
+}
+
 template <bool kMu0, bool kCount>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(6, 6))) k_walk(WalkArgs a) {
     __shared__ WalkShared sh;
@@ -842,6 +865,10 @@ int launch_search_walk(rl_bh_ctx* c, const SearchArgs& sa) {
     a.n_active = n_active;
     a.run_from = walk_min_sources(c);
     const unsigned blocks = std::max<uint32_t>(1U, std::min<uint32_t>(c->coop_blocks, c->walk_blocks_cap));
+    k_pack_walk<<<grid_for(c->tree.m), kBlock, 0, s>>>(c->tree.m, n_active, a.run_from, c->tree.meta[0],
+                                                        c->tree.meta[1], c->tree.rec32[0], c->tree.rec32[1],
+                                                        c->tree.wrec[0], c->tree.wrec[1]);
+    RL_HIP(c, hipGetLastError());
     const bool timed = !c->capturing;
     if (timed) RL_HIP(c, hipEventRecord(c->ev2, s));
     if (a.run_from > 0) {

@@ -231,7 +231,7 @@ __global__ void k_link(uint64_t m, uint64_t n, const uint64_t* __restrict__ keys
                        const double* __restrict__ pos, const double* __restrict__ net_box,
                        uint8_t* __restrict__ level, uint8_t* __restrict__ octant, uint32_t* __restrict__ parent,
                        double* __restrict__ nbox, uint32_t* __restrict__ link, uint8_t* __restrict__ isleaf,
-                       uint32_t* __restrict__ node_net, double4* __restrict__ rec0,
+                       uint32_t* __restrict__ leaf_of, uint32_t* __restrict__ node_net, double4* __restrict__ rec0,
                        double4* __restrict__ rec1) {
 #pragma clang fp contract(off)
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
@@ -282,6 +282,7 @@ __global__ void k_link(uint64_t m, uint64_t n, const uint64_t* __restrict__ keys
     if (leaf) {
         const uint32_t neuron = sorted[p];
         link[i] = neuron;
+        leaf_of[neuron] = static_cast<uint32_t>(i);
         const double4 r = make_double4(pos[neuron], pos[n + neuron], pos[2 * n + neuron], side);
         rec0[i] = r;
         rec1[i] = r;
@@ -347,30 +348,17 @@ __global__ void k_pack_meta(uint64_t m, uint64_t n, const uint32_t* __restrict__
 
 // ------------------------------------------------------------------ upward pass
 
-// Vacant dendrite counts of the leaves, into the count/link records and, on the FP32 path, into the
-// group walk's node records (w0/w1, or null).  One thread per node, so that the writes of neighbouring
-// leaves are neighbouring (the reads of the neurons' flags and counts are the scattered side).
-__global__ void k_leaf_fill(uint64_t m, const uint8_t* __restrict__ enabled, const uint32_t* __restrict__ vexc,
-                            const uint32_t* __restrict__ vinh, uint2* __restrict__ meta0, uint2* __restrict__ meta1,
-                            uint4* __restrict__ w0, uint4* __restrict__ w1) {
+__global__ void k_leaf_fill(uint64_t n, const uint32_t* __restrict__ leaf_of, const uint8_t* __restrict__ enabled,
+                            const uint32_t* __restrict__ vexc, const uint32_t* __restrict__ vinh,
+                            uint2* __restrict__ meta0, uint2* __restrict__ meta1) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (i >= m) {
+    if (i >= n) {
         return;
     }
-    const uint32_t packed = meta0[i].y;
-    if ((packed >> kLinkBits) != 0) {
-        return;   // inner node: the upward pass
-    }
-    const uint32_t nrn = packed & kLinkMask;   // a leaf links its neuron
-    const bool on = enabled[nrn] != 0;
-    const uint32_t ce = on ? vexc[nrn] : 0U;
-    const uint32_t ci = on ? vinh[nrn] : 0U;
-    meta0[i].x = ce;
-    meta1[i].x = ci;
-    if (w0 != nullptr) {
-        w0[2 * i + 1].x = ce;
-        w1[2 * i + 1].x = ci;
-    }
+    const uint32_t node = leaf_of[i];
+    const bool on = enabled[i] != 0;
+    meta0[node].x = on ? vexc[i] : 0U;
+    meta1[node].x = on ? vinh[i] : 0U;
 }
 
 __device__ __forceinline__ uint32_t quantize(double v, double lo, double scale) {
@@ -400,8 +388,7 @@ __device__ __forceinline__ uint32_t quant_side2(double side, double scale) {
 __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __restrict__ nbox,
                                double4* __restrict__ rec0, double4* __restrict__ rec1, uint2* __restrict__ meta0,
                                uint2* __restrict__ meta1, uint4* __restrict__ q0, uint4* __restrict__ q1,
-                               uint4* __restrict__ w0, uint4* __restrict__ w1, const uint32_t* __restrict__ node_net,
-                               QuantArgs qa) {
+                               const uint32_t* __restrict__ node_net, QuantArgs qa) {
 #pragma clang fp contract(off)
     const uint64_t t = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     const uint64_t i = first + (t >> 1U);
@@ -463,21 +450,13 @@ __global__ void k_upward_level(uint64_t first, uint64_t last, const double* __re
         const uint32_t g = node_net[i];
         const double* lo = qa.net_box + 6 * static_cast<uint64_t>(g);
         const double sc = qa.net_scale[g];
-        const uint4 v = make_uint4(quantize(r.x, lo[0], sc), quantize(r.y, lo[1], sc), quantize(r.z, lo[2], sc),
-                                   quant_side2(r.w, sc));
-        q[i] = v;
-        // the group walk's record of the node: the fixed-point record and the count (its link is set
-        // at the build)
-        uint4* w = s == 0 ? w0 : w1;
-        w[2 * i] = v;
-        w[2 * i + 1].x = total;
+        q[i] = make_uint4(quantize(r.x, lo[0], sc), quantize(r.y, lo[1], sc), quantize(r.z, lo[2], sc),
+                          quant_side2(r.w, sc));
     }
 }
 
 __global__ void k_quant_leaves(uint64_t m, const double4* __restrict__ rec, const uint32_t* __restrict__ node_net,
-                               const uint2* __restrict__ meta0, const uint2* __restrict__ meta1,
-                               uint4* __restrict__ q0, uint4* __restrict__ q1, uint4* __restrict__ w0,
-                               uint4* __restrict__ w1, QuantArgs qa) {
+                               uint4* __restrict__ q0, uint4* __restrict__ q1, QuantArgs qa) {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= m) {
         return;
@@ -493,13 +472,6 @@ __global__ void k_quant_leaves(uint64_t m, const double4* __restrict__ rec, cons
                                0xFF800000U);
     q0[i] = v;
     q1[i] = v;
-    // the group walk's records (the fixed-point record, then the count/link pair): the links are fixed
-    // here, the counts and inner records are rewritten by every leaf fill and upward pass
-    w0[2 * i] = v;
-    w1[2 * i] = v;
-    const uint2 m0 = meta0[i], m1 = meta1[i];
-    w0[2 * i + 1] = make_uint4(m0.x, m0.y, 0U, 0U);
-    w1[2 * i + 1] = make_uint4(m1.x, m1.y, 0U, 0U);
 }
 
 __global__ void k_quant_positions(uint64_t n, const double* __restrict__ pos, const uint32_t* __restrict__ net_of,
@@ -742,13 +714,13 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         t.level_start.assign(hls.begin(), hls.begin() + depth + 2);
         if ((status = dalloc(c, &t.level, m)) || (status = dalloc(c, &t.octant, m)) || (status = dalloc(c, &t.box, 6 * m)) ||
             (status = dalloc(c, &t.rec64[0], m)) || (status = dalloc(c, &t.rec64[1], m)) ||
-            (status = dalloc(c, &t.meta[0], m + kNodePad)) || (status = dalloc(c, &t.meta[1], m + kNodePad)) ||
+            (status = dalloc(c, &t.meta[0], m)) || (status = dalloc(c, &t.meta[1], m)) ||
             (status = dalloc(c, &parent, m)) || (status = dalloc(c, &link, m)) || (status = dalloc(c, &nchild, m)) ||
-            (status = dalloc(c, &isleaf, m)) || (status = dalloc(c, &t.node_net, m))) {
+            (status = dalloc(c, &isleaf, m)) || (status = dalloc(c, &c->leaf_of, n)) || (status = dalloc(c, &t.node_net, m))) {
             break;
         }
         if (c->cfg.precision == RL_PREC_FP32_MIXED) {
-            if ((status = dalloc(c, &t.rec32[0], m + kNodePad)) || (status = dalloc(c, &t.rec32[1], m + kNodePad)) ||
+            if ((status = dalloc(c, &t.rec32[0], m)) || (status = dalloc(c, &t.rec32[1], m)) ||
                 (status = dalloc(c, &t.wrec[0], 2 * (m + kNodePad))) || (status = dalloc(c, &t.wrec[1], 2 * (m + kNodePad))) ||
                 (status = dalloc(c, &c->qpos, 3 * n))) {
                 break;
@@ -756,7 +728,7 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         }
         RL_STEP(hipMemsetAsync(nchild, 0, m * sizeof(uint32_t), s));
         k_link<<<grid_for(m), kBlock, 0, s>>>(m, n, nkeys2, ls, lcp, khi2, klo2, idx2, netp, c->pos, c->net_box, t.level,
-                                               t.octant, parent, t.box, link, isleaf, t.node_net, t.rec64[0],
+                                               t.octant, parent, t.box, link, isleaf, c->leaf_of, t.node_net, t.rec64[0],
                                                t.rec64[1]);
         RL_STEP(hipGetLastError());
         k_children<<<grid_for(m), kBlock, 0, s>>>(m, ls, t.level, parent, link, nchild);
@@ -766,8 +738,7 @@ int build_tree(rl_bh_ctx* c, const double* h_xyz) {
         RL_STEP(hipGetLastError());
         if (c->cfg.precision == RL_PREC_FP32_MIXED) {
             const QuantArgs qa = quant_args(c);
-            k_quant_leaves<<<grid_for(m), kBlock, 0, s>>>(m, t.rec64[0], t.node_net, t.meta[0], t.meta[1], t.rec32[0],
-                                                          t.rec32[1], t.wrec[0], t.wrec[1], qa);
+            k_quant_leaves<<<grid_for(m), kBlock, 0, s>>>(m, t.rec64[0], t.node_net, t.rec32[0], t.rec32[1], qa);
             RL_STEP(hipGetLastError());
             k_quant_positions<<<grid_for(n), kBlock, 0, s>>>(n, c->pos, c->net_of, c->qpos, qa);
             RL_STEP(hipGetLastError());
@@ -793,8 +764,8 @@ int prepare_tree(rl_bh_ctx* c) {
     }
     Tree& t = c->tree;
     hipStream_t s = c->stream;
-    k_leaf_fill<<<grid_for(t.m), kBlock, 0, s>>>(t.m, c->enabled, c->vac[1], c->vac[2], t.meta[0], t.meta[1],
-                                                  t.wrec[0], t.wrec[1]);
+    k_leaf_fill<<<grid_for(c->n), kBlock, 0, s>>>(c->n, c->leaf_of, c->enabled, c->vac[1], c->vac[2], t.meta[0],
+                                                   t.meta[1]);
     RL_HIP(c, hipGetLastError());
     const QuantArgs qa = quant_args(c);
     for (int l = static_cast<int>(t.depth) - 1; l >= 0; l--) {
@@ -804,7 +775,7 @@ int prepare_tree(rl_bh_ctx* c) {
         }
         k_upward_level<<<grid_for(2 * (last - first)), kBlock, 0, s>>>(first, last, t.box, t.rec64[0], t.rec64[1],
                                                                         t.meta[0], t.meta[1], t.rec32[0], t.rec32[1],
-                                                                        t.wrec[0], t.wrec[1], t.node_net, qa);
+                                                                        t.node_net, qa);
         RL_HIP(c, hipGetLastError());
     }
     return RL_OK;
